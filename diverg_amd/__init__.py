@@ -1,0 +1,326 @@
+"""MI355X (gfx950) range-CRS Boolean sparse engine -- Python host binding.
+
+Mirrors DiVerG's operator interface for the distance-index path
+(range_sparse.hpp:430-477, 2283-2368; dindex.hpp:46-189):
+
+    ctx = Context()
+    c = ctx.range_spgemm(a, b)         # RangeSpGEMM::spgemm
+    c = ctx.range_spadd(a, b)          # RangeSpAdd::spadd
+    c = ctx.range_power(a, k)          # RangePower::power
+    d = ctx.create_distance_index(a, dlo, dup)   # util::create_distance_index
+    hits = ctx.query(d, i, j)          # CRSMatrix::operator()(i, j), batched
+
+Matrices are `RangeCRS` (host, numpy arrays) or `DeviceRCRS` (HBM-resident
+handles returned with location="device").  Every call goes through the C ABI
+in libdiverg_amd.so; there is no Python or CPU fallback -- a missing library
+raises `NativeMissing`.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from ._native import NativeMissing  # noqa: F401
+
+__all__ = ["RangeCRS", "DeviceRCRS", "Context", "DiVerGError", "generate_graph",
+           "graph_preset", "NativeMissing"]
+
+
+class DiVerGError(RuntimeError):
+    def __init__(self, status, message):
+        self.status = status
+        super().__init__(f"{N.STATUS.get(status, status)}: {message}")
+
+
+class RangeCRS:
+    """Host range-CRS matrix: CRSMatrix<RangeDynamic, bool, uint32_t, S>.
+
+    entries: uint32 array s0,e0,s1,e1,...; rowmap: uint32 or uint64 (S)
+    array of nrows+1 offsets into entries; nnz: number of set cells.
+    """
+
+    def __init__(self, ncols, entries, rowmap, nnz=None, nrows=None):
+        self.entries = np.ascontiguousarray(entries, dtype=np.uint32)
+        rowmap = np.asarray(rowmap)
+        if rowmap.dtype not in (np.uint32, np.uint64):
+            rowmap = rowmap.astype(np.uint64)
+        self.rowmap = np.ascontiguousarray(rowmap)
+        self.nrows = len(self.rowmap) - 1 if nrows is None else int(nrows)
+        self.ncols = int(ncols)
+        if nnz is None:  # crs_matrix.hpp:175-189
+            e = self.entries.astype(np.int64)
+            nnz = int((e[1::2] - e[0::2] + 1).sum())
+            if self.size_bytes == 4:
+                nnz %= 1 << 32
+        self.nnz = int(nnz)
+
+    @property
+    def size_bytes(self):
+        return self.rowmap.dtype.itemsize
+
+    def numRows(self):
+        return self.nrows
+
+    def numCols(self):
+        return self.ncols
+
+    def row(self, i):
+        b, e = int(self.rowmap[i]), int(self.rowmap[i + 1])
+        return self.entries[b:e]
+
+    def as_size(self, size_bytes):
+        dt = np.uint32 if size_bytes == 4 else np.uint64
+        return RangeCRS(self.ncols, self.entries, self.rowmap.astype(dt),
+                        self.nnz % (1 << 32) if size_bytes == 4 else self.nnz)
+
+    def _desc(self, keep):
+        d = N.dv_rcrs()
+        d.nrows = self.nrows
+        d.ncols = self.ncols
+        en = self.entries if len(self.entries) else np.zeros(2, np.uint32)
+        keep.extend([en, self.rowmap])
+        d.rowmap = self.rowmap.ctypes.data
+        d.entries = en.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.n_entries = len(self.entries)
+        d.nnz = self.nnz
+        d.size_bytes = self.size_bytes
+        d.location = N.DV_HOST
+        d.owner = None
+        return d
+
+    def __eq__(self, other):
+        return identical(self, other)
+
+    def __repr__(self):
+        return (f"RangeCRS({self.nrows}x{self.ncols}, ranges={len(self.entries) // 2}, "
+                f"nnz={self.nnz}, S=uint{8 * self.size_bytes})")
+
+
+def identical(a, b):
+    """check_identical of test_dindex.cpp:34-53: rows, cols, nnz, entries, rowmap."""
+    return (a.nrows == b.nrows and a.ncols == b.ncols and a.nnz == b.nnz
+            and np.array_equal(np.asarray(a.entries, np.uint64), np.asarray(b.entries, np.uint64))
+            and np.array_equal(np.asarray(a.rowmap, np.uint64), np.asarray(b.rowmap, np.uint64)))
+
+
+def _from_desc(d):
+    """Copy a library-allocated host descriptor into a RangeCRS."""
+    dt = np.uint32 if d.size_bytes == 4 else np.uint64
+    n = int(d.nrows)
+    rm = np.ctypeslib.as_array(C.cast(d.rowmap, C.POINTER(C.c_uint32 if d.size_bytes == 4
+                                                           else C.c_uint64)),
+                               shape=(n + 1,)).astype(dt, copy=True)
+    ne = int(d.n_entries)
+    en = (np.ctypeslib.as_array(d.entries, shape=(ne,)).copy() if ne
+          else np.zeros(0, np.uint32))
+    return RangeCRS(int(d.ncols), en, rm, nnz=int(d.nnz), nrows=n)
+
+
+class DeviceRCRS:
+    """An HBM-resident matrix owned by the library (freed with the handle)."""
+
+    def __init__(self, ctx, desc):
+        self._ctx = ctx
+        self.desc = desc
+
+    nrows = property(lambda self: int(self.desc.nrows))
+    ncols = property(lambda self: int(self.desc.ncols))
+    nnz = property(lambda self: int(self.desc.nnz))
+    n_entries = property(lambda self: int(self.desc.n_entries))
+    size_bytes = property(lambda self: int(self.desc.size_bytes))
+
+    def _desc(self, keep):
+        return self.desc
+
+    def to_host(self):
+        return self._ctx.to_host(self)
+
+    def free(self):
+        if self.desc is not None and self._ctx._h:
+            N.lib().dv_free(self._ctx._h, C.byref(self.desc))
+        self.desc = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def __repr__(self):
+        return (f"DeviceRCRS({self.nrows}x{self.ncols}, ranges={self.n_entries // 2}, "
+                f"nnz={self.nnz})")
+
+
+class BuildStats(dict):
+    pass
+
+
+class Context:
+    """A device context (dv_init / dv_finalize): replaces Kokkos
+    initialize/finalize (runtime.hpp:25-55)."""
+
+    def __init__(self, device=0):
+        self._h = None
+        h = C.c_void_p()
+        rc = N.lib().dv_init(device, C.byref(h))
+        if rc:
+            raise DiVerGError(rc, f"dv_init({device}) failed")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            N.lib().dv_finalize(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- helpers -------------------------------------------------------------
+    def _check(self, rc, what):
+        if rc:
+            msg = N.lib().dv_last_error(self._h)
+            raise DiVerGError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def _out(self, desc, location):
+        if location == "device":
+            return DeviceRCRS(self, desc)
+        m = _from_desc(desc)
+        N.lib().dv_free(self._h, C.byref(desc))
+        return m
+
+    @staticmethod
+    def _loc(location):
+        if location not in ("host", "device"):
+            raise ValueError("location must be 'host' or 'device'")
+        return N.DV_HOST if location == "host" else N.DV_DEVICE
+
+    # -- tuning --------------------------------------------------------------
+    def tuning(self):
+        t = N.dv_tuning()
+        self._check(N.lib().dv_get_tuning(self._h, C.byref(t)), "dv_get_tuning")
+        return {f: getattr(t, f) for f, _ in N.dv_tuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = N.dv_tuning()
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self._check(N.lib().dv_set_tuning(self._h, C.byref(t)), "dv_set_tuning")
+
+    # -- per-kernel timing ------------------------------------------------------
+    def profile(self, enable=True):
+        self._check(N.lib().dv_profile(self._h, 1 if enable else 0), "dv_profile")
+
+    def profile_read(self):
+        arr = (N.dv_kernel_time * len(N.KERNEL_KINDS))()
+        self._check(N.lib().dv_profile_read(self._h, arr), "dv_profile_read")
+        return {name: {f: getattr(arr[k], f) for f, _ in N.dv_kernel_time._fields_}
+                for k, name in enumerate(N.KERNEL_KINDS)}
+
+    # -- matrices ------------------------------------------------------------
+    def to_device(self, m):
+        keep = []
+        out = N.dv_rcrs()
+        self._check(N.lib().dv_copy(self._h, C.byref(m._desc(keep)), N.DV_DEVICE,
+                                    C.byref(out)), "dv_copy")
+        return DeviceRCRS(self, out)
+
+    def to_host(self, m):
+        keep = []
+        out = N.dv_rcrs()
+        self._check(N.lib().dv_copy(self._h, C.byref(m._desc(keep)), N.DV_HOST,
+                                    C.byref(out)), "dv_copy")
+        return self._out(out, "host")
+
+    def identity(self, n, size_bytes=8, location="host"):
+        out = N.dv_rcrs()
+        self._check(N.lib().dv_range_identity(self._h, n, size_bytes, self._loc(location),
+                                              C.byref(out)), "dv_range_identity")
+        return self._out(out, location)
+
+    def range_spadd(self, a, b, location="host"):
+        keep = []
+        out = N.dv_rcrs()
+        self._check(N.lib().dv_range_spadd(self._h, C.byref(a._desc(keep)),
+                                           C.byref(b._desc(keep)), self._loc(location),
+                                           C.byref(out)), "dv_range_spadd")
+        return self._out(out, location)
+
+    def range_spgemm(self, a, b, location="host"):
+        keep = []
+        out = N.dv_rcrs()
+        self._check(N.lib().dv_range_spgemm(self._h, C.byref(a._desc(keep)),
+                                            C.byref(b._desc(keep)), self._loc(location),
+                                            C.byref(out)), "dv_range_spgemm")
+        return self._out(out, location)
+
+    def range_power(self, a, k, location="host"):
+        keep = []
+        out = N.dv_rcrs()
+        self._check(N.lib().dv_range_power(self._h, C.byref(a._desc(keep)), k,
+                                           self._loc(location), C.byref(out)),
+                    "dv_range_power")
+        return self._out(out, location)
+
+    def create_distance_index(self, a, dlo, dup, location="host", stats=False):
+        keep = []
+        out = N.dv_rcrs()
+        st = N.dv_build_stats()
+        self._check(N.lib().dv_create_distance_index(
+            self._h, C.byref(a._desc(keep)), dlo, dup, self._loc(location), C.byref(out),
+            C.byref(st)), "dv_create_distance_index")
+        m = self._out(out, location)
+        if stats:
+            return m, BuildStats({f: getattr(st, f) for f, _ in N.dv_build_stats._fields_})
+        return m
+
+    def query(self, m, qi, qj):
+        """Batched D(i, j); raises DiVerGError(DV_ERR_RANGE) for pairs outside."""
+        keep = []
+        qi = np.ascontiguousarray(qi, dtype=np.uint32)
+        qj = np.ascontiguousarray(qj, dtype=np.uint32)
+        out = np.zeros(len(qi), dtype=np.uint8)
+        rc = N.lib().dv_query_batch(self._h, C.byref(m._desc(keep)),
+                                    qi.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                    qj.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                    out.ctypes.data_as(C.POINTER(C.c_uint8)), len(qi),
+                                    N.DV_HOST)
+        self._check(rc, "dv_query_batch")
+        return out
+
+
+def graph_preset(preset, component=0):
+    p = N.dv_graph_params()
+    rc = N.lib().dv_graph_preset(preset, component, C.byref(p))
+    if rc:
+        raise DiVerGError(rc, f"unknown preset {preset}")
+    return p
+
+
+def generate_graph(preset=None, component=0, size_bytes=8, params=None, with_vpos=False,
+                   **overrides):
+    """Seeded synthetic variation graph -> (RangeCRS A[, vpos])."""
+    p = params if params is not None else graph_preset(preset, component)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    out = N.dv_rcrs()
+    vp = C.POINTER(C.c_uint32)()
+    rc = N.lib().dv_generate_graph(C.byref(p), size_bytes, C.byref(out),
+                                   C.byref(vp) if with_vpos else None)
+    if rc:
+        raise DiVerGError(rc, "dv_generate_graph failed")
+    m = _from_desc(out)
+    N.lib().dv_free(None, C.byref(out))
+    if with_vpos:
+        vpos = np.ctypeslib.as_array(vp, shape=(m.nrows,)).copy()
+        N.lib().dv_free_host(C.cast(vp, C.c_void_p))
+        return m, vpos
+    return m

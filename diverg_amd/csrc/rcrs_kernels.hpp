@@ -1,0 +1,81 @@
+// Launchers of the gfx950 range-CRS kernels (rcrs_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dv {
+
+// Device view of an rCRS operand; `rowmap` holds S = uint32_t or uint64_t.
+struct DView {
+  const void *rowmap;
+  const uint32_t *entries;
+  uint64_t nrows;
+  uint64_t ncols;
+};
+
+// SpGEMM launch shape.
+struct SpgemmShape {
+  uint32_t lanes;      // lanes per row group: 64, 32, 16, 8 or 4
+  uint32_t lds_words;  // bit-window words per group
+  uint32_t stage;      // entries per row kept by the counting pass
+  uint32_t rows_per_group;
+  uint32_t grid;       // workgroups (multiple of 8)
+};
+
+// Device counters of one SpGEMM / SpAdd, read back once per operation.
+struct OpCounters {
+  unsigned long long nnz;
+  unsigned long long total;     // entries of C (rowmap[n])
+  unsigned long long slice;     // B-slice ranges gathered by the counting pass
+  unsigned int overflow;        // rows that need the direct-write pass
+  unsigned int pad;
+};
+
+void fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
+void zero_counters(hipStream_t s, OpCounters *c);
+
+void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *entries);
+
+// rowmap[0] = 0, rowmap[i+1] = cnt[0] + ... + cnt[i]; total -> ctr->total.
+// `tmp` needs scan_tmp_words(n) uint64 words.
+uint64_t scan_tmp_words(uint64_t n);
+void scan_counts(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n,
+                 void *rowmap, unsigned long long *tmp, OpCounters *ctr);
+
+void spadd_count(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt);
+void spadd_write(hipStream_t s, uint32_t sb, DView a, DView b,
+                 const void *c_rowmap, uint32_t *c_entries);
+void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t n_entries,
+               OpCounters *ctr);
+
+// Counting pass: per-row entry counts into `cnt`, the first `stage` entries
+// of each row into `stage_buf` (row-major, `stage` per row), rows with more
+// entries appended to `ovf_list`, and the set-cell count into ctr->nnz.
+void spgemm_count(hipStream_t s, uint32_t sb, DView a, DView b,
+                  const SpgemmShape &shape, uint32_t *cnt, uint32_t *stage_buf,
+                  uint32_t *ovf_list, OpCounters *ctr);
+// Copy staged rows (count <= stage) to their place in C.
+void spgemm_emit_staged(hipStream_t s, uint32_t sb, uint64_t nrows,
+                        const void *c_rowmap, const uint32_t *stage_buf,
+                        uint32_t stage, uint32_t *c_entries);
+// Recompute the listed rows and write them straight into C.
+void spgemm_direct(hipStream_t s, uint32_t sb, DView a, DView b,
+                   const SpgemmShape &shape, const uint32_t *rows,
+                   uint32_t nrows_listed, const void *c_rowmap,
+                   uint32_t *c_entries);
+
+// out[q] = M(qi[q], qj[q]); 2 and *bad = 1 for out-of-range pairs.
+void query(hipStream_t s, uint32_t sb, DView m, const uint32_t *qi,
+           const uint32_t *qj, uint8_t *out, uint64_t n, uint32_t *bad);
+
+// Sets *flag when rowmap[0] != 0, rowmap[n] != n_entries, a row length is
+// odd or negative, or a range is not s <= e < ncols.
+void validate_matrix(hipStream_t s, uint32_t sb, DView m, uint64_t n_entries, uint32_t *flag);
+
+// Number of B ranges each row of A gathers (sum over the row's ranges of the
+// B slice length) -- the work measure behind the size classes.
+void slice_lengths(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *len);
+
+}  // namespace dv

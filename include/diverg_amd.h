@@ -1,0 +1,179 @@
+/*
+ * diverg_amd.h -- C ABI of the MI355X (gfx950) range-CRS Boolean sparse
+ * engine: range SpAdd, range SpGEMM, range power, distance-index
+ * construction and locus-pair queries, plus a batched build over many graph
+ * components and a serving loop.
+ *
+ * Each entry point names the DiVerG interface it replaces (paths relative to
+ * the DiVerG source tree).  Signatures use plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Matrices are described by `dv_rcrs`: rows of closed column intervals
+ *    [s, e], stored flat as entries = s0,e0,s1,e1,... with `rowmap[i]` ..
+ *    `rowmap[i+1]` delimiting row i (the layout of
+ *    CRSMatrix<RangeDynamic,bool,uint32_t,S>, crs_matrix.hpp:1873-2004).
+ *    `size_bytes` is sizeof(S): 4 (uint32_t rowmap) or 8 (uint64_t rowmap).
+ *  - Inputs may live on the host or on the device (`location`); they are not
+ *    modified.  Outputs are allocated by the library at the requested
+ *    location and released with dv_free().
+ *  - `nnz` is the number of set cells.  With size_bytes == 4 it is reduced
+ *    modulo 2^32, which is what the reference's uint32_t accumulator yields
+ *    (range_sparse.hpp:1183-1264).
+ *  - Errors are returned as dv_status codes; the reference instead calls
+ *    exit(134) through DIVERG_ASSERT (utils.hpp:80-88).  dv_last_error()
+ *    gives a message for the last failing call on a context.
+ *  - Calls are blocking unless named *_async.  A context is used by one host
+ *    thread at a time.
+ */
+#ifndef DIVERG_AMD_H
+#define DIVERG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DV_ABI_VERSION 1
+
+typedef enum dv_status {
+  DV_OK = 0,
+  DV_ERR_INVALID = 1,   /* bad argument: shape mismatch, dlo > dup, dup == 0,
+                           odd entry count, unsupported size_bytes */
+  DV_ERR_OVERFLOW = 2,  /* ncols > UINT32_MAX - 1 (range_sparse.hpp:2235-2236)
+                           or an entry count that does not fit a 32-bit rowmap */
+  DV_ERR_NOMEM = 3,     /* host or device allocation failed */
+  DV_ERR_DEVICE = 4,    /* HIP runtime error */
+  DV_ERR_BUDGET = 5,    /* serving: request exceeds the HBM budget */
+  DV_ERR_RANGE = 6,     /* query (i, j) outside the matrix (crs_matrix.hpp:1222-1243) */
+  DV_ERR_BUSY = 7,      /* serving: no free slot; retry */
+  DV_ERR_NOT_FOUND = 8  /* serving: unknown ticket or index id */
+} dv_status;
+
+typedef enum dv_location { DV_HOST = 0, DV_DEVICE = 1 } dv_location;
+
+typedef struct dv_rcrs {
+  uint64_t nrows;
+  uint64_t ncols;
+  void *rowmap;        /* nrows + 1 elements of size_bytes each; rowmap[0] == 0 */
+  uint32_t *entries;   /* n_entries elements (n_entries == rowmap[nrows], even) */
+  uint64_t n_entries;
+  uint64_t nnz;
+  uint32_t size_bytes; /* 4 or 8 */
+  uint32_t location;   /* dv_location */
+  void *owner;         /* library-private; NULL for caller-owned storage */
+} dv_rcrs;
+
+typedef struct dv_context dv_context;
+
+/* Device work of one build, measured with HIP events on the context stream.
+ * alg_bytes sums bytes(X) + bytes(Y) + bytes(C) over every SpGEMM and SpAdd
+ * of the build, with bytes(M) = size_bytes * (nrows + 1) + 4 * n_entries. */
+typedef struct dv_build_stats {
+  double build_ms;       /* first kernel after the input is resident .. last kernel writing D */
+  double h2d_ms;         /* host -> device copy of the input (0 if already on the device) */
+  double d2h_ms;         /* device -> host copy of the output (0 if kept on the device) */
+  uint64_t alg_bytes;
+  uint64_t peak_device_bytes;
+  uint32_t n_spgemm;
+  uint32_t n_spadd;
+  uint64_t overflow_rows;  /* rows recomputed by the direct-write SpGEMM pass */
+} dv_build_stats;
+
+/* --- runtime ------------------------------------------------------------
+ * Replaces diverg::initialize_runtime / finalize_runtime / ScopedRuntime
+ * (runtime.hpp:25-55, src/eti/runtime.cpp:22-49). */
+int dv_init(int device, dv_context **ctx);
+int dv_finalize(dv_context *ctx);
+const char *dv_last_error(const dv_context *ctx);
+const char *dv_status_string(int status);
+int dv_abi_version(void);
+/* The HIP stream (hipStream_t) the context's work runs on. */
+void *dv_stream(dv_context *ctx);
+
+/* Launch-shape tuning (the reference's SparseConfig / ExecGrid,
+ * range_sparse_base.hpp:99-498).  0 keeps the current value.
+ *   lds_words      words of the per-row LDS bit window (chunked beyond it)
+ *   stage_entries  entries per row kept by the counting pass
+ *   rows_per_group consecutive rows handled by one lane group
+ *   lanes_small    lanes per row for rows with short B slices (4..64) */
+typedef struct dv_tuning {
+  uint32_t lds_words;
+  uint32_t stage_entries;
+  uint32_t rows_per_group;
+  uint32_t lanes_small;
+  uint32_t small_slice_max;  /* B ranges per row at or below which lanes_small is used */
+  uint32_t autotune;         /* 1: pick the above per size class from a probe */
+} dv_tuning;
+int dv_get_tuning(const dv_context *ctx, dv_tuning *t);
+int dv_set_tuning(dv_context *ctx, const dv_tuning *t);
+
+/* Per-kernel timing with HIP events on the context stream (no counterpart in
+ * the reference, whose DIVERG_STATS timers print symbolic/numeric times after
+ * a fence, range_sparse.hpp:2243-2278).  Kinds: 0 SpGEMM counting pass,
+ * 1 SpGEMM write passes (staged copy + direct rows), 2 row-pointer scans,
+ * 3 SpAdd.  alg_bytes sums bytes(X) + bytes(Y) + bytes(C) of the timed
+ * operations; slice_bytes sums the bytes of B-slice ranges the pass streams
+ * (8 per range, re-reads included). */
+#define DV_KERNEL_KINDS 4
+typedef struct dv_kernel_time {
+  uint64_t launches;
+  double total_ms;
+  uint64_t alg_bytes;
+  uint64_t slice_bytes;
+} dv_kernel_time;
+int dv_profile(dv_context *ctx, int enable); /* enabling also clears the totals */
+int dv_profile_read(const dv_context *ctx, dv_kernel_time out[DV_KERNEL_KINDS]);
+
+/* --- matrices ------------------------------------------------------------ */
+/* Copy `src` to `location` (H2D, D2H, D2D or H2H); replaces
+ * entries_device_view / rowmap_device_view (crs_matrix_device.hpp:192-238)
+ * and make_crs_matrix (crs_matrix_device.hpp:265-290). */
+int dv_copy(dv_context *ctx, const dv_rcrs *src, uint32_t location, dv_rcrs *dst);
+/* Release a matrix allocated by the library. */
+int dv_free(dv_context *ctx, dv_rcrs *m);
+
+/* --- hot path ------------------------------------------------------------ */
+/* create_range_identity_matrix (range_sparse.hpp:66-110). */
+int dv_range_identity(dv_context *ctx, uint64_t n, uint32_t size_bytes,
+                      uint32_t out_location, dv_rcrs *out);
+
+/* C = A v B.  RangeSpAdd<M,Space>::spadd (range_sparse.hpp:430-477) and the
+ * view-level range_spadd (range_sparse.hpp:364-428). */
+int dv_range_spadd(dv_context *ctx, const dv_rcrs *a, const dv_rcrs *b,
+                   uint32_t out_location, dv_rcrs *c);
+
+/* C = A . B (Boolean).  RangeSpGEMM<M,Config>::spgemm
+ * (range_sparse.hpp:2283-2332) and the view-level range_spgemm
+ * (range_sparse.hpp:2215-2281). */
+int dv_range_spgemm(dv_context *ctx, const dv_rcrs *a, const dv_rcrs *b,
+                    uint32_t out_location, dv_rcrs *c);
+
+/* C = A^k.  RangePower<M,Config>::power (range_sparse.hpp:2334-2368) and
+ * range_power_inplace (range_sparse.hpp:2370-2431). */
+int dv_range_power(dv_context *ctx, const dv_rcrs *a, uint32_t k,
+                   uint32_t out_location, dv_rcrs *c);
+
+/* D = A^dlo . (A + I)^(dup - dlo), or (A + I)^dup when dlo == 0.
+ * util::create_distance_index (dindex.hpp:46-189; compiled boundary
+ * dindex_eti.hpp:44-53, src/eti/dindex.cpp:25-39).  `stats` may be NULL. */
+int dv_create_distance_index(dv_context *ctx, const dv_rcrs *a, uint32_t dlo,
+                             uint32_t dup, uint32_t out_location, dv_rcrs *d,
+                             dv_build_stats *stats);
+
+/* out[q] = D(qi[q], qj[q]) for q < n.  CRSMatrixBase::operator()(i, j)
+ * (crs_matrix.hpp:1197-1218) via binary_search(..., RangeGroup)
+ * (crs_matrix.hpp:149-169).  qi, qj, out are host or device pointers
+ * (`query_location`); the matrix may be host or device resident.
+ * Out-of-range pairs give out[q] = 2 and DV_ERR_RANGE (at(), crs_matrix.hpp:
+ * 1222-1243, throws). */
+int dv_query_batch(dv_context *ctx, const dv_rcrs *m, const uint32_t *qi,
+                   const uint32_t *qj, uint8_t *out, uint64_t n,
+                   uint32_t query_location);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIVERG_AMD_H */

@@ -1,0 +1,152 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bit-exact: rowmap, entries and nnz (check_identical, test_dindex.cpp:34-53).
+"""
+import numpy as np
+import pytest
+
+import diverg_amd as dv
+from oracle import oracle as orc
+from tests.rcrs_util import (assert_identical, bool_matmul, from_dense, random_dense,
+                             to_dense)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    return dv.generate_graph(1)
+
+
+@pytest.fixture(autouse=True)
+def default_tuning(ctx):
+    t = ctx.tuning()
+    yield
+    ctx.set_tuning(**{k: v for k, v in t.items() if k != "autotune"}, autotune=t["autotune"])
+
+
+@pytest.mark.parametrize("size_bytes", [4, 8])
+@pytest.mark.parametrize("shape,density,run", [
+    ((1, 1, 1), 1.0, 1),
+    ((40, 70, 33), 0.2, 3),
+    ((300, 500, 700), 0.05, 8),
+    ((257, 257, 4100), 0.02, 64),
+    ((64, 3000, 129), 0.3, 5),
+])
+def test_spgemm_random(ctx, size_bytes, shape, density, run):
+    m, k, n = shape
+    rng = np.random.default_rng(hash((shape, size_bytes)) % (1 << 32))
+    da = random_dense(rng, m, k, density, run)
+    db = random_dense(rng, k, n, density, run)
+    a, b = from_dense(da, size_bytes), from_dense(db, size_bytes)
+    print(f"spgemm random {shape} S={size_bytes}", flush=True)
+    c = ctx.range_spgemm(a, b)
+    assert np.array_equal(to_dense(c), bool_matmul(da, db))
+    assert_identical(c, orc.spgemm(a, b), "spgemm vs oracle")
+
+
+def test_spgemm_empty_rows_and_matrix(ctx):
+    a = from_dense(np.zeros((5, 7), bool))
+    b = from_dense(np.ones((7, 9), bool))
+    c = ctx.range_spgemm(a, b)
+    assert c.nrows == 5 and len(c.entries) == 0 and c.nnz == 0
+    assert np.all(c.rowmap == 0)
+
+
+@pytest.mark.parametrize("lds_words,stage", [(512, 8), (1, 8), (2, 2), (3, 4), (64, 2)])
+def test_spgemm_window_chunks_and_stage(ctx, cfg1, lds_words, stage):
+    """Tiny LDS windows force the chunked path; tiny stages force the
+    direct-write recompute of most rows."""
+    ctx.set_tuning(lds_words=lds_words, stage_entries=stage)
+    ai = orc.spadd(orc.identity(cfg1.nrows), cfg1)
+    ai = dv.RangeCRS(ai.ncols, ai.entries, ai.rowmap, ai.nnz)
+    p = orc.power(ai, 37)
+    p = dv.RangeCRS(p.ncols, p.entries, p.rowmap, p.nnz)
+    print(f"spgemm chunks lds_words={lds_words} stage={stage}", flush=True)
+    c = ctx.range_spgemm(p, p)
+    assert_identical(c, orc.spgemm(p, p), "P.P")
+
+
+@pytest.mark.parametrize("size_bytes", [4, 8])
+def test_spadd_identity_plus_a(ctx, cfg1, size_bytes):
+    a = cfg1.as_size(size_bytes)
+    i = ctx.identity(a.nrows, size_bytes)
+    assert_identical(i, orc.identity(a.nrows), "identity")
+    c = ctx.range_spadd(i, a)
+    assert_identical(c, orc.spadd(i, a), "I + A")
+
+
+def test_spadd_random_and_noncanonical(ctx):
+    rng = np.random.default_rng(7)
+    da, db = random_dense(rng, 200, 300, 0.1, 6), random_dense(rng, 200, 300, 0.1, 6)
+    a, b = from_dense(da), from_dense(db)
+    c = ctx.range_spadd(a, b)
+    assert np.array_equal(to_dense(c), da | db)
+    assert_identical(c, orc.spadd(a, b), "spadd random")
+    # a non-canonical row: the tail after the first non-touching range is
+    # copied verbatim (range_sparse.hpp:328-355)
+    a = dv.RangeCRS(40, np.array([0, 1, 5, 6], np.uint32), np.array([0, 4], np.uint64))
+    b = dv.RangeCRS(40, np.array([2, 3, 20, 30, 8, 9], np.uint32), np.array([0, 6], np.uint64))
+    c = ctx.range_spadd(a, b)
+    assert c.entries.tolist() == [0, 3, 5, 6, 20, 30, 8, 9]
+    assert c.nnz == 19
+    assert_identical(c, orc.spadd(a, b), "spadd non-canonical")
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 5, 16, 150])
+def test_power(ctx, cfg1, k):
+    print(f"power k={k}", flush=True)
+    assert_identical(ctx.range_power(cfg1, k), orc.power(cfg1, k), f"A^{k}")
+
+
+@pytest.mark.parametrize("size_bytes", [4, 8])
+@pytest.mark.parametrize("dlo,dup", [(150, 450), (0, 250), (7, 7), (1, 8)])
+def test_distance_index_cfg1(ctx, cfg1, size_bytes, dlo, dup):
+    a = cfg1.as_size(size_bytes)
+    print(f"dindex cfg1 [{dlo},{dup}] S={size_bytes}", flush=True)
+    d, st = ctx.create_distance_index(a, dlo, dup, stats=True)
+    assert_identical(d, orc.dindex(a, dlo, dup), f"D[{dlo},{dup}]")
+    assert st["n_spadd"] == 1 and st["n_spgemm"] > 0
+
+
+def test_distance_index_bad_bounds(ctx, cfg1):
+    for dlo, dup in [(5, 4), (0, 0)]:
+        with pytest.raises(dv.DiVerGError) as e:
+            ctx.create_distance_index(cfg1, dlo, dup)
+        assert e.value.status == 1
+
+
+def test_device_resident_chain(ctx, cfg1):
+    da = ctx.to_device(cfg1)
+    sq = ctx.range_spgemm(da, da, location="device")
+    p4 = ctx.range_spgemm(sq, sq, location="device")
+    assert_identical(p4.to_host(), orc.power(cfg1, 4), "A^4 on device")
+    d = ctx.create_distance_index(da, 3, 9, location="device")
+    assert_identical(d.to_host(), orc.dindex(cfg1, 3, 9), "D on device")
+
+
+def test_query_batch(ctx, cfg1):
+    d = orc.dindex(cfg1, 150, 450)
+    d = dv.RangeCRS(d.ncols, d.entries, d.rowmap, d.nnz)
+    rng = np.random.default_rng(11)
+    n = 200000
+    qi = rng.integers(0, d.nrows, n, dtype=np.uint32)
+    near = rng.integers(0, 600, n, dtype=np.uint32)
+    qj = np.where(rng.random(n) < 0.5, np.minimum(qi + near, d.ncols - 1),
+                  rng.integers(0, d.ncols, n)).astype(np.uint32)
+    got = ctx.query(d, qi, qj)
+    exp, rc = orc.query(d, qi, qj)
+    assert rc == 0
+    assert np.array_equal(got, exp)
+    assert 0 < got.sum() < n
+    dd = ctx.to_device(d)
+    assert np.array_equal(ctx.query(dd, qi, qj), exp)
+    with pytest.raises(dv.DiVerGError) as e:
+        ctx.query(d, np.array([d.nrows], np.uint32), np.array([0], np.uint32))
+    assert e.value.status == 6
+
+
+def test_malformed_inputs_rejected(ctx):
+    bad = dv.RangeCRS(10, np.array([3, 12], np.uint32), np.array([0, 2], np.uint64))
+    with pytest.raises(dv.DiVerGError):
+        ctx.range_spgemm(bad, from_dense(np.ones((10, 10), bool)))
