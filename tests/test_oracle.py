@@ -1,0 +1,223 @@
+"""Pinning the CPU oracle (CPU only).
+
+Each oracle function is checked against something it does not compute
+itself: the reference's own known-answer arrays, dense Boolean products,
+dense walk-length layering, closed forms on chains, and a literal hand trace
+of the SpAdd code for non-canonical rows.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import diverg_amd as dv
+from oracle import oracle as orc
+from tests.rcrs_util import (assert_identical, bool_matmul, bool_power, from_dense,
+                             random_dense, to_dense, walk_index_dense)
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+def rcrs(j):
+    return dv.RangeCRS(j["ncols"], np.array(j["entries"], np.uint32),
+                       np.array(j["rowmap"], np.uint64), j.get("nnz"))
+
+
+def test_hbitvector_kat():
+    """test_hbitvector.cpp:569-661: scattering a row's ranges into the bit
+    vector and gathering map01|map10 bounds gives the maximal runs."""
+    k = load("hbitvector_kat.json")
+    m = dv.RangeCRS(k["ncols"], np.array(k["entries"], np.uint32),
+                    np.array(k["rowmap"], np.uint64))
+    c = orc.spgemm(orc.identity(5), m)
+    assert c.rowmap.tolist() == k["expected_rowmap"]
+    assert c.entries.tolist() == k["expected_entries"]
+    assert c.ncols == k["ncols"]
+
+
+def test_example26():
+    k = load("example26.json")
+    bits = [i for i, ch in enumerate(k["bits"]) if ch == "1"]
+    # one single-bit range per set bit (non-canonical input)
+    m = dv.RangeCRS(16, np.repeat(np.array(bits, np.uint32), 2), np.array([0, 2 * len(bits)]))
+    c = orc.spgemm(orc.identity(1), m)
+    assert c.entries.tolist() == k["expected_entries"]
+    assert c.nnz == len(bits)
+
+
+def test_query_tiny_kat():
+    k = load("crsmatrix_tiny.json")
+    dense = np.zeros((k["nrows"], k["ncols"]), bool)
+    for i, j in k["cells"]:
+        dense[i, j] = True
+    m = from_dense(dense)
+    assert m.nnz == 25
+    qi, qj = np.meshgrid(np.arange(10, dtype=np.uint32), np.arange(10, dtype=np.uint32),
+                         indexing="ij")
+    out, rc = orc.query(m, qi.ravel(), qj.ravel())
+    assert rc == 0
+    assert np.array_equal(out.reshape(10, 10).astype(bool), dense)
+    out, rc = orc.query(m, np.array([10], np.uint32), np.array([0], np.uint32))
+    assert rc != 0 and out[0] == 2
+
+
+@pytest.mark.parametrize("shape,density,run,seed", [
+    ((1, 1, 1), 1.0, 1, 0),
+    ((30, 40, 50), 0.3, 2, 1),
+    ((100, 300, 2500), 0.05, 40, 2),   # windows beyond the 2048-bit L1 band
+    ((64, 64, 63), 0.5, 8, 3),         # ncols not a multiple of 64
+    ((200, 150, 5000), 0.01, 300, 4),
+])
+def test_spgemm_dense(shape, density, run, seed):
+    m, k, n = shape
+    rng = np.random.default_rng(seed)
+    da, db = random_dense(rng, m, k, density, run), random_dense(rng, k, n, density, run)
+    c = orc.spgemm(from_dense(da), from_dense(db))
+    prod = bool_matmul(da, db)
+    assert_identical(c, from_dense(prod), "oracle spgemm vs dense")
+    assert c.nnz == prod.sum()
+
+
+def test_spgemm_full_and_empty():
+    a = from_dense(np.ones((70, 70), bool))
+    c = orc.spgemm(a, a)
+    assert c.entries.tolist() == [0, 69] * 70
+    z = from_dense(np.zeros((4, 70), bool))
+    c = orc.spgemm(z, a)
+    assert len(c.entries) == 0 and c.rowmap.tolist() == [0] * 5
+
+
+def test_spgemm_run_to_window_end():
+    """A run ending on the last bit of the window takes the fix-up of
+    range_sparse.hpp:1435-1440."""
+    b = from_dense(np.ones((1, 2048), bool))
+    c = orc.spgemm(orc.identity(1), b)
+    assert c.entries.tolist() == [0, 2047]
+    d = np.zeros((1, 4096), bool)
+    d[0, 100:4096] = True
+    c = orc.spgemm(orc.identity(1), from_dense(d))
+    assert c.entries.tolist() == [100, 4095]
+
+
+def test_spadd_union():
+    rng = np.random.default_rng(5)
+    da, db = random_dense(rng, 100, 700, 0.05, 20), random_dense(rng, 100, 700, 0.05, 20)
+    c = orc.spadd(from_dense(da), from_dense(db))
+    assert_identical(c, from_dense(da | db), "spadd union")
+
+
+def test_spadd_noncanonical_hand_trace():
+    """Hand trace of range_sparse.hpp:273-356 on a row pair: a = [0,1],[5,6];
+    b = [2,3],[20,30],[8,9].  [2,3] folds into [0,1]; [5,6] closes [0,3];
+    a is used up; [20,30] does not touch [5,6], so [5,6] and [20,30] are
+    written and the rest of b, [8,9], is copied verbatim."""
+    a = dv.RangeCRS(40, np.array([0, 1, 5, 6], np.uint32), np.array([0, 4], np.uint64))
+    b = dv.RangeCRS(40, np.array([2, 3, 20, 30, 8, 9], np.uint32), np.array([0, 6], np.uint64))
+    c = orc.spadd(a, b)
+    assert c.entries.tolist() == [0, 3, 5, 6, 20, 30, 8, 9]
+    assert c.nnz == 4 + 2 + 11 + 2
+    # an empty side copies the other verbatim
+    e = dv.RangeCRS(40, np.zeros(0, np.uint32), np.array([0, 0], np.uint64))
+    assert orc.spadd(e, b).entries.tolist() == [2, 3, 20, 30, 8, 9]
+    # ties take a first: a=[4,4] b=[4,9] -> [4,9]
+    t = orc.spadd(dv.RangeCRS(40, np.array([4, 4], np.uint32), np.array([0, 2])),
+                  dv.RangeCRS(40, np.array([4, 9], np.uint32), np.array([0, 2])))
+    assert t.entries.tolist() == [4, 9]
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 3, 7, 16])
+def test_power_dense(k):
+    rng = np.random.default_rng(k)
+    d = random_dense(rng, 90, 90, 0.03, 3)
+    p = orc.power(from_dense(d), k)
+    assert_identical(p, from_dense(bool_power(d, k)), f"A^{k}")
+
+
+def test_power_zero_is_identity_with_ncols_nnz():
+    a = from_dense(np.zeros((6, 6), bool))
+    p = orc.power(a, 0)
+    assert p.entries.tolist() == [i for i in range(6) for _ in (0, 1)]
+    assert p.nnz == 6
+
+
+@pytest.mark.parametrize("name", ["chain12", "snp", "deletion", "self_loop",
+                                  "two_components", "cycle"])
+def test_dindex_micro_graphs(name):
+    g = load("micro_graphs.json")[name]
+    a = rcrs(g["A"])
+    dense = to_dense(a)
+    for case in g["cases"]:
+        exp = rcrs(case["D"])
+        got = orc.dindex(a, case["dlo"], case["dup"])
+        assert_identical(got, exp, f"{name} [{case['dlo']},{case['dup']}]")
+        # and the fixture itself is the walk-length definition
+        assert np.array_equal(to_dense(exp), walk_index_dense(dense, case["dlo"], case["dup"]))
+
+
+def test_dindex_chain_closed_form():
+    """On a chain of L loci, row i of D is [i + d1, min(i + d2, L - 1)]."""
+    L, d1, d2 = 500, 13, 77
+    a = from_dense(np.eye(L, k=1, dtype=bool))
+    d = orc.dindex(a, d1, d2)
+    for i in range(L):
+        exp = [] if i + d1 > L - 1 else [i + d1, min(i + d2, L - 1)]
+        assert d.entries[d.rowmap[i]:d.rowmap[i + 1]].tolist() == exp
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_dindex_generated_vs_walks(seed):
+    a = dv.generate_graph(3, backbone_bp=400, seed=1000 + seed, sv_rate=0.01, sv_min=3,
+                          sv_max=40, snp_rate=0.05, indel_rate=0.02)
+    dense = to_dense(a)
+    for dlo, dup in [(0, 9), (5, 17), (12, 12)]:
+        got = orc.dindex(a, dlo, dup)
+        assert np.array_equal(to_dense(got), walk_index_dense(dense, dlo, dup))
+        assert_identical(got, from_dense(walk_index_dense(dense, dlo, dup)))
+
+
+def test_walk_rows_matches_dindex():
+    a = dv.generate_graph(1, backbone_bp=3000)
+    d = orc.dindex(a, 20, 60)
+    rows = np.arange(0, a.nrows, 37, dtype=np.uint32)
+    w = orc.walk_rows(a, rows, 20, 60)
+    for k, r in enumerate(rows):
+        assert np.array_equal(d.entries[d.rowmap[r]:d.rowmap[r + 1]],
+                              w.entries[w.rowmap[k]:w.rowmap[k + 1]])
+
+
+def test_golden_cfg1_2000bp():
+    z = np.load(os.path.join(GOLD, "cfg1_2000bp.npz"))
+    a = dv.generate_graph(1, backbone_bp=2000)
+    assert np.array_equal(a.rowmap, z["a_rowmap"]) and np.array_equal(a.entries, z["a_entries"])
+    d = orc.dindex(a, int(z["dlo"]), int(z["dup"]))
+    assert np.array_equal(d.rowmap, z["d_rowmap"])
+    assert np.array_equal(d.entries, z["d_entries"])
+    assert d.nnz == int(z["d_nnz"])
+
+
+def test_dindex_rejects_bad_bounds():
+    a = from_dense(np.eye(4, k=1, dtype=bool))
+    for dlo, dup in [(3, 2), (0, 0)]:
+        with pytest.raises(orc.OracleError):
+            orc.dindex(a, dlo, dup)
+
+
+def test_planted_mistakes_are_caught():
+    """The checks above fail on plausible mistakes: a dropped last range, an
+    off-by-one end, and a missing I in (A + I)."""
+    a = dv.generate_graph(1, backbone_bp=300)
+    dense = to_dense(a)
+    good = walk_index_dense(dense, 4, 11)
+    d = orc.dindex(a, 4, 11)
+    assert np.array_equal(to_dense(d), good)
+    e = d.entries.copy()
+    e[-1] -= 1  # off-by-one end
+    assert not np.array_equal(to_dense(dv.RangeCRS(d.ncols, e, d.rowmap)), good)
+    no_i = orc.spgemm(orc.power(a, 4), orc.power(a, 7))  # A^4 . A^7 instead of (A+I)^7
+    assert not np.array_equal(to_dense(no_i), good)
