@@ -210,8 +210,12 @@ class Context:
         return {f: getattr(t, f) for f, _ in N.dv_tuning._fields_}
 
     def set_tuning(self, **kw):
+        """Update the named fields of the launch tuning (dv_tuning)."""
         t = N.dv_tuning()
+        self._check(N.lib().dv_get_tuning(self._h, C.byref(t)), "dv_get_tuning")
         for k, v in kw.items():
+            if k not in dict(N.dv_tuning._fields_):
+                raise KeyError(k)
             setattr(t, k, int(v))
         self._check(N.lib().dv_set_tuning(self._h, C.byref(t)), "dv_set_tuning")
 

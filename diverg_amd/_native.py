@@ -48,6 +48,7 @@ class dv_tuning(C.Structure):
         ("rows_per_group", C.c_uint32),
         ("lanes_small", C.c_uint32),
         ("small_slice_max", C.c_uint32),
+        ("lane_slice_max", C.c_uint32),
         ("autotune", C.c_uint32),
     ]
 

@@ -75,6 +75,26 @@ bool Ctx::read_counters(OpCounters &out) {
   return true;
 }
 
+bool Ctx::scratch_carve(const uint64_t *bytes, void **out, int n) {
+  uint64_t total = 0;
+  for (int i = 0; i < n; ++i) total += (bytes[i] + 255) / 256 * 256;
+  if (total > scratch_cap) {
+    dfree(scratch);
+    scratch = nullptr;
+    scratch_cap = 0;
+    const uint64_t want = total + total / 4;
+    scratch = dalloc(want);
+    if (!scratch) return false;
+    scratch_cap = want;
+  }
+  auto *p = static_cast<char *>(scratch);
+  for (int i = 0; i < n; ++i) {
+    out[i] = p;
+    p += (bytes[i] + 255) / 256 * 256;
+  }
+  return true;
+}
+
 uint64_t mat_bytes(const DevMat &m) { return (uint64_t)m.sb * (m.nrows + 1) + 4 * m.n_entries; }
 
 DView view(const DevMat &m) { return DView{m.rowmap, m.entries, m.nrows, m.ncols}; }
@@ -307,12 +327,13 @@ int Ctx::spadd_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   c.ncols = a.ncols;
   c.sb = a.sb;
   c.owned = true;
-  uint32_t *cnt = static_cast<uint32_t *>(dalloc(4 * (n ? n : 1)));
-  auto *tmp = static_cast<unsigned long long *>(dalloc(8 * scan_tmp_words(n)));
+  void *buf[2];
+  const uint64_t sz[2] = {4 * (n ? n : 1), 8 * scan_tmp_words(n)};
+  const bool got = scratch_carve(sz, buf, 2);
+  auto *cnt = static_cast<uint32_t *>(buf[0]);
+  auto *tmp = static_cast<unsigned long long *>(buf[1]);
   c.rowmap = dalloc(c.sb * (n + 1));
-  if (!cnt || !tmp || !c.rowmap) {
-    dfree(cnt);
-    dfree(tmp);
+  if (!got || !c.rowmap) {
     release(c);
     return fail("device allocation failed (spadd)", DV_ERR_NOMEM);
   }
@@ -322,13 +343,9 @@ int Ctx::spadd_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   scan_counts(stream, c.sb, cnt, n, c.rowmap, tmp, d_ctr);
   OpCounters h{};
   if (!read_counters(h)) {
-    dfree(cnt);
-    dfree(tmp);
     release(c);
     return DV_ERR_DEVICE;
   }
-  dfree(cnt);
-  dfree(tmp);
   if (c.sb == 4 && h.total > 0xFFFFFFFFull) {
     release(c);
     return fail("spadd: entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
@@ -354,18 +371,15 @@ int Ctx::spadd_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   return DV_OK;
 }
 
-SpgemmShape Ctx::shape_for(const DevMat &a, uint32_t lanes) const {
-  SpgemmShape s{};
-  s.lanes = lanes;
-  s.lds_words = std::max<uint32_t>(1, tuning.lds_words * lanes / 64);
-  s.stage = tuning.stage_entries;
-  s.rows_per_group = std::max<uint32_t>(1, tuning.rows_per_group);
-  const uint64_t gpb = 256 / lanes;
-  uint64_t want = (a.nrows + gpb * s.rows_per_group - 1) / (gpb * s.rows_per_group);
-  const uint64_t cap = (uint64_t)num_cu * 16;
-  want = std::max<uint64_t>(8, std::min<uint64_t>(want, cap));
-  s.grid = (uint32_t)((want + 7) / 8 * 8);
-  return s;
+GroupLaunch Ctx::group_launch(uint32_t lanes, uint64_t nstage) const {
+  GroupLaunch g{};
+  g.lanes = lanes;
+  g.lds_words = std::max<uint32_t>(1, tuning.lds_words * lanes / 64);
+  g.stage = tuning.stage_entries;
+  g.rows_per_group = std::max<uint32_t>(1, tuning.rows_per_group);
+  g.grid = std::max<uint32_t>(8, (num_cu * 8 + 7) / 8 * 8);
+  g.nstage = nstage;
+  return g;
 }
 
 int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
@@ -380,37 +394,55 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   c.sb = a.sb;
   c.owned = true;
   const uint32_t KS = tuning.stage_entries;
-  uint32_t *cnt = static_cast<uint32_t *>(dalloc(4 * (n ? n : 1)));
-  uint32_t *stage = static_cast<uint32_t *>(dalloc(4ull * KS * (n ? n : 1)));
-  uint32_t *ovf = static_cast<uint32_t *>(dalloc(4 * (n ? n : 1)));
-  auto *tmp = static_cast<unsigned long long *>(dalloc(8 * scan_tmp_words(n)));
+  const uint64_t n1 = n ? n : 1;
+  void *buf[6];
+  const uint64_t sz[6] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * scan_tmp_words(n)};
+  const bool got = scratch_carve(sz, buf, 6);
+  auto *cnt = static_cast<uint32_t *>(buf[0]);
+  auto *stage = static_cast<uint32_t *>(buf[1]);
+  auto *ovf = static_cast<uint32_t *>(buf[2]);
+  auto *mid = static_cast<uint32_t *>(buf[3]);
+  auto *big = static_cast<uint32_t *>(buf[4]);
+  auto *tmp = static_cast<unsigned long long *>(buf[5]);
   c.rowmap = dalloc(c.sb * (n + 1));
-  auto cleanup = [&]() {
-    dfree(cnt);
-    dfree(stage);
-    dfree(ovf);
-    dfree(tmp);
-  };
-  if (!cnt || !stage || !ovf || !tmp || !c.rowmap) {
-    cleanup();
+  if (!got || !c.rowmap) {
     release(c);
     return fail("device allocation failed (spgemm)", DV_ERR_NOMEM);
   }
-  const SpgemmShape sh = shape_for(a, 64);
   zero_counters(stream, d_ctr);
   prof_mark(0);
-  spgemm_count(stream, a.sb, view(a), view(b), sh, cnt, stage, ovf, d_ctr);
+  // persistent lane-tier grid: few blocks keep the per-block counter atomics
+  // (nnz, slice ranges) uncontended
+  const uint32_t lane_grid =
+      (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)num_cu * 8));
+  spgemm_lane(stream, a.sb, view(a), view(b), KS, tuning.lane_slice_max, tuning.small_slice_max,
+              lane_grid, cnt, stage, ovf, mid, big, d_ctr);
+  if (n) {
+    GroupLaunch gm = group_launch(tuning.lanes_small, n);
+    gm.rows = mid;
+    gm.n_dev = &d_ctr->n_mid;
+    gm.cnt = cnt;
+    gm.stage_buf = stage;
+    gm.ovf_list = ovf;
+    gm.ctr = d_ctr;
+    spgemm_group(stream, a.sb, view(a), view(b), gm);
+    GroupLaunch gb = gm;
+    GroupLaunch g64 = group_launch(64, n);
+    gb.lanes = 64;
+    gb.lds_words = g64.lds_words;
+    gb.rows = big;
+    gb.n_dev = &d_ctr->n_big;
+    spgemm_group(stream, a.sb, view(a), view(b), gb);
+  }
   prof_mark(1);
   scan_counts(stream, c.sb, cnt, n, c.rowmap, tmp, d_ctr);
   prof_mark(2);
   OpCounters h{};
   if (!hip(hipGetLastError(), "spgemm count launch") || !read_counters(h)) {
-    cleanup();
     release(c);
     return DV_ERR_DEVICE;
   }
   if (c.sb == 4 && h.total > 0xFFFFFFFFull) {
-    cleanup();
     release(c);
     return fail("spgemm: entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
   }
@@ -418,20 +450,24 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   c.nnz = h.nnz;
   c.entries = static_cast<uint32_t *>(dalloc(4 * c.n_entries));
   if (!c.entries) {
-    cleanup();
     release(c);
     return fail("device allocation failed (spgemm entries)", DV_ERR_NOMEM);
   }
   prof_mark(3);
   spgemm_emit_staged(stream, c.sb, n, c.rowmap, stage, KS, c.entries);
-  spgemm_direct(stream, c.sb, view(a), view(b), sh, ovf, h.overflow, c.rowmap, c.entries);
+  if (h.overflow) {
+    GroupLaunch gd = group_launch(64, n);
+    gd.rows = ovf;
+    gd.n_host = h.overflow;
+    gd.c_rowmap = c.rowmap;
+    gd.c_entries = c.entries;
+    spgemm_group(stream, a.sb, view(a), view(b), gd);
+  }
   prof_mark(4);
   if (!hip(hipGetLastError(), "spgemm write launch")) {
-    cleanup();
     release(c);
     return DV_ERR_DEVICE;
   }
-  cleanup();
   {
     const uint64_t alg = mat_bytes(a) + mat_bytes(b) + mat_bytes(c);
     prof_add(0, 0, 1, alg, 16ull * h.slice);
@@ -441,6 +477,8 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   stats_alg_bytes += mat_bytes(a) + mat_bytes(b) + mat_bytes(c);
   stats_n_spgemm += 1;
   stats_overflow_rows += h.overflow;
+  stats_mid_rows += h.n_mid;
+  stats_big_rows += h.n_big;
   return DV_OK;
 }
 
@@ -613,19 +651,16 @@ int dv_get_tuning(const dv_context *c, dv_tuning *t) {
 
 int dv_set_tuning(dv_context *c, const dv_tuning *t) {
   if (!c || !t) return DV_ERR_INVALID;
-  dv_tuning n = c->tuning;
-  if (t->lds_words) n.lds_words = t->lds_words;
-  if (t->stage_entries) n.stage_entries = t->stage_entries;
-  if (t->rows_per_group) n.rows_per_group = t->rows_per_group;
-  if (t->lanes_small) n.lanes_small = t->lanes_small;
-  if (t->small_slice_max) n.small_slice_max = t->small_slice_max;
-  n.autotune = t->autotune;
+  const dv_tuning n = *t;
   const uint32_t ls = n.lanes_small;
   if (ls != 4 && ls != 8 && ls != 16 && ls != 32 && ls != 64)
     return C(c)->fail("lanes_small must be 4, 8, 16, 32 or 64");
-  if (n.lds_words * 8ull * 4 > c->lds_per_block)
-    return C(c)->fail("lds_words exceeds the LDS per workgroup");
-  if (n.stage_entries % 2) return C(c)->fail("stage_entries must be even");
+  if (n.lds_words < 1 || n.lds_words * 8ull * 4 > c->lds_per_block)
+    return C(c)->fail("lds_words must be 1 .. LDS per workgroup / 32");
+  if (n.stage_entries < 2 || n.stage_entries % 2 || n.stage_entries > 64)
+    return C(c)->fail("stage_entries must be even, 2 .. 64");
+  if (n.rows_per_group < 1) return C(c)->fail("rows_per_group must be >= 1");
+  if (n.lane_slice_max > (1u << 20)) return C(c)->fail("lane_slice_max must be 0 .. 2^20");
   c->tuning = n;
   return DV_OK;
 }

@@ -32,12 +32,19 @@ struct Ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t num_cu = 256;
   uint32_t lds_per_block = 65536;
-  dv_tuning tuning{512, 8, 16, 16, 16, 0};
+  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0};
   std::string last_error;
 
   // pool accounting
   std::unordered_map<void *, uint64_t> sizes;
   uint64_t cur_bytes = 0, peak_bytes = 0;
+
+  // scratch arena reused by consecutive operations on the stream
+  void *scratch = nullptr;
+  uint64_t scratch_cap = 0;
+  // Carve `n` buffers of the given byte sizes (256-byte aligned) from the
+  // arena, growing it when needed; nullptr on allocation failure.
+  bool scratch_carve(const uint64_t *bytes, void **out, int n);
 
   // one-word readbacks
   OpCounters *d_ctr = nullptr, *h_ctr = nullptr;
@@ -62,10 +69,10 @@ struct Ctx {
   }
 
   // per-operation statistics
-  uint64_t stats_alg_bytes = 0, stats_overflow_rows = 0;
+  uint64_t stats_alg_bytes = 0, stats_overflow_rows = 0, stats_mid_rows = 0, stats_big_rows = 0;
   uint32_t stats_n_spgemm = 0, stats_n_spadd = 0;
   void reset_stats() {
-    stats_alg_bytes = stats_overflow_rows = 0;
+    stats_alg_bytes = stats_overflow_rows = stats_mid_rows = stats_big_rows = 0;
     stats_n_spgemm = stats_n_spadd = 0;
   }
 
@@ -81,7 +88,7 @@ struct Ctx {
   int clone(const DevMat &src, DevMat &dst);
   int identity_dev(uint64_t n, uint32_t sb, DevMat &out);
   int spadd_dev(const DevMat &a, const DevMat &b, DevMat &c);
-  SpgemmShape shape_for(const DevMat &a, uint32_t lanes) const;
+  GroupLaunch group_launch(uint32_t lanes, uint64_t nstage) const;
   int spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c);
   int power_dev(const DevMat &a, uint32_t k, DevMat &out);
   int dindex_dev(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);

@@ -2,29 +2,35 @@
 //
 // Range SpGEMM (Gustavson, row-wise): row i of C = A.B is the union of the
 // B rows named by A(i)'s ranges, emitted as maximal runs
-// (range_sparse.hpp:1184-1264 count, :1340-1441 write).  Here each row is
-// owned by a group of G lanes inside one wave64 (G = 64 for long B slices,
-// fewer for short ones), with a private bit window in LDS:
-//   1. stream the row's B slices -- for each A range [s, e] the flat,
-//      contiguous entries B.entries[B.rowmap[s] .. B.rowmap[e+1]) -- as
-//      coalesced 8-byte (start, end) loads and reduce the column extent
-//      [lo, hi];
-//   2. clear the window words covering [lo, hi + 1] and OR each range in
-//      (ds_or_b64 on the two boundary words, plain all-ones stores between);
-//   3. gather: per word, run starts map01(x, c) = x & ~(x << 1 | c) and
-//      run-end successors map10(x, c) = (x << 1 | c) & ~x with c the top bit
-//      of the previous word (hbitvector.hpp:577-597); a group prefix sum of
-//      their popcounts places each bound, and the k-th bound of the row is
-//      written as its bit position minus (k & 1).
-// Bit hi + 1 always lies inside the window, so every run end is produced by
-// map10 and no last-word fix-up (range_sparse.hpp:1435-1440) is needed.
-// Windows wider than the LDS budget are processed in chunks, carrying the
-// top bit across chunk boundaries.
+// (range_sparse.hpp:1184-1264 count, :1340-1441 write).  For each A range
+// [s, e] the B rows s..e form one flat, contiguous slice of B's entries,
+// B.entries[B.rowmap[s] .. B.rowmap[e+1]), read as 8-byte (start, end) pairs.
 //
-// The counting pass keeps the first `stage` entries of every row, so rows
-// whose output fits are never recomputed: after the row-pointer scan they
-// are copied into place, and only the listed long rows run the scatter a
-// second time in direct-write mode.
+// Rows are split by work into three tiers, all producing the same canonical
+// runs:
+//   lane tier   one lane per row: the union is folded range by range into
+//               at most 4 disjoint intervals held in registers, then sorted
+//               and swept (a range-merge compaction).  Rows of banded
+//               matrices read neighbouring slices, so lane-per-row loads
+//               coalesce.  Covers rows whose union has at most 4 runs.
+//   16-lane and 64-lane groups (rows the lane tier hands over through
+//               wave-aggregated lists): the group reduces the extent
+//               [lo, hi], clears a private LDS bit window over [lo, hi + 1],
+//               ORs every range in (ds_or_b64 on boundary words, all-ones
+//               stores between), then gathers.  Slices up to 4 ranges per
+//               lane stay in registers between the two passes.
+// Gather, in every tier: per 64-bit word x with c the top bit of the previous
+// word, run starts are map01(x, c) = x & ~(x << 1 | c) and run-end successors
+// map10(x, c) = (x << 1 | c) & ~x (hbitvector.hpp:577-597); the k-th bound of
+// a row is written as its bit position minus (k & 1).  Bit hi + 1 always lies
+// inside the window, so every run end comes from map10 and no last-word
+// fix-up (range_sparse.hpp:1435-1440) is needed.  Windows wider than the LDS
+// budget are processed in chunks, carrying the top bit across chunks.
+//
+// The counting pass keeps the first `stage` entries of each row (column-major
+// staging, so lane-tier stores coalesce); after the row-pointer scan they are
+// copied into place and only rows with longer output are recomputed, in
+// direct-write mode.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,6 +42,7 @@ namespace dv {
 namespace {
 
 constexpr int BLOCK = 256;
+constexpr int CACHE = 4;             // group tiers: cached ranges per lane
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -105,30 +112,246 @@ struct Mat {
   const uint32_t *__restrict__ en;
 };
 
-// One row of C = A.B by a group of G lanes.  DIRECT: write entries at
-// c_en[c_base ...]; otherwise count, stage and list long rows.
+__device__ __forceinline__ uint2 ld_range(const uint32_t *__restrict__ en, uint64_t p) {
+  return *reinterpret_cast<const uint2 *>(en + p);
+}
+
+// Append `value` to list[*counter ...] for the flagged lanes of the wave with
+// one atomic per wave.  Must be reached by every lane of the wave.
+__device__ __forceinline__ void wave_append(bool flag, uint32_t value, uint32_t *list,
+                                            unsigned int *counter) {
+  const unsigned long long m = __ballot(flag);
+  if (m == 0) return;
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+  unsigned int base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(m));
+  base = (unsigned int)__shfl((int)base, (int)leader);
+  if (flag) list[base + __popcll(m & ((1ull << lane) - 1))] = value;
+}
+
+// Block sum of v added to *dst by one lane.  Callable repeatedly (the
+// trailing barrier protects the reduction buffer).
+__device__ void block_add_u64(unsigned long long v, unsigned long long *dst) {
+  __shared__ unsigned long long red[BLOCK / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
+    if (t) atomicAdd(dst, t);
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Lane tier: one lane per row.  The row's union is kept as at most
+// LANE_SLOTS disjoint intervals in registers.  Each slice range is folded
+// into the first interval it touches (overlap or adjacency), else into a free
+// slot; when no slot is free the intervals are compacted (sorted by start and
+// swept, merging touching neighbours) and the insert retried.  The final
+// compaction leaves the maximal runs in ascending order.  A row whose union
+// needs more than LANE_SLOTS runs, or whose slice exceeds lane_max ranges,
+// is handed to the group tiers.
+constexpr int LANE_SLOTS = 4;
+constexpr int LANE_BATCH = 8;  // slice loads issued together per lane
+
+struct Runs {
+  uint32_t lo[LANE_SLOTS], hi[LANE_SLOTS];  // empty slot: lo = 0xFFFFFFFF, hi = 0
+};
+
+__device__ __forceinline__ void cswap(Runs &r, int i, int j) {
+  const bool sw = r.lo[j] < r.lo[i];
+  const uint32_t l = sw ? r.lo[j] : r.lo[i], h = sw ? r.hi[j] : r.hi[i];
+  r.lo[j] = sw ? r.lo[i] : r.lo[j];
+  r.hi[j] = sw ? r.hi[i] : r.hi[j];
+  r.lo[i] = l;
+  r.hi[i] = h;
+}
+
+__device__ __forceinline__ void sort4(Runs &r) {
+  cswap(r, 0, 1);
+  cswap(r, 2, 3);
+  cswap(r, 0, 2);
+  cswap(r, 1, 3);
+  cswap(r, 1, 2);
+}
+
+// Sort slots by start (empty slots, start 0xFFFFFFFF, go last), fold each
+// slot that touches its predecessor forward into it (the predecessor is
+// emptied, so slot j always holds the running merged interval), and sort
+// again to move the emptied slots behind the live ones.
+__device__ __forceinline__ void compact(Runs &r) {
+  sort4(r);
+#pragma unroll
+  for (int j = 1; j < LANE_SLOTS; ++j) {
+    const bool live = r.lo[j - 1] <= r.hi[j - 1] && r.lo[j] <= r.hi[j];
+    if (live && r.lo[j] <= r.hi[j - 1] + 1u) {
+      r.lo[j] = r.lo[j - 1];
+      r.hi[j] = max(r.hi[j - 1], r.hi[j]);
+      r.lo[j - 1] = 0xFFFFFFFFu;
+      r.hi[j - 1] = 0u;
+    }
+  }
+  sort4(r);
+}
+
+// Fold [s, e] into the runs; false when no slot can take it.
+__device__ __forceinline__ bool insert(Runs &r, uint32_t s, uint32_t e) {
+  bool done = false;
+#pragma unroll
+  for (int t = 0; t < LANE_SLOTS; ++t) {
+    const bool live = r.lo[t] <= r.hi[t];
+    if (!done && live && s <= r.hi[t] + 1u && e + 1u >= r.lo[t]) {
+      r.lo[t] = min(r.lo[t], s);
+      r.hi[t] = max(r.hi[t], e);
+      done = true;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < LANE_SLOTS; ++t) {
+    if (!done && r.lo[t] > r.hi[t]) {
+      r.lo[t] = s;
+      r.hi[t] = e;
+      done = true;
+    }
+  }
+  return done;
+}
+
+template <typename S>
+__global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
+    Mat<S> A, Mat<S> B, uint64_t nrows, uint32_t KS, uint32_t lane_max, uint32_t mid_max,
+    uint32_t *__restrict__ cnt, uint32_t *__restrict__ stage,
+    uint32_t *__restrict__ ovf_list, uint32_t *__restrict__ mid_list,
+    uint32_t *__restrict__ big_list, OpCounters *__restrict__ ctr) {
+  unsigned long long nnz_acc = 0, slice_acc = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows;
+       base += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t row = base + threadIdx.x;
+    int cls = 0;  // 0 done here (or no row), 1 -> medium groups, 2 -> 64-lane groups
+    bool ovf = false;
+    if (row < nrows) {
+      const uint64_t a0 = A.rm[row], a1 = A.rm[row + 1];
+      uint64_t len = 0;
+      for (uint64_t ai = a0; ai < a1; ai += 2)
+        len += ((uint64_t)B.rm[(uint64_t)A.en[ai + 1] + 1] - B.rm[A.en[ai]]) >> 1;
+      slice_acc += len;
+      bool ok = len <= lane_max;
+      Runs r;
+#pragma unroll
+      for (int t = 0; t < LANE_SLOTS; ++t) {
+        r.lo[t] = 0xFFFFFFFFu;
+        r.hi[t] = 0u;
+      }
+      // The running interval [cl, ch] absorbs ranges that touch it (the
+      // common case in banded slices); any other range moves it into the
+      // slots and starts a new running interval.
+      uint32_t cl = 0xFFFFFFFFu, ch = 0u;
+      for (uint64_t ai = a0; ok && ai < a1; ai += 2) {
+        const uint64_t p0 = B.rm[A.en[ai]], p1 = B.rm[(uint64_t)A.en[ai + 1] + 1];
+        // LANE_BATCH independent loads in flight, then fold them in order
+        for (uint64_t p = p0; ok && p < p1; p += 2 * LANE_BATCH) {
+          uint2 v[LANE_BATCH];
+#pragma unroll
+          for (int k = 0; k < LANE_BATCH; ++k)
+            v[k] = p + 2 * k < p1 ? ld_range(B.en, p + 2 * k) : make_uint2(1u, 0u);
+#pragma unroll
+          for (int k = 0; k < LANE_BATCH; ++k) {
+            const uint32_t vs = v[k].x, ve = v[k].y;
+            if (vs > ve) continue;
+            if (vs <= ch + 1u && ve + 1u >= cl) {
+              cl = min(cl, vs);
+              ch = max(ch, ve);
+            } else {
+              if (cl <= ch && ok && !insert(r, cl, ch)) {
+                compact(r);
+                ok = insert(r, cl, ch);
+              }
+              cl = vs;
+              ch = ve;
+            }
+          }
+        }
+      }
+      if (ok && cl <= ch && !insert(r, cl, ch)) {
+        compact(r);
+        ok = insert(r, cl, ch);
+      }
+      if (!ok) {
+        cls = len <= mid_max ? 1 : 2;
+      } else {
+        compact(r);
+        uint32_t idx = 0;
+#pragma unroll
+        for (int t = 0; t < LANE_SLOTS; ++t) {
+          if (r.lo[t] <= r.hi[t]) {
+            nnz_acc += (unsigned long long)(r.hi[t] - r.lo[t]) + 1;
+            if (idx < KS) stage[(uint64_t)idx * nrows + row] = r.lo[t];
+            if (idx + 1 < KS) stage[(uint64_t)(idx + 1) * nrows + row] = r.hi[t];
+            idx += 2;
+          }
+        }
+        cnt[row] = idx;
+        ovf = idx > KS;
+      }
+    }
+    wave_append(ovf, (uint32_t)row, ovf_list, &ctr->overflow);
+    wave_append(cls == 1, (uint32_t)row, mid_list, &ctr->n_mid);
+    wave_append(cls == 2, (uint32_t)row, big_list, &ctr->n_big);
+  }
+  block_add_u64(nnz_acc, &ctr->nnz);
+  block_add_u64(slice_acc, &ctr->slice);
+}
+
+// ---------------------------------------------------------------------------
+// Group tiers: one row of C = A.B by a group of G lanes.  DIRECT: write the
+// row's entries at c_en[c_base ...]; otherwise count, stage and list rows
+// whose output exceeds the stage.
 template <typename S, int G, bool DIRECT>
-__device__ __forceinline__ void spgemm_row(uint64_t row, Mat<S> A, Mat<S> B,
-                                           uint64_t *lds, uint32_t W,
-                                           uint32_t lane, uint32_t KS,
-                                           uint32_t *cnt, uint32_t *stage,
-                                           uint32_t *ovf_list,
-                                           unsigned int *ovf_count,
+__device__ __forceinline__ void spgemm_row(uint64_t row, Mat<S> A, Mat<S> B, uint64_t *lds,
+                                           uint32_t W, uint32_t lane, uint32_t KS,
+                                           uint64_t nstage, uint32_t *cnt, uint32_t *stage,
+                                           uint32_t *ovf_list, unsigned int *ovf_count,
                                            uint32_t *c_en, uint64_t c_base,
-                                           unsigned long long &nnz_acc,
-                                           unsigned long long &slice_acc) {
+                                           unsigned long long &nnz_acc) {
   const uint64_t a0 = A.rm[row], a1 = A.rm[row + 1];
 
-  // 1. column extent of the row's B slices
+  // 1. column extent.  A single-segment slice of at most CACHE * G ranges is
+  //    kept in registers for the scatter pass.
+  uint2 rc[CACHE];
+  bool cached = false;
   uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-  for (uint64_t ai = a0; ai < a1; ai += 2) {
-    const uint32_t s = A.en[ai], e = A.en[ai + 1];
+  if (a1 - a0 == 2) {
+    const uint32_t s = A.en[a0], e = A.en[a0 + 1];
     const uint64_t p0 = B.rm[s], p1 = B.rm[(uint64_t)e + 1];
-    if (!DIRECT && lane == 0) slice_acc += (p1 - p0) >> 1;
-    for (uint64_t p = p0 + 2 * lane; p < p1; p += 2 * G) {
-      const uint2 r = *reinterpret_cast<const uint2 *>(B.en + p);
-      lo = min(lo, r.x);
-      hi = max(hi, r.y);
+    if (p1 - p0 <= 2ull * G * CACHE) {
+      cached = true;
+#pragma unroll
+      for (int k = 0; k < CACHE; ++k) {
+        const uint64_t p = p0 + 2ull * (lane + (uint32_t)k * G);
+        rc[k] = p < p1 ? ld_range(B.en, p) : make_uint2(1u, 0u);  // (1, 0): empty
+      }
+#pragma unroll
+      for (int k = 0; k < CACHE; ++k)
+        if (rc[k].x <= rc[k].y) {
+          lo = min(lo, rc[k].x);
+          hi = max(hi, rc[k].y);
+        }
+    }
+  }
+  if (!cached) {
+    for (uint64_t ai = a0; ai < a1; ai += 2) {
+      const uint32_t s = A.en[ai], e = A.en[ai + 1];
+      const uint64_t p1 = B.rm[(uint64_t)e + 1];
+#pragma unroll 4
+      for (uint64_t p = (uint64_t)B.rm[s] + 2 * lane; p < p1; p += 2 * G) {
+        const uint2 r = ld_range(B.en, p);
+        lo = min(lo, r.x);
+        hi = max(hi, r.y);
+      }
     }
   }
   lo = grp_min<G>(lo);
@@ -150,13 +373,21 @@ __device__ __forceinline__ void spgemm_row(uint64_t row, Mat<S> A, Mat<S> B,
     // 2. scatter the slice ranges clipped to this chunk
     const uint64_t cb = (w0 + c0) << 6;
     const uint64_t cl = cb + ((uint64_t)cw << 6) - 1;
-    for (uint64_t ai = a0; ai < a1; ai += 2) {
-      const uint32_t s = A.en[ai], e = A.en[ai + 1];
-      const uint64_t p1 = B.rm[(uint64_t)e + 1];
-      for (uint64_t p = (uint64_t)B.rm[s] + 2 * lane; p < p1; p += 2 * G) {
-        const uint2 r = *reinterpret_cast<const uint2 *>(B.en + p);
-        const uint64_t rs = max((uint64_t)r.x, cb), rf = min((uint64_t)r.y, cl);
+    if (cached) {
+#pragma unroll
+      for (int k = 0; k < CACHE; ++k) {
+        const uint64_t rs = max((uint64_t)rc[k].x, cb), rf = min((uint64_t)rc[k].y, cl);
         if (rs <= rf) window_set(lds, (uint32_t)(rs - cb), (uint32_t)(rf - cb));
+      }
+    } else {
+      for (uint64_t ai = a0; ai < a1; ai += 2) {
+        const uint32_t s = A.en[ai], e = A.en[ai + 1];
+        const uint64_t p1 = B.rm[(uint64_t)e + 1];
+        for (uint64_t p = (uint64_t)B.rm[s] + 2 * lane; p < p1; p += 2 * G) {
+          const uint2 r = ld_range(B.en, p);
+          const uint64_t rs = max((uint64_t)r.x, cb), rf = min((uint64_t)r.y, cl);
+          if (rs <= rf) window_set(lds, (uint32_t)(rs - cb), (uint32_t)(rf - cb));
+        }
       }
     }
     wave_sync();
@@ -191,7 +422,7 @@ __device__ __forceinline__ void spgemm_row(uint64_t row, Mat<S> A, Mat<S> B,
           if (DIRECT)
             c_en[c_base + idx] = val;
           else if (idx < KS)
-            stage[row * KS + idx] = val;
+            stage[(uint64_t)idx * nstage + row] = val;
           ++idx;
         }
       }
@@ -220,61 +451,35 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
   }
 }
 
-// Block sum of v added to *dst by one lane.  Callable repeatedly (the
-// trailing barrier protects the reduction buffer).
-__device__ void block_add_u64(unsigned long long v, unsigned long long *dst) {
-  __shared__ unsigned long long red[BLOCK / 64];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
-    if (t) atomicAdd(dst, t);
-  }
-  __syncthreads();
-}
-
-template <typename S, int G>
-__global__ __launch_bounds__(BLOCK) void k_spgemm_count(
-    Mat<S> A, Mat<S> B, uint64_t nrows, uint32_t W, uint32_t KS, uint32_t RPG,
-    uint32_t *__restrict__ cnt, uint32_t *__restrict__ stage,
-    uint32_t *__restrict__ ovf_list, OpCounters *__restrict__ ctr) {
+// Rows rows[0 .. n) with n = *n_dev (counting) or n_host (direct).  Groups
+// take RPG consecutive list entries at a time.
+template <typename S, int G, bool DIRECT>
+__global__ __launch_bounds__(BLOCK) void k_spgemm_group(
+    Mat<S> A, Mat<S> B, uint32_t W, uint32_t KS, uint32_t RPG, uint64_t nstage,
+    const uint32_t *__restrict__ rows, const unsigned int *__restrict__ n_dev, uint32_t n_host,
+    uint32_t *__restrict__ cnt, uint32_t *__restrict__ stage, uint32_t *__restrict__ ovf_list,
+    OpCounters *__restrict__ ctr, const S *__restrict__ c_rm, uint32_t *__restrict__ c_en) {
   extern __shared__ uint64_t lds_all[];
   constexpr uint32_t GPB = BLOCK / G;
   const uint32_t grp = threadIdx.x / G, lane = threadIdx.x % G;
   uint64_t *lds = lds_all + (size_t)grp * W;
-  unsigned long long nnz_acc = 0, slice_acc = 0;
+  const uint64_t nl = DIRECT ? n_host : *n_dev;
+  unsigned long long nnz_acc = 0;
   const uint64_t groups = (uint64_t)gridDim.x * GPB;
   const uint64_t gid = (uint64_t)xcd_tile(blockIdx.x, gridDim.x) * GPB + grp;
-  for (uint64_t r0 = gid * RPG; r0 < nrows; r0 += groups * RPG) {
-    const uint64_t r1 = min(r0 + RPG, nrows);
-    for (uint64_t row = uniform_u64<G>(r0); row < r1; ++row)
-      spgemm_row<S, G, false>(row, A, B, lds, W, lane, KS, cnt, stage, ovf_list,
-                              &ctr->overflow, nullptr, 0, nnz_acc, slice_acc);
+  for (uint64_t i0 = gid * RPG; i0 < nl; i0 += groups * RPG) {
+    const uint64_t i1 = min(i0 + RPG, nl);
+    for (uint64_t i = uniform_u64<G>(i0); i < i1; ++i) {
+      const uint64_t row = uniform_u64<G>(rows[i]);
+      spgemm_row<S, G, DIRECT>(row, A, B, lds, W, lane, KS, nstage, cnt, stage, ovf_list,
+                               DIRECT ? nullptr : &ctr->overflow, c_en,
+                               DIRECT ? (uint64_t)c_rm[row] : 0, nnz_acc);
+    }
   }
-  block_add_u64(nnz_acc, &ctr->nnz);
-  block_add_u64(slice_acc, &ctr->slice);
+  if (!DIRECT) block_add_u64(nnz_acc, &ctr->nnz);
 }
 
-template <typename S, int G>
-__global__ __launch_bounds__(BLOCK) void k_spgemm_direct(
-    Mat<S> A, Mat<S> B, uint32_t W, const uint32_t *__restrict__ rows,
-    uint32_t nlisted, const S *__restrict__ c_rm, uint32_t *__restrict__ c_en) {
-  extern __shared__ uint64_t lds_all[];
-  constexpr uint32_t GPB = BLOCK / G;
-  const uint32_t grp = threadIdx.x / G, lane = threadIdx.x % G;
-  uint64_t *lds = lds_all + (size_t)grp * W;
-  unsigned long long unused = 0, unused2 = 0;
-  const uint32_t groups = gridDim.x * GPB;
-  for (uint32_t i = blockIdx.x * GPB + grp; i < nlisted; i += groups) {
-    const uint64_t row = uniform_u64<G>(rows[i]);
-    spgemm_row<S, G, true>(row, A, B, lds, W, lane, 0, nullptr, nullptr, nullptr,
-                           nullptr, c_en, (uint64_t)c_rm[row], unused, unused2);
-  }
-}
-
+// Copy staged rows (count <= KS) into C; stage is column-major.
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_spgemm_emit_staged(
     uint64_t nrows, const S *__restrict__ c_rm, const uint32_t *__restrict__ stage,
@@ -284,7 +489,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_emit_staged(
     const uint64_t b = c_rm[row], e = c_rm[row + 1];
     const uint64_t c = e - b;
     if (c <= KS)
-      for (uint64_t k = 0; k < c; ++k) c_en[b + k] = stage[row * KS + k];
+      for (uint64_t k = 0; k < c; ++k) c_en[b + k] = stage[k * nrows + row];
   }
 }
 
@@ -570,45 +775,23 @@ Mat<S> mat(DView v) {
   return Mat<S>{static_cast<const S *>(v.rowmap), v.entries};
 }
 
-template <typename S, int G>
-void launch_count(hipStream_t s, DView a, DView b, const SpgemmShape &sh, uint32_t *cnt,
-                  uint32_t *stage, uint32_t *ovf, OpCounters *ctr) {
-  const size_t lds = (size_t)(BLOCK / G) * sh.lds_words * sizeof(uint64_t);
-  hipLaunchKernelGGL((k_spgemm_count<S, G>), dim3(sh.grid), dim3(BLOCK), lds, s, mat<S>(a),
-                     mat<S>(b), a.nrows, sh.lds_words, sh.stage, sh.rows_per_group, cnt,
-                     stage, ovf, ctr);
+template <typename S, int G, bool DIRECT>
+void launch_group(hipStream_t s, DView a, DView b, const GroupLaunch &g) {
+  const size_t lds = (size_t)(BLOCK / G) * g.lds_words * sizeof(uint64_t);
+  hipLaunchKernelGGL((k_spgemm_group<S, G, DIRECT>), dim3(g.grid), dim3(BLOCK), lds, s,
+                     mat<S>(a), mat<S>(b), g.lds_words, g.stage, g.rows_per_group, g.nstage,
+                     g.rows, g.n_dev, g.n_host, g.cnt, g.stage_buf, g.ovf_list, g.ctr,
+                     static_cast<const S *>(g.c_rowmap), g.c_entries);
 }
 
-template <typename S, int G>
-void launch_direct(hipStream_t s, DView a, DView b, const SpgemmShape &sh,
-                   const uint32_t *rows, uint32_t nl, const void *c_rm, uint32_t *c_en) {
-  const size_t lds = (size_t)(BLOCK / G) * sh.lds_words * sizeof(uint64_t);
-  const uint32_t grid = grid_for(nl, BLOCK / G, 1u << 20);
-  hipLaunchKernelGGL((k_spgemm_direct<S, G>), dim3(grid), dim3(BLOCK), lds, s, mat<S>(a),
-                     mat<S>(b), sh.lds_words, rows, nl, static_cast<const S *>(c_rm), c_en);
-}
-
-template <typename S>
-void count_by_lanes(hipStream_t s, DView a, DView b, const SpgemmShape &sh, uint32_t *cnt,
-                    uint32_t *stage, uint32_t *ovf, OpCounters *ctr) {
-  switch (sh.lanes) {
-    case 4: launch_count<S, 4>(s, a, b, sh, cnt, stage, ovf, ctr); break;
-    case 8: launch_count<S, 8>(s, a, b, sh, cnt, stage, ovf, ctr); break;
-    case 16: launch_count<S, 16>(s, a, b, sh, cnt, stage, ovf, ctr); break;
-    case 32: launch_count<S, 32>(s, a, b, sh, cnt, stage, ovf, ctr); break;
-    default: launch_count<S, 64>(s, a, b, sh, cnt, stage, ovf, ctr); break;
-  }
-}
-
-template <typename S>
-void direct_by_lanes(hipStream_t s, DView a, DView b, const SpgemmShape &sh,
-                     const uint32_t *rows, uint32_t nl, const void *c_rm, uint32_t *c_en) {
-  switch (sh.lanes) {
-    case 4: launch_direct<S, 4>(s, a, b, sh, rows, nl, c_rm, c_en); break;
-    case 8: launch_direct<S, 8>(s, a, b, sh, rows, nl, c_rm, c_en); break;
-    case 16: launch_direct<S, 16>(s, a, b, sh, rows, nl, c_rm, c_en); break;
-    case 32: launch_direct<S, 32>(s, a, b, sh, rows, nl, c_rm, c_en); break;
-    default: launch_direct<S, 64>(s, a, b, sh, rows, nl, c_rm, c_en); break;
+template <typename S, bool DIRECT>
+void group_by_lanes(hipStream_t s, DView a, DView b, const GroupLaunch &g) {
+  switch (g.lanes) {
+    case 4: launch_group<S, 4, DIRECT>(s, a, b, g); break;
+    case 8: launch_group<S, 8, DIRECT>(s, a, b, g); break;
+    case 16: launch_group<S, 16, DIRECT>(s, a, b, g); break;
+    case 32: launch_group<S, 32, DIRECT>(s, a, b, g); break;
+    default: launch_group<S, 64, DIRECT>(s, a, b, g); break;
   }
 }
 
@@ -686,13 +869,30 @@ void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t n_entries, OpCou
                      reinterpret_cast<const uint2 *>(entries), nr, ctr);
 }
 
-void spgemm_count(hipStream_t s, uint32_t sb, DView a, DView b, const SpgemmShape &shape,
-                  uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list, OpCounters *ctr) {
+void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, uint32_t lane_max,
+                 uint32_t mid_max, uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
+                 uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr) {
   if (!a.nrows) return;
   if (sb == 4)
-    count_by_lanes<uint32_t>(s, a, b, shape, cnt, stage_buf, ovf_list, ctr);
+    hipLaunchKernelGGL(k_spgemm_lane<uint32_t>, dim3(grid), dim3(BLOCK), 0, s, mat<uint32_t>(a),
+                       mat<uint32_t>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list,
+                       mid_list, big_list, ctr);
   else
-    count_by_lanes<uint64_t>(s, a, b, shape, cnt, stage_buf, ovf_list, ctr);
+    hipLaunchKernelGGL(k_spgemm_lane<uint64_t>, dim3(grid), dim3(BLOCK), 0, s, mat<uint64_t>(a),
+                       mat<uint64_t>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list,
+                       mid_list, big_list, ctr);
+}
+
+void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunch &g) {
+  const bool direct = g.c_entries != nullptr;
+  if (direct && g.n_host == 0) return;
+  if (sb == 4) {
+    if (direct) group_by_lanes<uint32_t, true>(s, a, b, g);
+    else group_by_lanes<uint32_t, false>(s, a, b, g);
+  } else {
+    if (direct) group_by_lanes<uint64_t, true>(s, a, b, g);
+    else group_by_lanes<uint64_t, false>(s, a, b, g);
+  }
 }
 
 void spgemm_emit_staged(hipStream_t s, uint32_t sb, uint64_t nrows, const void *c_rowmap,
@@ -706,16 +906,6 @@ void spgemm_emit_staged(hipStream_t s, uint32_t sb, uint64_t nrows, const void *
     hipLaunchKernelGGL(k_spgemm_emit_staged<uint64_t>, dim3(grid_for(nrows)), dim3(BLOCK), 0,
                        s, nrows, static_cast<const uint64_t *>(c_rowmap), stage_buf, stage,
                        c_entries);
-}
-
-void spgemm_direct(hipStream_t s, uint32_t sb, DView a, DView b, const SpgemmShape &shape,
-                   const uint32_t *rows, uint32_t nrows_listed, const void *c_rowmap,
-                   uint32_t *c_entries) {
-  if (!nrows_listed) return;
-  if (sb == 4)
-    direct_by_lanes<uint32_t>(s, a, b, shape, rows, nrows_listed, c_rowmap, c_entries);
-  else
-    direct_by_lanes<uint64_t>(s, a, b, shape, rows, nrows_listed, c_rowmap, c_entries);
 }
 
 void query(hipStream_t s, uint32_t sb, DView m, const uint32_t *qi, const uint32_t *qj,

@@ -15,22 +15,36 @@ struct DView {
   uint64_t ncols;
 };
 
-// SpGEMM launch shape.
-struct SpgemmShape {
-  uint32_t lanes;      // lanes per row group: 64, 32, 16, 8 or 4
-  uint32_t lds_words;  // bit-window words per group
-  uint32_t stage;      // entries per row kept by the counting pass
-  uint32_t rows_per_group;
-  uint32_t grid;       // workgroups (multiple of 8)
-};
-
 // Device counters of one SpGEMM / SpAdd, read back once per operation.
 struct OpCounters {
   unsigned long long nnz;
   unsigned long long total;     // entries of C (rowmap[n])
   unsigned long long slice;     // B-slice ranges gathered by the counting pass
-  unsigned int overflow;        // rows that need the direct-write pass
+  unsigned int overflow;        // rows whose output exceeds the stage
+  unsigned int n_mid;           // rows handed to the 16-lane groups
+  unsigned int n_big;           // rows handed to the 64-lane groups
   unsigned int pad;
+};
+
+// One launch of the group tiers over a row list.  Counting mode when
+// c_entries is NULL (list length read from *n_dev on the device), direct
+// write mode otherwise (list length n_host).
+struct GroupLaunch {
+  uint32_t lanes;           // 4, 8, 16, 32 or 64
+  uint32_t lds_words;       // bit-window words per group
+  uint32_t stage;           // entries per row kept by the counting pass
+  uint32_t rows_per_group;  // consecutive list entries per group visit
+  uint32_t grid;            // workgroups (multiple of 8)
+  uint64_t nstage;          // rows of the staging buffer (its column stride)
+  const uint32_t *rows;
+  const unsigned int *n_dev;
+  uint32_t n_host;
+  uint32_t *cnt;
+  uint32_t *stage_buf;
+  uint32_t *ovf_list;
+  OpCounters *ctr;
+  const void *c_rowmap;
+  uint32_t *c_entries;
 };
 
 void fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
@@ -50,21 +64,22 @@ void spadd_write(hipStream_t s, uint32_t sb, DView a, DView b,
 void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t n_entries,
                OpCounters *ctr);
 
-// Counting pass: per-row entry counts into `cnt`, the first `stage` entries
-// of each row into `stage_buf` (row-major, `stage` per row), rows with more
-// entries appended to `ovf_list`, and the set-cell count into ctr->nnz.
-void spgemm_count(hipStream_t s, uint32_t sb, DView a, DView b,
-                  const SpgemmShape &shape, uint32_t *cnt, uint32_t *stage_buf,
-                  uint32_t *ovf_list, OpCounters *ctr);
+// Lane tier of the counting pass: one lane per row of A.  Rows it cannot
+// finish (more than lane_max slice ranges, or a union of more than 4 runs)
+// are appended to mid_list (slices of at most mid_max ranges) or big_list.
+// Writes per-row entry counts, the first `stage` entries of each row into
+// stage_buf (column-major, stride nrows), rows with more output to ovf_list,
+// and adds set cells to ctr->nnz.
+void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, uint32_t lane_max,
+                 uint32_t mid_max,
+                 uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
+                 uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr);
+// Group tiers (counting or direct write), see GroupLaunch.
+void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunch &g);
 // Copy staged rows (count <= stage) to their place in C.
 void spgemm_emit_staged(hipStream_t s, uint32_t sb, uint64_t nrows,
                         const void *c_rowmap, const uint32_t *stage_buf,
                         uint32_t stage, uint32_t *c_entries);
-// Recompute the listed rows and write them straight into C.
-void spgemm_direct(hipStream_t s, uint32_t sb, DView a, DView b,
-                   const SpgemmShape &shape, const uint32_t *rows,
-                   uint32_t nrows_listed, const void *c_rowmap,
-                   uint32_t *c_entries);
 
 // out[q] = M(qi[q], qj[q]); 2 and *bad = 1 for out-of-range pairs.
 void query(hipStream_t s, uint32_t sb, DView m, const uint32_t *qi,

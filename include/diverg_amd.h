@@ -93,18 +93,26 @@ int dv_abi_version(void);
 void *dv_stream(dv_context *ctx);
 
 /* Launch-shape tuning (the reference's SparseConfig / ExecGrid,
- * range_sparse_base.hpp:99-498).  0 keeps the current value.
- *   lds_words      words of the per-row LDS bit window (chunked beyond it)
- *   stage_entries  entries per row kept by the counting pass
- *   rows_per_group consecutive rows handled by one lane group
- *   lanes_small    lanes per row for rows with short B slices (4..64) */
+ * range_sparse_base.hpp:99-498).  dv_set_tuning replaces the whole set;
+ * read it with dv_get_tuning first.
+ *   lds_words       LDS bit-window words of a 64-lane group (a G-lane group
+ *                   gets lds_words * G / 64); wider windows are chunked
+ *   stage_entries   entries per row kept by the counting pass (even)
+ *   rows_per_group  consecutive rows a lane group takes per visit
+ *   lanes_small     lanes per row for medium rows: 4, 8, 16, 32 or 64
+ *   small_slice_max medium rows: at most this many B-slice ranges
+ *   lane_slice_max  lane-per-row tier: at most this many slice ranges
+ *                   (0 sends every non-empty row to the groups)
+ *   autotune        1: dv_create_distance_index* picks lanes_small and
+ *                   small_slice_max per build from timed probes */
 typedef struct dv_tuning {
   uint32_t lds_words;
   uint32_t stage_entries;
   uint32_t rows_per_group;
   uint32_t lanes_small;
-  uint32_t small_slice_max;  /* B ranges per row at or below which lanes_small is used */
-  uint32_t autotune;         /* 1: pick the above per size class from a probe */
+  uint32_t small_slice_max;
+  uint32_t lane_slice_max;
+  uint32_t autotune;
 } dv_tuning;
 int dv_get_tuning(const dv_context *ctx, dv_tuning *t);
 int dv_set_tuning(dv_context *ctx, const dv_tuning *t);

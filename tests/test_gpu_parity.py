@@ -53,18 +53,38 @@ def test_spgemm_empty_rows_and_matrix(ctx):
     assert np.all(c.rowmap == 0)
 
 
-@pytest.mark.parametrize("lds_words,stage", [(512, 8), (1, 8), (2, 2), (3, 4), (64, 2)])
-def test_spgemm_window_chunks_and_stage(ctx, cfg1, lds_words, stage):
-    """Tiny LDS windows force the chunked path; tiny stages force the
-    direct-write recompute of most rows."""
-    ctx.set_tuning(lds_words=lds_words, stage_entries=stage)
+TIERS = [
+    # lds_words, stage, lanes_small, small_slice_max, lane_slice_max
+    (512, 8, 16, 64, 4096),   # defaults: lane tier + 16-lane + 64-lane groups
+    (512, 8, 16, 64, 16),     # lane tier up to 16 slice ranges
+    (1, 8, 16, 64, 16),       # one-word windows: every group row is chunked
+    (2, 2, 16, 64, 0),        # lane tier off, two-entry stage (most rows recomputed)
+    (3, 4, 4, 64, 0),         # 4-lane groups, 3-word windows
+    (64, 2, 8, 100000, 0),    # every row to 8-lane groups
+    (512, 8, 32, 100000, 0),  # every row to 32-lane groups
+    (512, 8, 16, 0, 0),       # every row to 64-lane groups
+    (8, 6, 64, 64, 3),        # small lane tier, 64-lane medium groups
+]
+
+
+@pytest.mark.parametrize("tier", TIERS, ids=[str(t) for t in TIERS])
+def test_spgemm_tiers_chunks_and_stage(ctx, cfg1, tier):
+    """Every tier, tiny LDS windows (chunked path) and tiny stages (direct
+    recompute) give the same product."""
+    lds, stage, lanes, mid_max, lane_max = tier
+    ctx.set_tuning(lds_words=lds, stage_entries=stage, lanes_small=lanes,
+                   small_slice_max=mid_max, lane_slice_max=lane_max)
     ai = orc.spadd(orc.identity(cfg1.nrows), cfg1)
     ai = dv.RangeCRS(ai.ncols, ai.entries, ai.rowmap, ai.nnz)
-    p = orc.power(ai, 37)
-    p = dv.RangeCRS(p.ncols, p.entries, p.rowmap, p.nnz)
-    print(f"spgemm chunks lds_words={lds_words} stage={stage}", flush=True)
-    c = ctx.range_spgemm(p, p)
-    assert_identical(c, orc.spgemm(p, p), "P.P")
+    print(f"spgemm tiers {tier}", flush=True)
+    for k in (1, 4, 37):
+        p = orc.power(ai, k)
+        p = dv.RangeCRS(p.ncols, p.entries, p.rowmap, p.nnz)
+        assert_identical(ctx.range_spgemm(p, p), orc.spgemm(p, p), f"P{k}.P{k} {tier}")
+    rng = np.random.default_rng(3)
+    da, db = random_dense(rng, 120, 900, 0.05, 30), random_dense(rng, 900, 2000, 0.02, 90)
+    a, b = from_dense(da), from_dense(db)
+    assert_identical(ctx.range_spgemm(a, b), orc.spgemm(a, b), f"random {tier}")
 
 
 @pytest.mark.parametrize("size_bytes", [4, 8])
