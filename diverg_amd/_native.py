@@ -38,6 +38,9 @@ class dv_build_stats(C.Structure):
         ("n_spgemm", C.c_uint32),
         ("n_spadd", C.c_uint32),
         ("overflow_rows", C.c_uint64),
+        ("deferred_rows", C.c_uint64),
+        ("planned", C.c_uint32),
+        ("retried", C.c_uint32),
     ]
 
 
@@ -50,6 +53,7 @@ class dv_tuning(C.Structure):
         ("small_slice_max", C.c_uint32),
         ("lane_slice_max", C.c_uint32),
         ("autotune", C.c_uint32),
+        ("plan_mode", C.c_uint32),
     ]
 
 

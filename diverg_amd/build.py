@@ -11,8 +11,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdiverg_amd.so")
 
-SOURCES = ["rcrs_kernels.hip", "dv_api.hip", "dv_batch.hip", "dv_server.hip",
-           "vgraph_gen.cpp"]
+SOURCES = ["rcrs_kernels.hip", "dv_api.hip", "dv_ops.hip", "dv_plan.hip", "dv_batch.hip",
+           "dv_server.hip", "vgraph_gen.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

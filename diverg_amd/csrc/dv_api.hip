@@ -66,12 +66,17 @@ void Ctx::release(DevMat &m) {
   m = DevMat{};
 }
 
-bool Ctx::read_counters(OpCounters &out) {
+bool Ctx::read_counters(OpCounters &out, uint32_t *status) {
   if (!hip(hipMemcpyAsync(h_ctr, d_ctr, sizeof(OpCounters), hipMemcpyDeviceToHost, stream),
            "counter readback"))
     return false;
+  if (status && !hip(hipMemcpyAsync(h_status, d_status, sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, stream),
+                     "status readback"))
+    return false;
   if (!hip(hipStreamSynchronize(stream), "stream synchronize")) return false;
   out = *h_ctr;
+  if (status) *status = *h_status;
   return true;
 }
 
@@ -301,257 +306,6 @@ void Ctx::forget(void *p) {
   }
 }
 
-int Ctx::identity_dev(uint64_t n, uint32_t sb, DevMat &out) {
-  out = DevMat{};
-  out.nrows = out.ncols = n;
-  out.n_entries = 2 * n;
-  out.nnz = n;  // range_sparse.hpp:108
-  out.sb = sb;
-  out.owned = true;
-  out.rowmap = dalloc(sb * (n + 1));
-  out.entries = static_cast<uint32_t *>(dalloc(4 * out.n_entries));
-  if (!out.rowmap || !out.entries) {
-    release(out);
-    return fail("device allocation failed (identity)", DV_ERR_NOMEM);
-  }
-  identity(stream, sb, n, out.rowmap, out.entries);
-  return hip(hipGetLastError(), "identity launch") ? DV_OK : DV_ERR_DEVICE;
-}
-
-int Ctx::spadd_dev(const DevMat &a, const DevMat &b, DevMat &c) {
-  if (a.nrows != b.nrows || a.ncols != b.ncols || a.sb != b.sb)
-    return fail("spadd: operands differ in shape or size type");
-  const uint64_t n = a.nrows;
-  c = DevMat{};
-  c.nrows = n;
-  c.ncols = a.ncols;
-  c.sb = a.sb;
-  c.owned = true;
-  void *buf[2];
-  const uint64_t sz[2] = {4 * (n ? n : 1), 8 * scan_tmp_words(n)};
-  const bool got = scratch_carve(sz, buf, 2);
-  auto *cnt = static_cast<uint32_t *>(buf[0]);
-  auto *tmp = static_cast<unsigned long long *>(buf[1]);
-  c.rowmap = dalloc(c.sb * (n + 1));
-  if (!got || !c.rowmap) {
-    release(c);
-    return fail("device allocation failed (spadd)", DV_ERR_NOMEM);
-  }
-  zero_counters(stream, d_ctr);
-  prof_mark(0);
-  spadd_count(stream, a.sb, view(a), view(b), cnt);
-  scan_counts(stream, c.sb, cnt, n, c.rowmap, tmp, d_ctr);
-  OpCounters h{};
-  if (!read_counters(h)) {
-    release(c);
-    return DV_ERR_DEVICE;
-  }
-  if (c.sb == 4 && h.total > 0xFFFFFFFFull) {
-    release(c);
-    return fail("spadd: entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
-  }
-  c.n_entries = h.total;
-  c.entries = static_cast<uint32_t *>(dalloc(4 * c.n_entries));
-  if (!c.entries) {
-    release(c);
-    return fail("device allocation failed (spadd entries)", DV_ERR_NOMEM);
-  }
-  spadd_write(stream, c.sb, view(a), view(b), c.rowmap, c.entries);
-  zero_counters(stream, d_ctr);
-  range_nnz(stream, c.entries, c.n_entries, d_ctr);
-  prof_mark(1);
-  if (!read_counters(h)) {
-    release(c);
-    return DV_ERR_DEVICE;
-  }
-  c.nnz = h.nnz;
-  prof_add(3, 0, 1, mat_bytes(a) + mat_bytes(b) + mat_bytes(c), 0);
-  stats_alg_bytes += mat_bytes(a) + mat_bytes(b) + mat_bytes(c);
-  stats_n_spadd += 1;
-  return DV_OK;
-}
-
-GroupLaunch Ctx::group_launch(uint32_t lanes, uint64_t nstage) const {
-  GroupLaunch g{};
-  g.lanes = lanes;
-  g.lds_words = std::max<uint32_t>(1, tuning.lds_words * lanes / 64);
-  g.stage = tuning.stage_entries;
-  g.rows_per_group = std::max<uint32_t>(1, tuning.rows_per_group);
-  g.grid = std::max<uint32_t>(8, (num_cu * 8 + 7) / 8 * 8);
-  g.nstage = nstage;
-  return g;
-}
-
-int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
-  if (a.ncols != b.nrows) return fail("spgemm: a.ncols != b.nrows");
-  if (a.sb != b.sb) return fail("spgemm: operands differ in size type");
-  if (a.ncols > 0xFFFFFFFEull || b.ncols > 0xFFFFFFFEull)
-    return fail("spgemm: ncols exceeds uint32_t max - 1", DV_ERR_OVERFLOW);
-  const uint64_t n = a.nrows;
-  c = DevMat{};
-  c.nrows = n;
-  c.ncols = b.ncols;
-  c.sb = a.sb;
-  c.owned = true;
-  const uint32_t KS = tuning.stage_entries;
-  const uint64_t n1 = n ? n : 1;
-  void *buf[6];
-  const uint64_t sz[6] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * scan_tmp_words(n)};
-  const bool got = scratch_carve(sz, buf, 6);
-  auto *cnt = static_cast<uint32_t *>(buf[0]);
-  auto *stage = static_cast<uint32_t *>(buf[1]);
-  auto *ovf = static_cast<uint32_t *>(buf[2]);
-  auto *mid = static_cast<uint32_t *>(buf[3]);
-  auto *big = static_cast<uint32_t *>(buf[4]);
-  auto *tmp = static_cast<unsigned long long *>(buf[5]);
-  c.rowmap = dalloc(c.sb * (n + 1));
-  if (!got || !c.rowmap) {
-    release(c);
-    return fail("device allocation failed (spgemm)", DV_ERR_NOMEM);
-  }
-  zero_counters(stream, d_ctr);
-  prof_mark(0);
-  // persistent lane-tier grid: few blocks keep the per-block counter atomics
-  // (nnz, slice ranges) uncontended
-  const uint32_t lane_grid =
-      (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)num_cu * 8));
-  spgemm_lane(stream, a.sb, view(a), view(b), KS, tuning.lane_slice_max, tuning.small_slice_max,
-              lane_grid, cnt, stage, ovf, mid, big, d_ctr);
-  if (n) {
-    GroupLaunch gm = group_launch(tuning.lanes_small, n);
-    gm.rows = mid;
-    gm.n_dev = &d_ctr->n_mid;
-    gm.cnt = cnt;
-    gm.stage_buf = stage;
-    gm.ovf_list = ovf;
-    gm.ctr = d_ctr;
-    spgemm_group(stream, a.sb, view(a), view(b), gm);
-    GroupLaunch gb = gm;
-    GroupLaunch g64 = group_launch(64, n);
-    gb.lanes = 64;
-    gb.lds_words = g64.lds_words;
-    gb.rows = big;
-    gb.n_dev = &d_ctr->n_big;
-    spgemm_group(stream, a.sb, view(a), view(b), gb);
-  }
-  prof_mark(1);
-  scan_counts(stream, c.sb, cnt, n, c.rowmap, tmp, d_ctr);
-  prof_mark(2);
-  OpCounters h{};
-  if (!hip(hipGetLastError(), "spgemm count launch") || !read_counters(h)) {
-    release(c);
-    return DV_ERR_DEVICE;
-  }
-  if (c.sb == 4 && h.total > 0xFFFFFFFFull) {
-    release(c);
-    return fail("spgemm: entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
-  }
-  c.n_entries = h.total;
-  c.nnz = h.nnz;
-  c.entries = static_cast<uint32_t *>(dalloc(4 * c.n_entries));
-  if (!c.entries) {
-    release(c);
-    return fail("device allocation failed (spgemm entries)", DV_ERR_NOMEM);
-  }
-  prof_mark(3);
-  spgemm_emit_staged(stream, c.sb, n, c.rowmap, stage, KS, c.entries);
-  if (h.overflow) {
-    GroupLaunch gd = group_launch(64, n);
-    gd.rows = ovf;
-    gd.n_host = h.overflow;
-    gd.c_rowmap = c.rowmap;
-    gd.c_entries = c.entries;
-    spgemm_group(stream, a.sb, view(a), view(b), gd);
-  }
-  prof_mark(4);
-  if (!hip(hipGetLastError(), "spgemm write launch")) {
-    release(c);
-    return DV_ERR_DEVICE;
-  }
-  {
-    const uint64_t alg = mat_bytes(a) + mat_bytes(b) + mat_bytes(c);
-    prof_add(0, 0, 1, alg, 16ull * h.slice);
-    prof_add(2, 1, 2, 0, 0);
-    prof_add(1, 3, 4, 0, 0);
-  }
-  stats_alg_bytes += mat_bytes(a) + mat_bytes(b) + mat_bytes(c);
-  stats_n_spgemm += 1;
-  stats_overflow_rows += h.overflow;
-  stats_mid_rows += h.n_mid;
-  stats_big_rows += h.n_big;
-  return DV_OK;
-}
-
-// range_sparse.hpp:2376-2431: C = I; loop { if (k & 1) C = C.A; k >>= 1;
-// if (k) A = A.A; else break; }.  `a` is not modified.
-int Ctx::power_dev(const DevMat &a_in, uint32_t k, DevMat &out) {
-  if (a_in.nrows != a_in.ncols) return fail("power: matrix is not square");
-  DevMat a = a_in;
-  a.owned = false;
-  DevMat cm;
-  int rc = identity_dev(a.ncols, a.sb, cm);
-  if (rc) return rc;
-  for (;;) {
-    if (k & 1) {
-      DevMat t;
-      rc = spgemm_dev(cm, a, t);
-      if (rc) break;
-      release(cm);
-      cm = t;
-    }
-    k >>= 1;
-    if (k != 0) {
-      DevMat t;
-      rc = spgemm_dev(a, a, t);
-      if (rc) break;
-      release(a);
-      a = t;
-    } else {
-      break;
-    }
-  }
-  release(a);
-  if (rc) {
-    release(cm);
-    return rc;
-  }
-  out = cm;
-  return DV_OK;
-}
-
-// dindex.hpp:46-189.
-int Ctx::dindex_dev(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
-  if (!(dlo <= dup && dup != 0)) return fail("create_distance_index: need dlo <= dup and dup != 0");
-  if (a.nrows != a.ncols) return fail("create_distance_index: matrix is not square");
-  DevMat ident, rai, rad, raid;
-  int rc = identity_dev(a.nrows, a.sb, ident);
-  if (rc) return rc;
-  rc = spadd_dev(ident, a, rai);
-  release(ident);
-  if (rc) return rc;
-  if (dlo != 0) {
-    rc = power_dev(a, dlo, rad);
-    if (rc) {
-      release(rai);
-      return rc;
-    }
-  }
-  rc = power_dev(rai, dup - dlo, raid);
-  release(rai);
-  if (rc) {
-    release(rad);
-    return rc;
-  }
-  if (dlo != 0) {
-    rc = spgemm_dev(rad, raid, d);
-    release(rad);
-    release(raid);
-    return rc;
-  }
-  d = raid;
-  return DV_OK;
-}
-
 }  // namespace dv
 
 // ---------------------------------------------------------------------------
@@ -609,7 +363,9 @@ int dv_init(int device, dv_context **out) {
             hipHostMalloc((void **)&c->h_ctr, sizeof(dv::OpCounters), hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc((void **)&c->h_flag, sizeof(uint32_t), hipHostMallocDefault) == hipSuccess &&
             hipMalloc((void **)&c->d_ctr, sizeof(dv::OpCounters)) == hipSuccess &&
-            hipMalloc((void **)&c->d_flag, sizeof(uint32_t)) == hipSuccess;
+            hipMalloc((void **)&c->d_flag, sizeof(uint32_t)) == hipSuccess &&
+            hipHostMalloc((void **)&c->h_status, sizeof(uint32_t), hipHostMallocDefault) == hipSuccess &&
+            hipMalloc((void **)&c->d_status, sizeof(uint32_t)) == hipSuccess;
   if (!ok) {
     (void)hipGetLastError();
     dv_finalize(c);
@@ -624,12 +380,18 @@ int dv_finalize(dv_context *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &kv : c->sizes) (void)hipFreeAsync(kv.first, c->stream);
   c->sizes.clear();
+  c->bufs.clear();
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c->pev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->plan_ev) (void)hipEventDestroy(e);
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+  if (c->h_status) (void)hipHostFree(c->h_status);
+  if (c->d_status) (void)hipFree(c->d_status);
+  if (c->h_plan_ctr) (void)hipHostFree(c->h_plan_ctr);
+  if (c->d_plan_ctr) (void)hipFree(c->d_plan_ctr);
   if (c->h_flag) (void)hipHostFree(c->h_flag);
   if (c->d_ctr) (void)hipFree(c->d_ctr);
   if (c->d_flag) (void)hipFree(c->d_flag);
@@ -661,6 +423,7 @@ int dv_set_tuning(dv_context *c, const dv_tuning *t) {
     return C(c)->fail("stage_entries must be even, 2 .. 64");
   if (n.rows_per_group < 1) return C(c)->fail("rows_per_group must be >= 1");
   if (n.lane_slice_max > (1u << 20)) return C(c)->fail("lane_slice_max must be 0 .. 2^20");
+  if (n.plan_mode > 2) return C(c)->fail("plan_mode must be 0, 1 or 2");
   c->tuning = n;
   return DV_OK;
 }
@@ -800,7 +563,8 @@ int dv_create_distance_index(dv_context *ctx, const dv_rcrs *a, uint32_t dlo, ui
   DevMat da, dd;
   if ((rc = to_dev(c, a, da, &st.h2d_ms))) return rc;
   c->reset_stats();
-  const uint64_t base_bytes = c->cur_bytes;
+  // an input resident in caller memory is not in the context's accounting
+  const uint64_t input_bytes = a->location == DV_DEVICE ? dv::mat_bytes(da) : 0;
   c->peak_bytes = c->cur_bytes;
   (void)hipEventRecord(c->ev[2], c->stream);
   rc = c->dindex_dev(da, dlo, dup, dd);
@@ -815,7 +579,12 @@ int dv_create_distance_index(dv_context *ctx, const dv_rcrs *a, uint32_t dlo, ui
   st.n_spgemm = c->stats_n_spgemm;
   st.n_spadd = c->stats_n_spadd;
   st.overflow_rows = c->stats_overflow_rows;
-  st.peak_device_bytes = c->peak_bytes - std::min(base_bytes, c->peak_bytes);
+  st.deferred_rows = c->stats_mid_rows + c->stats_big_rows;
+  st.planned = c->stats_planned;
+  st.retried = c->stats_retried;
+  // everything the context holds at the high-water mark of the build: the
+  // resident input, cached plan buffers, scratch and the output
+  st.peak_device_bytes = c->peak_bytes + input_bytes;
   rc = export_mat(c, dd, loc, out, &st.d2h_ms);
   if (stats) *stats = st;
   return rc;

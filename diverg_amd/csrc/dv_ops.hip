@@ -1,0 +1,304 @@
+// Device-side operations of the library: identity, range SpAdd and range
+// SpGEMM as enqueue-only halves (count + row-pointer scan, then write), and
+// the exact forms that read the entry count back between the halves, as the
+// reference's view-level drivers do (range_sparse.hpp:364-428, 2215-2281).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "dv_runtime.hpp"
+
+namespace dv {
+
+int Ctx::identity_dev(uint64_t n, uint32_t sb, DevMat &out) {
+  out = DevMat{};
+  out.nrows = out.ncols = n;
+  out.n_entries = 2 * n;
+  out.nnz = n;  // range_sparse.hpp:108
+  out.sb = sb;
+  out.owned = true;
+  out.rowmap = dalloc(sb * (n + 1));
+  out.entries = static_cast<uint32_t *>(dalloc(4 * out.n_entries));
+  if (!out.rowmap || !out.entries) {
+    release(out);
+    return fail("device allocation failed (identity)", DV_ERR_NOMEM);
+  }
+  identity(stream, sb, n, out.rowmap, out.entries);
+  return hip(hipGetLastError(), "identity launch") ? DV_OK : DV_ERR_DEVICE;
+}
+
+bool Ctx::op_scratch(uint64_t n, OpScratch &s) {
+  const uint64_t n1 = n ? n : 1;
+  const uint32_t KS = tuning.stage_entries;
+  void *buf[6];
+  const uint64_t sz[6] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * scan_tmp_words(n)};
+  if (!scratch_carve(sz, buf, 6)) return false;
+  s.cnt = static_cast<uint32_t *>(buf[0]);
+  s.stage = static_cast<uint32_t *>(buf[1]);
+  s.ovf = static_cast<uint32_t *>(buf[2]);
+  s.mid = static_cast<uint32_t *>(buf[3]);
+  s.big = static_cast<uint32_t *>(buf[4]);
+  s.tmp = static_cast<unsigned long long *>(buf[5]);
+  s.stage_entries = KS;
+  return true;
+}
+
+GroupLaunch Ctx::group_launch(uint32_t lanes, uint64_t nstage) const {
+  GroupLaunch g{};
+  g.lanes = lanes;
+  g.lds_words = std::max<uint32_t>(1, tuning.lds_words * lanes / 64);
+  g.stage = tuning.stage_entries;
+  g.rows_per_group = std::max<uint32_t>(1, tuning.rows_per_group);
+  g.grid = std::max<uint32_t>(8, (num_cu * 8 + 7) / 8 * 8);
+  g.nstage = nstage;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// SpAdd halves (range_sparse.hpp:112-357)
+void Ctx::enq_spadd_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
+                          const OpScratch &s, OpCounters *ctr, uint64_t cap,
+                          unsigned int *status) {
+  spadd_count(stream, a.sb, view(a), view(b), s.cnt, status);
+  scan_counts(stream, c_sb, s.cnt, a.nrows, c_rowmap, s.tmp, ctr, cap, status);
+}
+
+void Ctx::enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
+                          uint32_t *c_entries, uint64_t cap, OpCounters *ctr,
+                          const unsigned int *status) {
+  spadd_write(stream, a.sb, view(a), view(b), c_rowmap, c_entries, status);
+  range_nnz(stream, c_entries, cap, ctr, status);
+}
+
+// ---------------------------------------------------------------------------
+// SpGEMM halves: the lane tier, the two group tiers over the rows it hands
+// over, the row-pointer scan; then the staged copy and the direct rewrite of
+// rows whose output exceeds the stage.
+void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
+                           const OpScratch &s, OpCounters *ctr, uint64_t cap,
+                           unsigned int *status) {
+  const uint64_t n = a.nrows;
+  if (n) {
+    // persistent lane-tier grid: few blocks keep the per-block counter atomics
+    // (nnz, slice ranges) uncontended
+    const uint32_t lane_grid =
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)num_cu * 8));
+    spgemm_lane(stream, a.sb, view(a), view(b), s.stage_entries, tuning.lane_slice_max,
+                tuning.small_slice_max, lane_grid, s.cnt, s.stage, s.ovf, s.mid, s.big, ctr,
+                status);
+    GroupLaunch g = group_launch(tuning.lanes_small, n);
+    g.rows = s.mid;
+    g.n_dev = &ctr->n_mid;
+    g.cnt = s.cnt;
+    g.stage_buf = s.stage;
+    g.ovf_list = s.ovf;
+    g.ctr = ctr;
+    g.status = status;
+    spgemm_group(stream, a.sb, view(a), view(b), g);
+    g.lanes = 64;
+    g.lds_words = group_launch(64, n).lds_words;
+    g.rows = s.big;
+    g.n_dev = &ctr->n_big;
+    spgemm_group(stream, a.sb, view(a), view(b), g);
+  }
+  scan_counts(stream, c_sb, s.cnt, n, c_rowmap, s.tmp, ctr, cap, status);
+}
+
+void Ctx::enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
+                           uint32_t *c_entries, const OpScratch &s, OpCounters *ctr,
+                           const unsigned int *status) {
+  const uint64_t n = a.nrows;
+  if (!n) return;
+  spgemm_emit_staged(stream, a.sb, n, c_rowmap, s.stage, s.stage_entries, c_entries, status);
+  GroupLaunch g = group_launch(64, n);
+  g.rows = s.ovf;
+  g.n_dev = &ctr->overflow;
+  g.c_rowmap = c_rowmap;
+  g.c_entries = c_entries;
+  g.status = status;
+  spgemm_group(stream, a.sb, view(a), view(b), g);
+}
+
+// ---------------------------------------------------------------------------
+// exact forms: one readback between the halves sizes C.entries
+int Ctx::spadd_dev(const DevMat &a, const DevMat &b, DevMat &c) {
+  if (a.nrows != b.nrows || a.ncols != b.ncols || a.sb != b.sb)
+    return fail("spadd: operands differ in shape or size type");
+  const uint64_t n = a.nrows;
+  c = DevMat{};
+  c.nrows = n;
+  c.ncols = a.ncols;
+  c.sb = a.sb;
+  c.owned = true;
+  OpScratch s;
+  const bool got = op_scratch(n, s);
+  c.rowmap = dalloc(c.sb * (n + 1));
+  if (!got || !c.rowmap) {
+    release(c);
+    return fail("device allocation failed (spadd)", DV_ERR_NOMEM);
+  }
+  zero_counters(stream, d_ctr);
+  fill_u32(stream, d_status, 1, 0u);
+  prof_mark(0);
+  enq_spadd_count(a, b, c.rowmap, c.sb, s, d_ctr, ~0ull, d_status);
+  OpCounters h{};
+  uint32_t st = 0;
+  if (!read_counters(h, &st)) {
+    release(c);
+    return DV_ERR_DEVICE;
+  }
+  if (st & 2) {
+    release(c);
+    return fail("spadd: entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
+  }
+  c.n_entries = h.total;
+  c.entries = static_cast<uint32_t *>(dalloc(4 * c.n_entries));
+  if (!c.entries) {
+    release(c);
+    return fail("device allocation failed (spadd entries)", DV_ERR_NOMEM);
+  }
+  enq_spadd_write(a, b, c.rowmap, c.entries, c.n_entries, d_ctr, d_status);
+  prof_mark(1);
+  if (!read_counters(h, &st)) {
+    release(c);
+    return DV_ERR_DEVICE;
+  }
+  c.nnz = h.nnz;
+  prof_add(3, 0, 1, mat_bytes(a) + mat_bytes(b) + mat_bytes(c), 0);
+  stats_alg_bytes += mat_bytes(a) + mat_bytes(b) + mat_bytes(c);
+  stats_n_spadd += 1;
+  return DV_OK;
+}
+
+int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
+  if (a.ncols != b.nrows) return fail("spgemm: a.ncols != b.nrows");
+  if (a.sb != b.sb) return fail("spgemm: operands differ in size type");
+  if (a.ncols > 0xFFFFFFFEull || b.ncols > 0xFFFFFFFEull)
+    return fail("spgemm: ncols exceeds uint32_t max - 1", DV_ERR_OVERFLOW);
+  const uint64_t n = a.nrows;
+  c = DevMat{};
+  c.nrows = n;
+  c.ncols = b.ncols;
+  c.sb = a.sb;
+  c.owned = true;
+  OpScratch s;
+  const bool got = op_scratch(n, s);
+  c.rowmap = dalloc(c.sb * (n + 1));
+  if (!got || !c.rowmap) {
+    release(c);
+    return fail("device allocation failed (spgemm)", DV_ERR_NOMEM);
+  }
+  zero_counters(stream, d_ctr);
+  fill_u32(stream, d_status, 1, 0u);
+  prof_mark(0);
+  enq_spgemm_count(a, b, c.rowmap, c.sb, s, d_ctr, ~0ull, d_status);
+  prof_mark(2);
+  OpCounters h{};
+  uint32_t st = 0;
+  if (!hip(hipGetLastError(), "spgemm count launch") || !read_counters(h, &st)) {
+    release(c);
+    return DV_ERR_DEVICE;
+  }
+  if (st & 2) {
+    release(c);
+    return fail("spgemm: entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
+  }
+  c.n_entries = h.total;
+  c.nnz = h.nnz;
+  c.entries = static_cast<uint32_t *>(dalloc(4 * c.n_entries));
+  if (!c.entries) {
+    release(c);
+    return fail("device allocation failed (spgemm entries)", DV_ERR_NOMEM);
+  }
+  prof_mark(3);
+  enq_spgemm_write(a, b, c.rowmap, c.entries, s, d_ctr, d_status);
+  prof_mark(4);
+  if (!hip(hipGetLastError(), "spgemm write launch")) {
+    release(c);
+    return DV_ERR_DEVICE;
+  }
+  {
+    const uint64_t alg = mat_bytes(a) + mat_bytes(b) + mat_bytes(c);
+    prof_add(0, 0, 2, alg, 16ull * h.slice);
+    prof_add(1, 3, 4, 0, 0);
+  }
+  stats_alg_bytes += mat_bytes(a) + mat_bytes(b) + mat_bytes(c);
+  stats_n_spgemm += 1;
+  stats_overflow_rows += h.overflow;
+  stats_mid_rows += h.n_mid;
+  stats_big_rows += h.n_big;
+  if (n) stats_max_ratio = std::max(stats_max_ratio, (double)c.n_entries / (2.0 * (double)n));
+  return DV_OK;
+}
+
+// range_sparse.hpp:2376-2431: C = I; loop { if (k & 1) C = C.A; k >>= 1;
+// if (k) A = A.A; else break; }.  `a` is not modified.
+int Ctx::power_exact(const DevMat &a_in, uint32_t k, DevMat &out) {
+  if (a_in.nrows != a_in.ncols) return fail("power: matrix is not square");
+  DevMat a = a_in;
+  a.owned = false;
+  DevMat cm;
+  int rc = identity_dev(a.ncols, a.sb, cm);
+  if (rc) return rc;
+  for (;;) {
+    if (k & 1) {
+      DevMat t;
+      rc = spgemm_dev(cm, a, t);
+      if (rc) break;
+      release(cm);
+      cm = t;
+    }
+    k >>= 1;
+    if (k != 0) {
+      DevMat t;
+      rc = spgemm_dev(a, a, t);
+      if (rc) break;
+      release(a);
+      a = t;
+    } else {
+      break;
+    }
+  }
+  release(a);
+  if (rc) {
+    release(cm);
+    return rc;
+  }
+  out = cm;
+  return DV_OK;
+}
+
+// dindex.hpp:46-189 with one readback per operation.
+int Ctx::dindex_exact(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
+  if (!(dlo <= dup && dup != 0)) return fail("create_distance_index: need dlo <= dup and dup != 0");
+  if (a.nrows != a.ncols) return fail("create_distance_index: matrix is not square");
+  DevMat ident, rai, rad, raid;
+  int rc = identity_dev(a.nrows, a.sb, ident);
+  if (rc) return rc;
+  rc = spadd_dev(ident, a, rai);
+  release(ident);
+  if (rc) return rc;
+  if (dlo != 0) {
+    rc = power_exact(a, dlo, rad);
+    if (rc) {
+      release(rai);
+      return rc;
+    }
+  }
+  rc = power_exact(rai, dup - dlo, raid);
+  release(rai);
+  if (rc) {
+    release(rad);
+    return rc;
+  }
+  if (dlo != 0) {
+    rc = spgemm_dev(rad, raid, d);
+    release(rad);
+    release(raid);
+    return rc;
+  }
+  d = raid;
+  return DV_OK;
+}
+
+}  // namespace dv

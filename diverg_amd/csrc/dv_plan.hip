@@ -1,0 +1,377 @@
+// Planned builds: a distance-index build (or a power) is laid out on the host
+// as a fixed sequence of operations over matrix slots, every SpGEMM output
+// gets an entries buffer of a predicted capacity, and the whole sequence is
+// enqueued on the stream with no host round trip.  The scans flag a capacity
+// overflow on the device; every later kernel that reads operands then does
+// nothing, and after the single readback the build is repeated in exact
+// (read-back-per-operation) form with a larger prediction.
+//
+// Exponentiation order and results follow range_power_inplace
+// (range_sparse.hpp:2376-2431) and create_distance_index (dindex.hpp:46-189).
+// One deviation with no effect on any output array: the first product of a
+// power chain, I.P, is not computed when P is itself an SpGEMM output (which
+// is canonical, so I.P == P element for element); C starts as P.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "dv_runtime.hpp"
+
+namespace dv {
+
+int Plan::add_slot(uint64_t nrows, uint64_t ncols, uint32_t sb) {
+  PlanSlot s;
+  s.nrows = nrows;
+  s.ncols = ncols;
+  s.sb = sb;
+  slots.push_back(s);
+  return (int)slots.size() - 1;
+}
+
+int Plan::external(const DevMat &m) {
+  const int s = add_slot(m.nrows, m.ncols, m.sb);
+  slots[s].rowmap = m.rowmap;
+  slots[s].entries = m.entries;
+  slots[s].cap = m.n_entries;
+  slots[s].n_entries = m.n_entries;
+  slots[s].nnz = m.nnz;
+  slots[s].external = true;
+  return s;
+}
+
+int Plan::op(PlanKind kind, int a, int b, uint64_t nrows, uint64_t ncols, uint32_t sb) {
+  const int c = add_slot(nrows, ncols, sb);
+  slots[c].producer = (int)ops.size();
+  slots[c].kind = kind;
+  ops.push_back(PlanOp{kind, a, b, c});
+  if (a >= 0) slots[a].last_use = (int)ops.size() - 1;
+  if (b >= 0) slots[b].last_use = (int)ops.size() - 1;
+  return c;
+}
+
+int Plan::identity(uint64_t n, uint32_t sb) { return op(P_IDENT, -1, -1, n, n, sb); }
+
+int Plan::spadd(int a, int b) {
+  return op(P_SPADD, a, b, slots[a].nrows, slots[a].ncols, slots[a].sb);
+}
+
+int Plan::spgemm(int a, int b) {
+  return op(P_SPGEMM, a, b, slots[a].nrows, slots[b].ncols, slots[a].sb);
+}
+
+// C = P^k by binary exponentiation (range_sparse.hpp:2376-2431).
+int Plan::power(int p, uint32_t k) {
+  const uint64_t n = slots[p].ncols;
+  const uint32_t sb = slots[p].sb;
+  if (k == 0) return identity(n, sb);
+  int c = -1;
+  for (;;) {
+    if (k & 1) {
+      if (c >= 0)
+        c = spgemm(c, p);
+      else if (slots[p].kind == P_SPGEMM)
+        c = p;  // I.P == P for a canonical SpGEMM output
+      else
+        c = spgemm(identity(n, sb), p);
+    }
+    k >>= 1;
+    if (k == 0) break;
+    p = spgemm(p, p);
+  }
+  return c;
+}
+
+// dindex.hpp:46-189.
+int Plan::dindex(int a, uint32_t dlo, uint32_t dup) {
+  const uint64_t n = slots[a].nrows;
+  const uint32_t sb = slots[a].sb;
+  const int rai = spadd(identity(n, sb), a);
+  const int rad = dlo != 0 ? power(a, dlo) : -1;
+  const int raid = power(rai, dup - dlo);
+  if (dlo == 0) return raid;
+  return spgemm(rad, raid);
+}
+
+// Pull a buffer pair from the context's cache (best fit) or allocate one.
+int Ctx::buf_acquire(uint64_t rm_bytes, uint64_t cap) {
+  int best = -1;
+  for (int i = 0; i < (int)bufs.size(); ++i) {
+    const PlanBuf &b = bufs[i];
+    if (b.in_use || b.rm_bytes < rm_bytes || b.cap < cap) continue;
+    if (best < 0 || b.rm_bytes + 4 * b.cap < bufs[best].rm_bytes + 4 * bufs[best].cap) best = i;
+  }
+  if (best >= 0) {
+    bufs[best].in_use = true;
+    return best;
+  }
+  PlanBuf b;
+  b.rm_bytes = rm_bytes;
+  b.cap = std::max<uint64_t>(cap, 2);
+  b.rowmap = dalloc(rm_bytes);
+  b.entries = static_cast<uint32_t *>(dalloc(4 * b.cap));
+  if (!b.rowmap || !b.entries) {
+    dfree(b.rowmap);
+    dfree(b.entries);
+    return -1;
+  }
+  b.in_use = true;
+  bufs.push_back(b);
+  return (int)bufs.size() - 1;
+}
+
+void Ctx::buf_trim() {
+  std::vector<PlanBuf> keep;
+  for (const PlanBuf &b : bufs) {
+    if (b.in_use) {
+      keep.push_back(b);
+    } else {
+      dfree(b.rowmap);
+      dfree(b.entries);
+    }
+  }
+  bufs = keep;
+}
+
+uint64_t Ctx::predict_cap(const Plan &p, const PlanSlot &s) const {
+  switch (s.kind) {
+    case P_IDENT:
+      return 2 * s.nrows;
+    case P_SPADD:  // a merge never has more entries than its two inputs
+      return p.slots[p.ops[s.producer].a].cap + p.slots[p.ops[s.producer].b].cap;
+    default: {
+      if (tuning.plan_mode == 2) return 1;  // forces the exact fallback
+      const double f = std::max(1.0, plan_factor);
+      return (uint64_t)std::ceil(2.0 * f * (double)s.nrows) + 64;
+    }
+  }
+}
+
+// Enqueue every operation of the plan; no host synchronisation.
+int Ctx::plan_enqueue(Plan &p) {
+  const int nops = (int)p.ops.size();
+  if (!plan_ctr_reserve(nops)) return fail("device allocation failed (plan counters)", DV_ERR_NOMEM);
+  uint64_t nmax = 0;
+  for (const PlanOp &o : p.ops) nmax = std::max(nmax, p.slots[o.c].nrows);
+  OpScratch s;
+  if (!op_scratch(nmax, s)) return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
+  fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
+           (uint32_t)(nops * sizeof(OpCounters) / 4), 0u);
+  fill_u32(stream, d_status, 1, 0u);
+  const bool timing = prof && plan_events_reserve(nops);
+  for (int i = 0; i < nops; ++i) {
+    const PlanOp &o = p.ops[i];
+    PlanSlot &c = p.slots[o.c];
+    c.cap = predict_cap(p, c);
+    c.buf = buf_acquire((uint64_t)c.sb * (c.nrows + 1), c.cap);
+    if (c.buf < 0) return fail("device allocation failed (plan buffers)", DV_ERR_NOMEM);
+    c.rowmap = bufs[c.buf].rowmap;
+    c.entries = bufs[c.buf].entries;
+    // c.cap stays the prediction (a reused buffer may be larger)
+    OpCounters *ctr = d_plan_ctr + i;
+    if (timing) (void)hipEventRecord(plan_ev[4 * i], stream);
+    if (o.kind == P_IDENT) {
+      identity(stream, c.sb, c.nrows, c.rowmap, c.entries);
+      c.n_entries = 2 * c.nrows;
+      c.nnz = c.nrows;
+      if (timing)
+        for (int e = 1; e < 4; ++e) (void)hipEventRecord(plan_ev[4 * i + e], stream);
+    } else {
+      const DevMat a = p.slots[o.a].as_mat(), b = p.slots[o.b].as_mat();
+      if (o.kind == P_SPADD) {
+        enq_spadd_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status);
+        if (timing) (void)hipEventRecord(plan_ev[4 * i + 1], stream);
+        if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
+        enq_spadd_write(a, b, c.rowmap, c.entries, c.cap, ctr, d_status);
+      } else {
+        enq_spgemm_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status);
+        if (timing) (void)hipEventRecord(plan_ev[4 * i + 1], stream);
+        if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
+        enq_spgemm_write(a, b, c.rowmap, c.entries, s, ctr, d_status);
+      }
+      if (timing) (void)hipEventRecord(plan_ev[4 * i + 3], stream);
+    }
+    // buffers whose last reader is this operation go back to the cache
+    for (PlanSlot &x : p.slots)
+      if (!x.external && x.buf >= 0 && x.last_use == i && &x != &p.slots[p.out]) {
+        bufs[x.buf].in_use = false;
+        x.released = true;
+      }
+  }
+  return hip(hipGetLastError(), "plan launch") ? DV_OK : DV_ERR_DEVICE;
+}
+
+// Read the plan's counters back (one synchronisation) and fill in the slots.
+int Ctx::plan_finish(Plan &p, uint32_t &status_out) {
+  const int nops = (int)p.ops.size();
+  bool ok = hip(hipMemcpyAsync(h_plan_ctr, d_plan_ctr, nops * sizeof(OpCounters),
+                               hipMemcpyDeviceToHost, stream),
+                "plan readback") &&
+            hip(hipMemcpyAsync(h_status, d_status, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               stream),
+                "plan status") &&
+            hip(hipStreamSynchronize(stream), "plan synchronize");
+  if (!ok) return DV_ERR_DEVICE;
+  status_out = *h_status;
+  const bool timing = prof && (int)plan_ev.size() >= 4 * nops;
+  for (int i = 0; i < nops; ++i) {
+    const PlanOp &o = p.ops[i];
+    PlanSlot &c = p.slots[o.c];
+    const OpCounters &h = h_plan_ctr[i];
+    if (o.kind != P_IDENT) {
+      c.n_entries = h.total;
+      c.nnz = h.nnz;
+      const uint64_t alg = p.slots[o.a].bytes() + p.slots[o.b].bytes() + c.bytes();
+      stats_alg_bytes += alg;
+      if (o.kind == P_SPGEMM) {
+        stats_n_spgemm += 1;
+        stats_overflow_rows += h.overflow;
+        stats_mid_rows += h.n_mid;
+        stats_big_rows += h.n_big;
+        if (c.nrows)
+          stats_max_ratio = std::max(stats_max_ratio, (double)c.n_entries / (2.0 * (double)c.nrows));
+      } else {
+        stats_n_spadd += 1;
+      }
+      if (timing && !status_out) {
+        float t0 = 0, t1 = 0, t2 = 0;
+        (void)hipEventElapsedTime(&t0, plan_ev[4 * i], plan_ev[4 * i + 1]);
+        (void)hipEventElapsedTime(&t1, plan_ev[4 * i + 1], plan_ev[4 * i + 2]);
+        (void)hipEventElapsedTime(&t2, plan_ev[4 * i + 2], plan_ev[4 * i + 3]);
+        if (o.kind == P_SPGEMM) {
+          prof_t[0].launches += 1;
+          prof_t[0].total_ms += t0;
+          prof_t[0].alg_bytes += alg;
+          prof_t[0].slice_bytes += 16ull * h.slice;
+          prof_t[1].launches += 1;
+          prof_t[1].total_ms += t2;
+        } else {
+          prof_t[3].launches += 1;
+          prof_t[3].total_ms += t0 + t1 + t2;
+          prof_t[3].alg_bytes += alg;
+        }
+      }
+    }
+  }
+  return DV_OK;
+}
+
+// Run a plan; on a capacity overflow fall back to `exact` and learn a larger
+// capacity prediction.  `out` receives a copy of the output slot in
+// exactly sized buffers.
+int Ctx::plan_run(Plan &p, DevMat &out, const std::function<int(DevMat &)> &exact) {
+  reset_stats();
+  int rc = plan_enqueue(p);
+  uint32_t st = 0;
+  if (!rc) rc = plan_finish(p, st);
+  // whatever happens, all plan buffers return to the cache
+  auto release_all = [&]() {
+    for (PlanSlot &x : p.slots)
+      if (!x.external && x.buf >= 0 && !x.released) {
+        bufs[x.buf].in_use = false;
+        x.released = true;
+      }
+  };
+  if (rc) {
+    release_all();
+    return rc;
+  }
+  if (st & 2) {
+    release_all();
+    return fail("entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
+  }
+  if (st & 1) {
+    release_all();
+    plan_retries += 1;
+    reset_stats();
+    stats_max_ratio = 0;
+    stats_retried = 1;
+    rc = exact(out);
+    if (!rc && tuning.plan_mode != 2) plan_factor = std::max(plan_factor, stats_max_ratio * 1.25);
+    return rc;
+  }
+  stats_planned = 1;
+  const PlanSlot &o = p.slots[p.out];
+  out = DevMat{};
+  out.nrows = o.nrows;
+  out.ncols = o.ncols;
+  out.sb = o.sb;
+  out.n_entries = o.n_entries;
+  out.nnz = o.nnz;
+  out.owned = true;
+  out.rowmap = dalloc(o.sb * (o.nrows + 1));
+  out.entries = static_cast<uint32_t *>(dalloc(4 * o.n_entries));
+  if (!out.rowmap || !out.entries) {
+    release_all();
+    release(out);
+    return fail("device allocation failed (output)", DV_ERR_NOMEM);
+  }
+  bool ok = hip(hipMemcpyAsync(out.rowmap, o.rowmap, o.sb * (o.nrows + 1),
+                               hipMemcpyDeviceToDevice, stream),
+                "output rowmap copy") &&
+            (o.n_entries == 0 ||
+             hip(hipMemcpyAsync(out.entries, o.entries, 4 * o.n_entries,
+                                hipMemcpyDeviceToDevice, stream),
+                 "output entries copy"));
+  release_all();
+  if (!ok) {
+    release(out);
+    return DV_ERR_DEVICE;
+  }
+  return DV_OK;
+}
+
+bool Ctx::plan_ctr_reserve(int nops) {
+  if (nops <= plan_ctr_cap) return true;
+  if (d_plan_ctr) (void)hipFree(d_plan_ctr);
+  if (h_plan_ctr) (void)hipHostFree(h_plan_ctr);
+  d_plan_ctr = nullptr;
+  h_plan_ctr = nullptr;
+  plan_ctr_cap = 0;
+  const int want = std::max(64, nops);
+  if (hipMalloc((void **)&d_plan_ctr, want * sizeof(OpCounters)) != hipSuccess ||
+      hipHostMalloc((void **)&h_plan_ctr, want * sizeof(OpCounters), hipHostMallocDefault) !=
+          hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  plan_ctr_cap = want;
+  return true;
+}
+
+bool Ctx::plan_events_reserve(int nops) {
+  while ((int)plan_ev.size() < 4 * nops) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    plan_ev.push_back(e);
+  }
+  return true;
+}
+
+int Ctx::dindex_dev(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
+  if (!(dlo <= dup && dup != 0)) return fail("create_distance_index: need dlo <= dup and dup != 0");
+  if (a.nrows != a.ncols) return fail("create_distance_index: matrix is not square");
+  if (tuning.plan_mode == 1) {
+    reset_stats();
+    return dindex_exact(a, dlo, dup, d);
+  }
+  Plan p;
+  p.out = p.dindex(p.external(a), dlo, dup);
+  return plan_run(p, d, [&](DevMat &o) { return dindex_exact(a, dlo, dup, o); });
+}
+
+int Ctx::power_dev(const DevMat &a, uint32_t k, DevMat &out) {
+  if (a.nrows != a.ncols) return fail("power: matrix is not square");
+  if (tuning.plan_mode == 1) {
+    reset_stats();
+    return power_exact(a, k, out);
+  }
+  Plan p;
+  p.out = p.power(p.external(a), k);
+  return plan_run(p, out, [&](DevMat &o) { return power_exact(a, k, o); });
+}
+
+}  // namespace dv

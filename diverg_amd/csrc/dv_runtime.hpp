@@ -4,8 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 #include "dv_internal.hpp"
 #include "rcrs_kernels.hpp"
@@ -26,13 +28,83 @@ struct DevMat {
 uint64_t mat_bytes(const DevMat &m);
 DView view(const DevMat &m);
 
+// Scratch of one SpAdd / SpGEMM (carved from the context's arena).
+struct OpScratch {
+  uint32_t *cnt = nullptr;    // per-row entry counts
+  uint32_t *stage = nullptr;  // staged entries, column-major
+  uint32_t *ovf = nullptr;    // rows whose output exceeds the stage
+  uint32_t *mid = nullptr;    // rows for the medium groups
+  uint32_t *big = nullptr;    // rows for the 64-lane groups
+  unsigned long long *tmp = nullptr;  // scan tile sums
+  uint32_t stage_entries = 0;
+};
+
+// ---------------------------------------------------------------------------
+// planned builds (dv_plan.hip)
+enum PlanKind { P_EXT, P_IDENT, P_SPADD, P_SPGEMM };
+
+struct PlanSlot {
+  uint64_t nrows = 0, ncols = 0;
+  uint32_t sb = 8;
+  PlanKind kind = P_EXT;
+  void *rowmap = nullptr;
+  uint32_t *entries = nullptr;
+  uint64_t cap = 0;         // entries capacity
+  uint64_t n_entries = 0;   // known after the readback (static for I and inputs)
+  uint64_t nnz = 0;
+  int producer = -1;        // index of the op writing the slot
+  int last_use = -1;        // index of the last op reading the slot
+  int buf = -1;             // context buffer backing the slot
+  bool external = false;    // caller memory
+  bool released = false;
+  DevMat as_mat() const {
+    DevMat m;
+    m.nrows = nrows;
+    m.ncols = ncols;
+    m.sb = sb;
+    m.rowmap = rowmap;
+    m.entries = entries;
+    m.n_entries = n_entries;
+    m.nnz = nnz;
+    return m;
+  }
+  uint64_t bytes() const { return (uint64_t)sb * (nrows + 1) + 4 * n_entries; }
+};
+
+struct PlanOp {
+  PlanKind kind;
+  int a, b, c;
+};
+
+struct Plan {
+  std::vector<PlanSlot> slots;
+  std::vector<PlanOp> ops;
+  int out = -1;
+  int add_slot(uint64_t nrows, uint64_t ncols, uint32_t sb);
+  int external(const DevMat &m);
+  int op(PlanKind kind, int a, int b, uint64_t nrows, uint64_t ncols, uint32_t sb);
+  int identity(uint64_t n, uint32_t sb);
+  int spadd(int a, int b);
+  int spgemm(int a, int b);
+  int power(int p, uint32_t k);
+  int dindex(int a, uint32_t dlo, uint32_t dup);
+};
+
+// A rowmap + entries buffer pair kept by the context across builds.
+struct PlanBuf {
+  void *rowmap = nullptr;
+  uint32_t *entries = nullptr;
+  uint64_t rm_bytes = 0, cap = 0;
+  bool in_use = false;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t num_cu = 256;
   uint32_t lds_per_block = 65536;
-  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0};
+  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0};
   std::string last_error;
 
   // pool accounting
@@ -43,12 +115,21 @@ struct Ctx {
   void *scratch = nullptr;
   uint64_t scratch_cap = 0;
   // Carve `n` buffers of the given byte sizes (256-byte aligned) from the
-  // arena, growing it when needed; nullptr on allocation failure.
+  // arena, growing it when needed; false on allocation failure.
   bool scratch_carve(const uint64_t *bytes, void **out, int n);
 
   // one-word readbacks
   OpCounters *d_ctr = nullptr, *h_ctr = nullptr;
   uint32_t *d_flag = nullptr, *h_flag = nullptr;
+  unsigned int *d_status = nullptr, *h_status = nullptr;
+
+  // planned builds
+  std::vector<PlanBuf> bufs;
+  double plan_factor = 2.0;  // predicted SpGEMM output entries per row, halved
+  uint64_t plan_retries = 0;
+  OpCounters *d_plan_ctr = nullptr, *h_plan_ctr = nullptr;
+  int plan_ctr_cap = 0;
+  std::vector<hipEvent_t> plan_ev;
 
   // per-kernel timing (dv_profile)
   bool prof = false;
@@ -71,9 +152,12 @@ struct Ctx {
   // per-operation statistics
   uint64_t stats_alg_bytes = 0, stats_overflow_rows = 0, stats_mid_rows = 0, stats_big_rows = 0;
   uint32_t stats_n_spgemm = 0, stats_n_spadd = 0;
+  double stats_max_ratio = 0;
+  uint32_t stats_planned = 0, stats_retried = 0;
   void reset_stats() {
     stats_alg_bytes = stats_overflow_rows = stats_mid_rows = stats_big_rows = 0;
     stats_n_spgemm = stats_n_spadd = 0;
+    stats_planned = stats_retried = 0;
   }
 
   bool hip(hipError_t e, const char *what);
@@ -82,14 +166,39 @@ struct Ctx {
   void dfree(void *p);
   void forget(void *p);
   void release(DevMat &m);
-  bool read_counters(OpCounters &out);
+  bool read_counters(OpCounters &out, uint32_t *status = nullptr);
 
   int validate(const DevMat &m);
   int clone(const DevMat &src, DevMat &dst);
+
+  // dv_ops.hip
+  bool op_scratch(uint64_t n, OpScratch &s);
+  GroupLaunch group_launch(uint32_t lanes, uint64_t nstage) const;
+  void enq_spadd_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
+                       const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status);
+  void enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
+                       uint32_t *c_entries, uint64_t cap, OpCounters *ctr,
+                       const unsigned int *status);
+  void enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
+                        const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status);
+  void enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
+                        uint32_t *c_entries, const OpScratch &s, OpCounters *ctr,
+                        const unsigned int *status);
   int identity_dev(uint64_t n, uint32_t sb, DevMat &out);
   int spadd_dev(const DevMat &a, const DevMat &b, DevMat &c);
-  GroupLaunch group_launch(uint32_t lanes, uint64_t nstage) const;
   int spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c);
+  int power_exact(const DevMat &a, uint32_t k, DevMat &out);
+  int dindex_exact(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
+
+  // dv_plan.hip
+  int buf_acquire(uint64_t rm_bytes, uint64_t cap);
+  void buf_trim();
+  uint64_t predict_cap(const Plan &p, const PlanSlot &s) const;
+  bool plan_ctr_reserve(int nops);
+  bool plan_events_reserve(int nops);
+  int plan_enqueue(Plan &p);
+  int plan_finish(Plan &p, uint32_t &status);
+  int plan_run(Plan &p, DevMat &out, const std::function<int(DevMat &)> &exact);
   int power_dev(const DevMat &a, uint32_t k, DevMat &out);
   int dindex_dev(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
 };

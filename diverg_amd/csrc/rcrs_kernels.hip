@@ -226,7 +226,9 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     Mat<S> A, Mat<S> B, uint64_t nrows, uint32_t KS, uint32_t lane_max, uint32_t mid_max,
     uint32_t *__restrict__ cnt, uint32_t *__restrict__ stage,
     uint32_t *__restrict__ ovf_list, uint32_t *__restrict__ mid_list,
-    uint32_t *__restrict__ big_list, OpCounters *__restrict__ ctr) {
+    uint32_t *__restrict__ big_list, OpCounters *__restrict__ ctr,
+    const unsigned int *__restrict__ status) {
+  if (*status) return;  // an earlier step overflowed: operands are not valid
   unsigned long long nnz_acc = 0, slice_acc = 0;
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows;
        base += (uint64_t)gridDim.x * BLOCK) {
@@ -458,12 +460,14 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_group(
     Mat<S> A, Mat<S> B, uint32_t W, uint32_t KS, uint32_t RPG, uint64_t nstage,
     const uint32_t *__restrict__ rows, const unsigned int *__restrict__ n_dev, uint32_t n_host,
     uint32_t *__restrict__ cnt, uint32_t *__restrict__ stage, uint32_t *__restrict__ ovf_list,
-    OpCounters *__restrict__ ctr, const S *__restrict__ c_rm, uint32_t *__restrict__ c_en) {
+    OpCounters *__restrict__ ctr, const S *__restrict__ c_rm, uint32_t *__restrict__ c_en,
+    const unsigned int *__restrict__ status) {
   extern __shared__ uint64_t lds_all[];
+  if (*status) return;
   constexpr uint32_t GPB = BLOCK / G;
   const uint32_t grp = threadIdx.x / G, lane = threadIdx.x % G;
   uint64_t *lds = lds_all + (size_t)grp * W;
-  const uint64_t nl = DIRECT ? n_host : *n_dev;
+  const uint64_t nl = n_dev ? *n_dev : n_host;
   unsigned long long nnz_acc = 0;
   const uint64_t groups = (uint64_t)gridDim.x * GPB;
   const uint64_t gid = (uint64_t)xcd_tile(blockIdx.x, gridDim.x) * GPB + grp;
@@ -483,7 +487,8 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_group(
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_spgemm_emit_staged(
     uint64_t nrows, const S *__restrict__ c_rm, const uint32_t *__restrict__ stage,
-    uint32_t KS, uint32_t *__restrict__ c_en) {
+    uint32_t KS, uint32_t *__restrict__ c_en, const unsigned int *__restrict__ status) {
+  if (*status) return;
   for (uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; row < nrows;
        row += (uint64_t)gridDim.x * BLOCK) {
     const uint64_t b = c_rm[row], e = c_rm[row + 1];
@@ -540,9 +545,14 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tiles(const uint32_t *__restrict
   }
 }
 
+// Exclusive scan of the tile sums; the grand total goes to ctr->total and,
+// when it exceeds the capacity of the output entries (or a 32-bit rowmap),
+// sets bit 0 (capacity) or bit 1 (rowmap width) of *status.
 __global__ __launch_bounds__(BLOCK) void k_scan_spine(unsigned long long *__restrict__ tile_sum,
                                                       uint64_t ntiles,
-                                                      OpCounters *__restrict__ ctr) {
+                                                      OpCounters *__restrict__ ctr,
+                                                      unsigned long long cap, uint32_t sb,
+                                                      unsigned int *__restrict__ status) {
   unsigned long long carry = 0;
   for (uint64_t base = 0; base < ntiles; base += BLOCK) {
     const uint64_t i = base + threadIdx.x;
@@ -552,7 +562,11 @@ __global__ __launch_bounds__(BLOCK) void k_scan_spine(unsigned long long *__rest
     if (i < ntiles) tile_sum[i] = carry + incl - v;
     carry += tot;
   }
-  if (threadIdx.x == 0) ctr->total = carry;
+  if (threadIdx.x == 0) {
+    ctr->total = carry;
+    if (carry > cap) atomicOr(status, 1u);
+    if (sb == 4 && carry > 0xFFFFFFFFull) atomicOr(status, 2u);
+  }
 }
 
 template <typename S>
@@ -653,7 +667,9 @@ __device__ __forceinline__ uint64_t spadd_row(const uint32_t *__restrict__ ae, u
 
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_spadd_count(Mat<S> A, Mat<S> B, uint64_t n,
-                                                       uint32_t *__restrict__ cnt) {
+                                                       uint32_t *__restrict__ cnt,
+                                                       const unsigned int *__restrict__ status) {
+  if (*status) return;
   for (uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; row < n;
        row += (uint64_t)gridDim.x * BLOCK)
     cnt[row] = (uint32_t)spadd_row<S, false>(A.en, A.rm[row], A.rm[row + 1], B.en,
@@ -663,7 +679,9 @@ __global__ __launch_bounds__(BLOCK) void k_spadd_count(Mat<S> A, Mat<S> B, uint6
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_spadd_write(Mat<S> A, Mat<S> B, uint64_t n,
                                                        const S *__restrict__ c_rm,
-                                                       uint32_t *__restrict__ c_en) {
+                                                       uint32_t *__restrict__ c_en,
+                                                       const unsigned int *__restrict__ status) {
+  if (*status) return;
   for (uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; row < n;
        row += (uint64_t)gridDim.x * BLOCK)
     spadd_row<S, true>(A.en, A.rm[row], A.rm[row + 1], B.en, B.rm[row], B.rm[row + 1],
@@ -671,8 +689,12 @@ __global__ __launch_bounds__(BLOCK) void k_spadd_write(Mat<S> A, Mat<S> B, uint6
 }
 
 // nnz = sum of (e - s + 1) over all ranges (crs_matrix.hpp:175-189).
-__global__ __launch_bounds__(BLOCK) void k_range_nnz(const uint2 *__restrict__ r, uint64_t nr,
-                                                     OpCounters *__restrict__ ctr) {
+// nr = ctr->total / 2 ranges (the entry count written by the scan).
+__global__ __launch_bounds__(BLOCK) void k_range_nnz(const uint2 *__restrict__ r,
+                                                     OpCounters *__restrict__ ctr,
+                                                     const unsigned int *__restrict__ status) {
+  if (*status) return;
+  const uint64_t nr = ctr->total / 2;
   unsigned long long s = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nr;
        i += (uint64_t)gridDim.x * BLOCK) {
@@ -781,7 +803,7 @@ void launch_group(hipStream_t s, DView a, DView b, const GroupLaunch &g) {
   hipLaunchKernelGGL((k_spgemm_group<S, G, DIRECT>), dim3(g.grid), dim3(BLOCK), lds, s,
                      mat<S>(a), mat<S>(b), g.lds_words, g.stage, g.rows_per_group, g.nstage,
                      g.rows, g.n_dev, g.n_host, g.cnt, g.stage_buf, g.ovf_list, g.ctr,
-                     static_cast<const S *>(g.c_rowmap), g.c_entries);
+                     static_cast<const S *>(g.c_rowmap), g.c_entries, g.status);
 }
 
 template <typename S, bool DIRECT>
@@ -823,14 +845,15 @@ void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *en
 uint64_t scan_tmp_words(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
 void scan_counts(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
-                 unsigned long long *tmp, OpCounters *ctr) {
+                 unsigned long long *tmp, OpCounters *ctr, uint64_t cap, unsigned int *status) {
   if (n == 0) {
     fill_u32(s, static_cast<uint32_t *>(rowmap), sb / 4, 0u);
     return;
   }
   const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, tmp);
-  hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(BLOCK), 0, s, tmp, ntiles, ctr);
+  hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(BLOCK), 0, s, tmp, ntiles, ctr,
+                     (unsigned long long)cap, sb, status);
   if (sb == 4)
     hipLaunchKernelGGL(k_scan_write<uint32_t>, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt,
                        n, tmp, static_cast<uint32_t *>(rowmap));
@@ -839,53 +862,55 @@ void scan_counts(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, vo
                        n, tmp, static_cast<uint64_t *>(rowmap));
 }
 
-void spadd_count(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt) {
+void spadd_count(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt,
+                 const unsigned int *status) {
   if (!a.nrows) return;
   if (sb == 4)
     hipLaunchKernelGGL(k_spadd_count<uint32_t>, dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
-                       mat<uint32_t>(a), mat<uint32_t>(b), a.nrows, cnt);
+                       mat<uint32_t>(a), mat<uint32_t>(b), a.nrows, cnt, status);
   else
     hipLaunchKernelGGL(k_spadd_count<uint64_t>, dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
-                       mat<uint64_t>(a), mat<uint64_t>(b), a.nrows, cnt);
+                       mat<uint64_t>(a), mat<uint64_t>(b), a.nrows, cnt, status);
 }
 
 void spadd_write(hipStream_t s, uint32_t sb, DView a, DView b, const void *c_rowmap,
-                 uint32_t *c_entries) {
+                 uint32_t *c_entries, const unsigned int *status) {
   if (!a.nrows) return;
   if (sb == 4)
     hipLaunchKernelGGL(k_spadd_write<uint32_t>, dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
                        mat<uint32_t>(a), mat<uint32_t>(b), a.nrows,
-                       static_cast<const uint32_t *>(c_rowmap), c_entries);
+                       static_cast<const uint32_t *>(c_rowmap), c_entries, status);
   else
     hipLaunchKernelGGL(k_spadd_write<uint64_t>, dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
                        mat<uint64_t>(a), mat<uint64_t>(b), a.nrows,
-                       static_cast<const uint64_t *>(c_rowmap), c_entries);
+                       static_cast<const uint64_t *>(c_rowmap), c_entries, status);
 }
 
-void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t n_entries, OpCounters *ctr) {
-  if (n_entries < 2) return;
-  const uint64_t nr = n_entries / 2;
-  hipLaunchKernelGGL(k_range_nnz, dim3(grid_for(nr, BLOCK * 8, 8192)), dim3(BLOCK), 0, s,
-                     reinterpret_cast<const uint2 *>(entries), nr, ctr);
+void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t max_entries, OpCounters *ctr,
+               const unsigned int *status) {
+  if (max_entries < 2) return;
+  hipLaunchKernelGGL(k_range_nnz, dim3(grid_for(max_entries / 2, BLOCK * 8, 8192)), dim3(BLOCK),
+                     0, s, reinterpret_cast<const uint2 *>(entries), ctr, status);
 }
 
 void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, uint32_t lane_max,
                  uint32_t mid_max, uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
-                 uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr) {
+                 uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
+                 const unsigned int *status) {
   if (!a.nrows) return;
   if (sb == 4)
     hipLaunchKernelGGL(k_spgemm_lane<uint32_t>, dim3(grid), dim3(BLOCK), 0, s, mat<uint32_t>(a),
                        mat<uint32_t>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list,
-                       mid_list, big_list, ctr);
+                       mid_list, big_list, ctr, status);
   else
     hipLaunchKernelGGL(k_spgemm_lane<uint64_t>, dim3(grid), dim3(BLOCK), 0, s, mat<uint64_t>(a),
                        mat<uint64_t>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list,
-                       mid_list, big_list, ctr);
+                       mid_list, big_list, ctr, status);
 }
 
 void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunch &g) {
   const bool direct = g.c_entries != nullptr;
-  if (direct && g.n_host == 0) return;
+  if (direct && !g.n_dev && g.n_host == 0) return;
   if (sb == 4) {
     if (direct) group_by_lanes<uint32_t, true>(s, a, b, g);
     else group_by_lanes<uint32_t, false>(s, a, b, g);
@@ -896,16 +921,17 @@ void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunc
 }
 
 void spgemm_emit_staged(hipStream_t s, uint32_t sb, uint64_t nrows, const void *c_rowmap,
-                        const uint32_t *stage_buf, uint32_t stage, uint32_t *c_entries) {
+                        const uint32_t *stage_buf, uint32_t stage, uint32_t *c_entries,
+                        const unsigned int *status) {
   if (!nrows) return;
   if (sb == 4)
     hipLaunchKernelGGL(k_spgemm_emit_staged<uint32_t>, dim3(grid_for(nrows)), dim3(BLOCK), 0,
                        s, nrows, static_cast<const uint32_t *>(c_rowmap), stage_buf, stage,
-                       c_entries);
+                       c_entries, status);
   else
     hipLaunchKernelGGL(k_spgemm_emit_staged<uint64_t>, dim3(grid_for(nrows)), dim3(BLOCK), 0,
                        s, nrows, static_cast<const uint64_t *>(c_rowmap), stage_buf, stage,
-                       c_entries);
+                       c_entries, status);
 }
 
 void query(hipStream_t s, uint32_t sb, DView m, const uint32_t *qi, const uint32_t *qj,

@@ -27,8 +27,8 @@ struct OpCounters {
 };
 
 // One launch of the group tiers over a row list.  Counting mode when
-// c_entries is NULL (list length read from *n_dev on the device), direct
-// write mode otherwise (list length n_host).
+// c_entries is NULL, direct-write mode otherwise.  The list length is *n_dev
+// (read on the device) when n_dev is set, else n_host.
 struct GroupLaunch {
   uint32_t lanes;           // 4, 8, 16, 32 or 64
   uint32_t lds_words;       // bit-window words per group
@@ -45,6 +45,7 @@ struct GroupLaunch {
   OpCounters *ctr;
   const void *c_rowmap;
   uint32_t *c_entries;
+  const unsigned int *status;  // kernels exit at once when *status != 0
 };
 
 void fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
@@ -53,16 +54,21 @@ void zero_counters(hipStream_t s, OpCounters *c);
 void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *entries);
 
 // rowmap[0] = 0, rowmap[i+1] = cnt[0] + ... + cnt[i]; total -> ctr->total.
+// A total above `cap` sets bit 0 of *status, above a 32-bit rowmap bit 1.
 // `tmp` needs scan_tmp_words(n) uint64 words.
 uint64_t scan_tmp_words(uint64_t n);
 void scan_counts(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n,
-                 void *rowmap, unsigned long long *tmp, OpCounters *ctr);
+                 void *rowmap, unsigned long long *tmp, OpCounters *ctr, uint64_t cap,
+                 unsigned int *status);
 
-void spadd_count(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt);
+// Kernels that read operands take `status` and do nothing once it is set.
+void spadd_count(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt,
+                 const unsigned int *status);
 void spadd_write(hipStream_t s, uint32_t sb, DView a, DView b,
-                 const void *c_rowmap, uint32_t *c_entries);
-void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t n_entries,
-               OpCounters *ctr);
+                 const void *c_rowmap, uint32_t *c_entries, const unsigned int *status);
+// nnz of the ctr->total entries (at most max_entries) added to ctr->nnz.
+void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t max_entries,
+               OpCounters *ctr, const unsigned int *status);
 
 // Lane tier of the counting pass: one lane per row of A.  Rows it cannot
 // finish (more than lane_max slice ranges, or a union of more than 4 runs)
@@ -73,13 +79,14 @@ void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t n_entries,
 void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, uint32_t lane_max,
                  uint32_t mid_max,
                  uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
-                 uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr);
+                 uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
+                 const unsigned int *status);
 // Group tiers (counting or direct write), see GroupLaunch.
 void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunch &g);
 // Copy staged rows (count <= stage) to their place in C.
 void spgemm_emit_staged(hipStream_t s, uint32_t sb, uint64_t nrows,
                         const void *c_rowmap, const uint32_t *stage_buf,
-                        uint32_t stage, uint32_t *c_entries);
+                        uint32_t stage, uint32_t *c_entries, const unsigned int *status);
 
 // out[q] = M(qi[q], qj[q]); 2 and *bad = 1 for out-of-range pairs.
 void query(hipStream_t s, uint32_t sb, DView m, const uint32_t *qi,

@@ -75,10 +75,15 @@ typedef struct dv_build_stats {
   double h2d_ms;         /* host -> device copy of the input (0 if already on the device) */
   double d2h_ms;         /* device -> host copy of the output (0 if kept on the device) */
   uint64_t alg_bytes;
-  uint64_t peak_device_bytes;
+  uint64_t peak_device_bytes;  /* HBM held at the build's high-water mark: the input,
+                                  the context's cached buffers and scratch, the output */
   uint32_t n_spgemm;
   uint32_t n_spadd;
   uint64_t overflow_rows;  /* rows recomputed by the direct-write SpGEMM pass */
+  uint64_t deferred_rows;  /* rows the lane-per-row tier handed to lane groups */
+  uint32_t planned;        /* 1: enqueued as one planned sequence, one readback */
+  uint32_t retried;        /* 1: the plan overflowed a predicted capacity and the
+                              build was repeated with a readback per operation */
 } dv_build_stats;
 
 /* --- runtime ------------------------------------------------------------
@@ -104,7 +109,13 @@ void *dv_stream(dv_context *ctx);
  *   lane_slice_max  lane-per-row tier: at most this many slice ranges
  *                   (0 sends every non-empty row to the groups)
  *   autotune        1: dv_create_distance_index* picks lanes_small and
- *                   small_slice_max per build from timed probes */
+ *                   small_slice_max per build from timed probes
+ *   plan_mode       0: index builds and powers are enqueued as one sequence
+ *                   with predicted output capacities and a single readback
+ *                   (exact fallback on overflow); 1: one readback per
+ *                   operation, as the reference's drivers; 2: as 0 with a
+ *                   capacity prediction too small to hold any product, which
+ *                   always takes the fallback (for tests) */
 typedef struct dv_tuning {
   uint32_t lds_words;
   uint32_t stage_entries;
@@ -113,6 +124,7 @@ typedef struct dv_tuning {
   uint32_t small_slice_max;
   uint32_t lane_slice_max;
   uint32_t autotune;
+  uint32_t plan_mode;
 } dv_tuning;
 int dv_get_tuning(const dv_context *ctx, dv_tuning *t);
 int dv_set_tuning(dv_context *ctx, const dv_tuning *t);

@@ -1,5 +1,6 @@
 """The C ABI library and the host logic (CPU only: no compute calls)."""
 import ctypes
+import subprocess
 import os
 import re
 
@@ -43,8 +44,8 @@ def test_abi_version_and_status_strings():
 
 def test_struct_sizes_match_header():
     assert ctypes.sizeof(N.dv_rcrs) == 8 * 6 + 4 * 2 + 8
-    assert ctypes.sizeof(N.dv_build_stats) == 8 * 5 + 4 * 2 + 8
-    assert ctypes.sizeof(N.dv_tuning) == 4 * 7
+    assert ctypes.sizeof(N.dv_build_stats) == 8 * 5 + 4 * 2 + 8 * 2 + 4 * 2
+    assert ctypes.sizeof(N.dv_tuning) == 4 * 8
     assert ctypes.sizeof(N.dv_kernel_time) == 8 * 4
 
 
@@ -124,3 +125,20 @@ def test_init_without_device_reports_device_error():
     with pytest.raises(dv.DiVerGError) as e:
         dv.Context(0)
     assert e.value.status == 4
+
+
+def test_cpp_wrapper_headers_compile(tmp_path):
+    """The drop-in headers compile as the reference's call sites use them."""
+    src = tmp_path / "use.cpp"
+    src.write_text(
+        "#include <diverg/dindex.hpp>\n"
+        "using M = diverg::CRSMatrix<diverg::crs_matrix::RangeDynamic, bool, uint32_t, uint64_t>;\n"
+        "using M32 = diverg::CRSMatrix<diverg::crs_matrix::RangeDynamic, bool, uint32_t, uint32_t>;\n"
+        "M f(M const& a) { return diverg::util::create_distance_index(a, 150, 450); }\n"
+        "M32 g(M32 const& a) { return diverg::range_power(diverg::range_spadd(a, a), 3); }\n"
+        "M h(M const& a) { return diverg::range_spgemm(a, a, diverg::DefaultSparseConfiguration{}); }\n")
+    res = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                          "-I" + os.path.join(ROOT, "include"),
+                          "-I" + os.path.join(ROOT, "diverg_amd", "include"), str(src)],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr

@@ -114,19 +114,36 @@ def test_spadd_random_and_noncanonical(ctx):
 
 
 @pytest.mark.parametrize("k", [0, 1, 2, 5, 16, 150])
-def test_power(ctx, cfg1, k):
-    print(f"power k={k}", flush=True)
+@pytest.mark.parametrize("plan_mode", [0, 1, 2])
+def test_power(ctx, cfg1, k, plan_mode):
+    ctx.set_tuning(plan_mode=plan_mode)
+    print(f"power k={k} plan_mode={plan_mode}", flush=True)
     assert_identical(ctx.range_power(cfg1, k), orc.power(cfg1, k), f"A^{k}")
 
 
+def test_power_noncanonical_input(ctx):
+    """The first product of a chain is I.A when A is the caller's matrix, so
+    a non-canonical input comes back canonical, as in the reference."""
+    a = dv.RangeCRS(6, np.array([0, 1, 1, 3, 5, 5, 4, 4], np.uint32),
+                    np.array([0, 4, 4, 6, 8, 8, 8], np.uint64))
+    for k in (1, 3):
+        assert_identical(ctx.range_power(a, k), orc.power(a, k), f"A^{k} non-canonical")
+
+
 @pytest.mark.parametrize("size_bytes", [4, 8])
-@pytest.mark.parametrize("dlo,dup", [(150, 450), (0, 250), (7, 7), (1, 8)])
-def test_distance_index_cfg1(ctx, cfg1, size_bytes, dlo, dup):
+@pytest.mark.parametrize("dlo,dup", [(150, 450), (0, 250), (7, 7), (1, 8), (5, 5), (0, 1)])
+@pytest.mark.parametrize("plan_mode", [0, 1, 2])
+def test_distance_index_cfg1(ctx, cfg1, size_bytes, dlo, dup, plan_mode):
+    """Planned builds (one readback), exact builds (a readback per op) and
+    planned builds forced through the overflow fallback agree with the oracle."""
+    ctx.set_tuning(plan_mode=plan_mode)
     a = cfg1.as_size(size_bytes)
-    print(f"dindex cfg1 [{dlo},{dup}] S={size_bytes}", flush=True)
+    print(f"dindex cfg1 [{dlo},{dup}] S={size_bytes} plan_mode={plan_mode}", flush=True)
     d, st = ctx.create_distance_index(a, dlo, dup, stats=True)
     assert_identical(d, orc.dindex(a, dlo, dup), f"D[{dlo},{dup}]")
     assert st["n_spadd"] == 1 and st["n_spgemm"] > 0
+    assert st["planned"] == (plan_mode == 0)
+    assert st["retried"] == (plan_mode == 2)
 
 
 def test_distance_index_bad_bounds(ctx, cfg1):
