@@ -9,9 +9,10 @@ D = A^300 . (A + I)^300 with A already resident in HBM; the output stays in
 HBM.  `value` is loci indexed per second (rows of D per second of build).
 
 The JSON line also carries:
-  roofline      the SpGEMM counting kernel (the dominant kernel): algorithmic
-                bytes bytes(X) + bytes(Y) + bytes(C) per launch over its mean
-                launch time, HIP events on the library stream, against 8 TB/s
+  roofline      the SpGEMM lane-per-row kernel k_spgemm_lane (the dominant
+                kernel): algorithmic bytes bytes(X) + bytes(Y) + bytes(C) of its
+                product per launch over its mean launch time (HIP events on the
+                library stream), against 8 TB/s
   cpu_baseline  the CPU oracle (the reference's OpenMP algorithm restated in
                 C) building the same index on the host, timed once
   bit_exact     the GPU index equals the oracle's (rowmap, entries, nnz)
@@ -57,7 +58,20 @@ def main():
     ap.add_argument("--dlo", type=int, default=300)
     ap.add_argument("--dup", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy kernel three times "
+                         "(the reference for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        x = torch.arange(1 << 28, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": 4 << 28, "dispatches": 3}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
@@ -100,7 +114,7 @@ def main():
     build_ms = float(np.median([s["build_ms"] for s in stats]))
     value = a.nrows * args.steps / elapsed
 
-    cnt = prof["spgemm_count"]
+    cnt = prof["spgemm_lane"]
     mean_ms = cnt["total_ms"] / max(1, cnt["launches"])
     alg_per_launch = cnt["alg_bytes"] / max(1, cnt["launches"])
     achieved = alg_per_launch / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else 0.0
@@ -108,7 +122,7 @@ def main():
         if mean_ms > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": "k_spgemm_count",
+        "kernel": "k_spgemm_lane",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",

@@ -66,7 +66,7 @@ class dv_kernel_time(C.Structure):
     ]
 
 
-KERNEL_KINDS = ["spgemm_count", "spgemm_write", "scan", "spadd"]
+KERNEL_KINDS = ["spgemm_lane", "spgemm_write", "spgemm_groups_scan", "spadd"]
 
 
 class dv_graph_params(C.Structure):

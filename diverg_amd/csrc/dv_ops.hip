@@ -76,7 +76,7 @@ void Ctx::enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap
 // rows whose output exceeds the stage.
 void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                            const OpScratch &s, OpCounters *ctr, uint64_t cap,
-                           unsigned int *status) {
+                           unsigned int *status, hipEvent_t after_lane) {
   const uint64_t n = a.nrows;
   if (n) {
     // persistent lane-tier grid: few blocks keep the per-block counter atomics
@@ -86,6 +86,7 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
     spgemm_lane(stream, a.sb, view(a), view(b), s.stage_entries, tuning.lane_slice_max,
                 tuning.small_slice_max, lane_grid, s.cnt, s.stage, s.ovf, s.mid, s.big, ctr,
                 status);
+    if (after_lane) (void)hipEventRecord(after_lane, stream);
     GroupLaunch g = group_launch(tuning.lanes_small, n);
     g.rows = s.mid;
     g.n_dev = &ctr->n_mid;
@@ -100,6 +101,8 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
     g.rows = s.big;
     g.n_dev = &ctr->n_big;
     spgemm_group(stream, a.sb, view(a), view(b), g);
+  } else if (after_lane) {
+    (void)hipEventRecord(after_lane, stream);
   }
   scan_counts(stream, c_sb, s.cnt, n, c_rowmap, s.tmp, ctr, cap, status);
 }
@@ -191,7 +194,7 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   zero_counters(stream, d_ctr);
   fill_u32(stream, d_status, 1, 0u);
   prof_mark(0);
-  enq_spgemm_count(a, b, c.rowmap, c.sb, s, d_ctr, ~0ull, d_status);
+  enq_spgemm_count(a, b, c.rowmap, c.sb, s, d_ctr, ~0ull, d_status, prof ? pev[1] : nullptr);
   prof_mark(2);
   OpCounters h{};
   uint32_t st = 0;
@@ -219,7 +222,8 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   }
   {
     const uint64_t alg = mat_bytes(a) + mat_bytes(b) + mat_bytes(c);
-    prof_add(0, 0, 2, alg, 16ull * h.slice);
+    prof_add(0, 0, 1, alg, 8ull * h.slice);
+    prof_add(2, 1, 2, 0, 0);
     prof_add(1, 3, 4, 0, 0);
   }
   stats_alg_bytes += mat_bytes(a) + mat_bytes(b) + mat_bytes(c);

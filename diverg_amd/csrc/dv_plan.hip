@@ -184,8 +184,8 @@ int Ctx::plan_enqueue(Plan &p) {
         if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
         enq_spadd_write(a, b, c.rowmap, c.entries, c.cap, ctr, d_status);
       } else {
-        enq_spgemm_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status);
-        if (timing) (void)hipEventRecord(plan_ev[4 * i + 1], stream);
+        enq_spgemm_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status,
+                         timing ? plan_ev[4 * i + 1] : nullptr);
         if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
         enq_spgemm_write(a, b, c.rowmap, c.entries, s, ctr, d_status);
       }
@@ -242,9 +242,11 @@ int Ctx::plan_finish(Plan &p, uint32_t &status_out) {
           prof_t[0].launches += 1;
           prof_t[0].total_ms += t0;
           prof_t[0].alg_bytes += alg;
-          prof_t[0].slice_bytes += 16ull * h.slice;
+          prof_t[0].slice_bytes += 8ull * h.slice;
           prof_t[1].launches += 1;
           prof_t[1].total_ms += t2;
+          prof_t[2].launches += 1;
+          prof_t[2].total_ms += t1;
         } else {
           prof_t[3].launches += 1;
           prof_t[3].total_ms += t0 + t1 + t2;

@@ -179,8 +179,10 @@ struct Ctx {
   void enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
                        uint32_t *c_entries, uint64_t cap, OpCounters *ctr,
                        const unsigned int *status);
+  // after_lane (optional) is recorded right after the lane-tier kernel
   void enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
-                        const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status);
+                        const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status,
+                        hipEvent_t after_lane = nullptr);
   void enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
                         uint32_t *c_entries, const OpScratch &s, OpCounters *ctr,
                         const unsigned int *status);

@@ -131,11 +131,12 @@ int dv_set_tuning(dv_context *ctx, const dv_tuning *t);
 
 /* Per-kernel timing with HIP events on the context stream (no counterpart in
  * the reference, whose DIVERG_STATS timers print symbolic/numeric times after
- * a fence, range_sparse.hpp:2243-2278).  Kinds: 0 SpGEMM counting pass,
- * 1 SpGEMM write passes (staged copy + direct rows), 2 row-pointer scans,
+ * a fence, range_sparse.hpp:2243-2278).  Kinds: 0 the SpGEMM lane-per-row
+ * counting kernel (k_spgemm_lane), 1 SpGEMM write passes (staged copy +
+ * direct rows), 2 the SpGEMM group-tier kernels and the row-pointer scan,
  * 3 SpAdd.  alg_bytes sums bytes(X) + bytes(Y) + bytes(C) of the timed
- * operations; slice_bytes sums the bytes of B-slice ranges the pass streams
- * (8 per range, re-reads included). */
+ * SpGEMMs; slice_bytes sums 8 bytes per B-slice range the lane and group
+ * tiers gathered. */
 #define DV_KERNEL_KINDS 4
 typedef struct dv_kernel_time {
   uint64_t launches;
