@@ -8,6 +8,20 @@ index for [d1, d2] = [300, 600].  A step is one full index build
 D = A^300 . (A + I)^300 with A already resident in HBM; the output stays in
 HBM.  `value` is loci indexed per second (rows of D per second of build).
 
+Further legs (fields of the same JSON line; --legs selects them):
+  chromosome    the 5e7 bp graph with structural variants (uint64 rowmap,
+                [200, 1000]): build time, peak HBM, rows checked against
+                the oracle's independent walk-length BFS
+  serving       the chromosome index (and the 5e6 bp one) resident in HBM, a
+                Poisson stream of locus-pair queries (mean 2e6/s offered;
+                sources uniform, half the targets 0-1000 bp downstream on the
+                backbone) with component builds of 1e5-2e7 bp interleaved:
+                queries/s, p50/p99 latency, build completion times, answers
+                checked against the CPU lookup
+  batched       24 components (log-normal lengths, median 5e6 bp) built in
+                one planned build over their block-diagonal assembly vs 24
+                plain builds; bit-exact per component
+
 The JSON line also carries:
   roofline      the SpGEMM lane-per-row kernel k_spgemm_lane (the dominant
                 kernel): algorithmic bytes bytes(X) + bytes(Y) + bytes(C) of its
@@ -58,6 +72,10 @@ def main():
     ap.add_argument("--dlo", type=int, default=300)
     ap.add_argument("--dup", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--legs", default="plain,chromosome,serving,batched",
+                    help="comma-separated legs to run (plain is always run)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-rate", type=float, default=2.0e6)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel three times "
                          "(the reference for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -194,8 +212,180 @@ def main():
         "cpu_baseline": cpu,
         "kernels_ms": {k: round(v["total_ms"] / args.steps, 4) for k, v in prof.items()},
     }
+    legs = set(args.legs.split(","))
+    if "chromosome" in legs or "serving" in legs:
+        out["chromosome"], chrom = leg_chromosome(ctx)
+        if "serving" in legs:
+            out["serving"] = leg_serving(chrom, result_host, args)
+        del chrom
+    if "batched" in legs:
+        out["batched"] = leg_batched(ctx)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def leg_chromosome(ctx):
+    """5e7 bp graph with SVs, uint64 rowmap, [200, 1000]."""
+    import diverg_amd as dv
+    from oracle import oracle as orc
+    t = time.time()
+    a, vpos = dv.generate_graph(3, size_bytes=8, with_vpos=True)
+    log(f"chromosome graph: {a.nrows} loci, {len(a.entries) // 2} ranges ({time.time() - t:.1f}s)")
+    da = ctx.to_device(a)
+    d = ctx.create_distance_index(da, 200, 1000, location="device")  # warm
+    d.free()
+    d, st = ctx.create_distance_index(da, 200, 1000, location="device", stats=True)
+    dh = d.to_host()
+    d.free()
+    del da
+    rng = np.random.default_rng(0xD1E5)
+    rows = np.sort(rng.choice(a.nrows, 1500, replace=False)).astype(np.uint32)
+    w = orc.walk_rows(a, rows, 200, 1000)
+    ok = all(np.array_equal(dh.entries[dh.rowmap[r]:dh.rowmap[r + 1]],
+                            w.entries[w.rowmap[k]:w.rowmap[k + 1]]) for k, r in enumerate(rows))
+    res = {
+        "workload": "5e7 bp backbone, SNP 0.01, indel 0.001, SV 1e-5 (30% inversions), "
+                    "uint64 rowmap, [200,1000]",
+        "loci": a.nrows,
+        "build_s": round(st["build_ms"] / 1e3, 5),
+        "loci_per_s": round(a.nrows / (st["build_ms"] / 1e3), 1),
+        "spgemms": st["n_spgemm"],
+        "alg_GBps": round(st["alg_bytes"] / (st["build_ms"] * 1e-3) / 1e9, 2),
+        "peak_hbm_bytes": st["peak_device_bytes"],
+        "index_ranges": len(dh.entries) // 2,
+        "index_nnz": dh.nnz,
+        "rows_checked_vs_walk_bfs": len(rows),
+        "rows_bit_exact": bool(ok),
+    }
+    log(f"chromosome build {res['build_s']}s, rows bit-exact {ok}")
+    return res, (a, vpos, dh)
+
+
+def leg_serving(chrom, idx2, args):
+    """Continuous batching of a Poisson query stream with interleaved builds."""
+    import diverg_amd as dv
+    from oracle import oracle as orc
+    a, vpos, dh = chrom
+    indexes = [dh, idx2]
+    rng = np.random.default_rng(0xD1E55)
+    n = int(args.serve_rate * args.serve_seconds)
+    arrival = np.cumsum(rng.exponential(1e6 / args.serve_rate, n))
+    qidx = (rng.random(n) < 0.1).astype(np.uint32)  # 90% chromosome index, 10% the 5e6 bp one
+    qi = np.zeros(n, np.uint32)
+    qj = np.zeros(n, np.uint32)
+    # chromosome index: targets 0-1000 bp downstream on the backbone for half
+    sel = qidx == 0
+    m = int(sel.sum())
+    src = rng.integers(0, a.nrows, m, dtype=np.int64)
+    down = vpos[src].astype(np.int64) + rng.integers(0, 1001, m)
+    down = np.minimum(down, int(vpos[-1]))
+    near_t = np.searchsorted(vpos, down, side="left").astype(np.int64)
+    uni_t = rng.integers(0, a.nrows, m, dtype=np.int64)
+    qi[sel] = src
+    qj[sel] = np.where(rng.random(m) < 0.5, near_t, uni_t)
+    sel2 = ~sel
+    m2 = int(sel2.sum())
+    s2 = rng.integers(0, idx2.nrows, m2, dtype=np.int64)
+    qi[sel2] = s2
+    qj[sel2] = np.where(rng.random(m2) < 0.5, np.minimum(s2 + rng.integers(0, 1001, m2),
+                                                         idx2.ncols - 1),
+                        rng.integers(0, idx2.ncols, m2))
+    # builds of components of 1e5 - 2e7 bp, log-uniform, spread over the stream
+    nb = 4
+    lens = np.exp(rng.uniform(np.log(1e5), np.log(2e7), nb)).astype(np.int64)
+    builds = []
+    for b in range(nb):
+        comp = dv.generate_graph(2, backbone_bp=int(lens[b]), seed=0xD1E55000 + b, size_bytes=8)
+        builds.append((comp, 200, 1000, float((b + 0.5) * args.serve_seconds * 1e6 / nb)))
+    log(f"serving: {n} queries over {args.serve_seconds}s, builds of {lens.tolist()} bp")
+    with dv.Server(query_cus=32) as srv:
+        ids = [srv.add_index(x) for x in indexes]
+        res = srv.replay(qidx, qi, qj, arrival, builds)
+        built = [srv.get_index(int(i)) for i in res["build_index"]]
+    ok = True
+    for k in range(2):
+        s = qidx == k
+        exp, rc = orc.query(indexes[k], qi[s], qj[s])
+        ok = ok and rc == 0 and np.array_equal(res["answers"][s], exp)
+    lat = res["latency_us"]
+    done = arrival + lat
+    span = (done.max() - arrival.min()) / 1e6
+    # builds in the mix against plain builds of the same components
+    bok = True
+    with dv.Context(0) as c2:
+        for (comp, dlo, dup, _), got in zip(builds, built):
+            ref = c2.create_distance_index(comp, dlo, dup)
+            bok = bok and dv.identical(got, ref)
+    out = {
+        "workload": "indexes: chromosome graph [200,1000] + 5e6 bp graph [300,600]; Poisson "
+                    f"stream {args.serve_rate:.0f} req/s offered for {args.serve_seconds}s; "
+                    "builds of 1e5-2e7 bp components interleaved",
+        "queries": n,
+        "offered_qps": args.serve_rate,
+        "queries_per_s": round(n / span, 1),
+        "p50_us": round(float(np.percentile(lat, 50)), 2),
+        "p99_us": round(float(np.percentile(lat, 99)), 2),
+        "max_us": round(float(lat.max()), 2),
+        "builds_bp": lens.tolist(),
+        "build_completion_s": [round(x / 1e6, 4) for x in res["build_done_us"].tolist()],
+        "answers_bit_exact_vs_cpu_lookup": bool(ok),
+        "builds_in_mix_equal_alone": bool(bok),
+    }
+    log(f"serving: {out['queries_per_s']:.0f} q/s, p50 {out['p50_us']} us, "
+        f"p99 {out['p99_us']} us, exact {ok}, builds {out['build_completion_s']}")
+    return out
+
+
+def leg_batched(ctx):
+    """24 components in one planned build vs 24 plain builds."""
+    import diverg_amd as dv
+    t = time.time()
+    comps = [dv.generate_graph(4, component=c, size_bytes=8) for c in range(24)]
+    total = sum(c.nrows for c in comps)
+    log(f"batched: 24 components, {total} loci ({time.time() - t:.1f}s to generate)")
+    dcomps = [ctx.to_device(c) for c in comps]
+    del comps
+    ctx.set_tuning(graphs=1)
+    outs = ctx.create_distance_index_batched(dcomps, 300, 600, location="device")  # warm
+    for o in outs:
+        o.free()
+    hip_sync()
+    t = time.perf_counter()
+    outs, st = ctx.create_distance_index_batched(dcomps, 300, 600, location="device", stats=True)
+    hip_sync()
+    batched_s = time.perf_counter() - t
+    ctx.set_tuning(graphs=0)
+    hip_sync()
+    t = time.perf_counter()
+    plain = []
+    for dc in dcomps:
+        plain.append(ctx.create_distance_index(dc, 300, 600, location="device"))
+    hip_sync()
+    plain_s = time.perf_counter() - t
+    exact = True
+    for o, p in zip(outs, plain):
+        exact = exact and dv.identical(o.to_host(), p.to_host())
+        o.free()
+        p.free()
+    res = {
+        "workload": "24 components, log-normal lengths (median 5e6 bp, sigma 1, clipped "
+                    "1e5-1.2e8), SNP 0.01, indel 0.001, uint64 rowmap, [300,600]",
+        "components": 24,
+        "loci": total,
+        "batched_s": round(batched_s, 4),
+        "sequential_plain_s": round(plain_s, 4),
+        "speedup": round(plain_s / batched_s, 3),
+        "batched_loci_per_s": round(total / batched_s, 1),
+        "spgemms": st["n_spgemm"],
+        "peak_hbm_bytes": st["peak_device_bytes"],
+        "bit_exact_per_component": bool(exact),
+    }
+    log(f"batched {batched_s:.3f}s vs sequential {plain_s:.3f}s, exact {exact}")
+    return res
 
 
 if __name__ == "__main__":

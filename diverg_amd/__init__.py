@@ -22,8 +22,8 @@ import numpy as np
 from . import _native as N
 from ._native import NativeMissing  # noqa: F401
 
-__all__ = ["RangeCRS", "DeviceRCRS", "Context", "DiVerGError", "generate_graph",
-           "graph_preset", "NativeMissing"]
+__all__ = ["RangeCRS", "DeviceRCRS", "Context", "Server", "DiVerGError", "generate_graph",
+           "graph_preset", "identical", "NativeMissing"]
 
 
 class DiVerGError(RuntimeError):
@@ -286,6 +286,24 @@ class Context:
             return m, BuildStats({f: getattr(st, f) for f, _ in N.dv_build_stats._fields_})
         return m
 
+    def create_distance_index_batched(self, comps, dlo, dup, location="host", stats=False):
+        """Indexes of independent components in one planned build over their
+        block-diagonal assembly; outputs[c] equals create_distance_index(comps[c])."""
+        keep = []
+        n = len(comps)
+        arr = (N.dv_rcrs * max(1, n))()
+        for i, m in enumerate(comps):
+            arr[i] = m._desc(keep)
+        outs = (N.dv_rcrs * max(1, n))()
+        st = N.dv_build_stats()
+        self._check(N.lib().dv_create_distance_index_batched(
+            self._h, arr, n, dlo, dup, self._loc(location), outs, C.byref(st)),
+            "dv_create_distance_index_batched")
+        res = [self._out(outs[i], location) for i in range(n)]
+        if stats:
+            return res, BuildStats({f: getattr(st, f) for f, _ in N.dv_build_stats._fields_})
+        return res
+
     def query(self, m, qi, qj):
         """Batched D(i, j); raises DiVerGError(DV_ERR_RANGE) for pairs outside."""
         keep = []
@@ -299,6 +317,131 @@ class Context:
                                     N.DV_HOST)
         self._check(rc, "dv_query_batch")
         return out
+
+
+class Server:
+    """Serving loop over HBM-resident indexes (dv_server_*): continuous
+    batching of locus-pair queries, index builds admitted by HBM budget."""
+
+    def __init__(self, device=0, hbm_budget=0, max_batch=0, max_inflight=0, query_cus=0):
+        self._h = None
+        o = N.dv_server_opts(hbm_budget, max_batch, max_inflight, query_cus, 0)
+        h = C.c_void_p()
+        rc = N.lib().dv_server_start(device, C.byref(o), C.byref(h))
+        if rc:
+            raise DiVerGError(rc, "dv_server_start failed")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            N.lib().dv_server_stop(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _check(rc, what):
+        if rc:
+            raise DiVerGError(rc, what)
+
+    def add_index(self, m):
+        keep = []
+        i = C.c_uint32()
+        self._check(N.lib().dv_server_add_index(self._h, C.byref(m._desc(keep)), C.byref(i)),
+                    "dv_server_add_index")
+        return i.value
+
+    def submit_queries(self, index_id, qi, qj):
+        qi = np.ascontiguousarray(qi, dtype=np.uint32)
+        qj = np.ascontiguousarray(qj, dtype=np.uint32)
+        t = C.c_uint64()
+        self._check(N.lib().dv_server_submit_queries(
+            self._h, index_id, qi.ctypes.data_as(N.U32P), qj.ctypes.data_as(N.U32P), len(qi),
+            C.byref(t)), "dv_server_submit_queries")
+        return t.value
+
+    def submit_build(self, a, dlo, dup):
+        keep = []
+        t = C.c_uint64()
+        self._check(N.lib().dv_server_submit_build(self._h, C.byref(a._desc(keep)), dlo, dup,
+                                                   C.byref(t)), "dv_server_submit_build")
+        return t.value
+
+    @staticmethod
+    def _info(i):
+        return {f: getattr(i, f) for f, _ in N.dv_request_info._fields_}
+
+    def poll(self, ticket):
+        i = N.dv_request_info()
+        rc = N.lib().dv_server_poll(self._h, ticket, C.byref(i))
+        if rc not in (0, 7):
+            raise DiVerGError(rc, "dv_server_poll")
+        return self._info(i) if rc == 0 else None
+
+    def wait(self, ticket):
+        i = N.dv_request_info()
+        self._check(N.lib().dv_server_wait(self._h, ticket, C.byref(i)), "dv_server_wait")
+        return self._info(i)
+
+    def results(self, ticket, n):
+        out = np.zeros(n, np.uint8)
+        self._check(N.lib().dv_server_results(self._h, ticket,
+                                              out.ctypes.data_as(C.POINTER(C.c_uint8)), n),
+                    "dv_server_results")
+        return out
+
+    def get_index(self, index_id):
+        d = N.dv_rcrs()
+        self._check(N.lib().dv_server_get_index(self._h, index_id, C.byref(d)),
+                    "dv_server_get_index")
+        m = _from_desc(d)
+        N.lib().dv_free(None, C.byref(d))
+        return m
+
+    def replay(self, qidx, qi, qj, arrival_us, builds=()):
+        """Replay a query stream (and build requests) in real time.
+
+        builds: iterable of (matrix, dlo, dup, arrival_us).  Returns answers,
+        per-query latency (us), per-build completion time (us after its
+        arrival) and produced index ids, and the elapsed wall time."""
+        n = len(qi)
+        qidx = np.ascontiguousarray(qidx, dtype=np.uint32)
+        qi = np.ascontiguousarray(qi, dtype=np.uint32)
+        qj = np.ascontiguousarray(qj, dtype=np.uint32)
+        arr = np.ascontiguousarray(arrival_us, dtype=np.float64)
+        ans = np.zeros(n, np.uint8)
+        lat = np.zeros(n, np.float64)
+        builds = list(builds)
+        nb = len(builds)
+        keep = []
+        bd = (N.dv_rcrs * max(1, nb))()
+        for k, (m, _, _, _) in enumerate(builds):
+            bd[k] = m._desc(keep)
+        blo = np.array([b[1] for b in builds] or [0], np.uint32)
+        bup = np.array([b[2] for b in builds] or [0], np.uint32)
+        bus = np.array([b[3] for b in builds] or [0], np.float64)
+        bdone = np.zeros(max(1, nb), np.float64)
+        bidx = np.zeros(max(1, nb), np.uint32)
+        el = C.c_double()
+        rc = N.lib().dv_server_replay(
+            self._h, n, qidx.ctypes.data_as(N.U32P), qi.ctypes.data_as(N.U32P),
+            qj.ctypes.data_as(N.U32P), arr.ctypes.data_as(N.DP),
+            ans.ctypes.data_as(C.POINTER(C.c_uint8)), lat.ctypes.data_as(N.DP), nb, bd,
+            blo.ctypes.data_as(N.U32P), bup.ctypes.data_as(N.U32P), bus.ctypes.data_as(N.DP),
+            bdone.ctypes.data_as(N.DP), bidx.ctypes.data_as(N.U32P), C.byref(el))
+        self._check(rc, "dv_server_replay")
+        return {"answers": ans, "latency_us": lat, "build_done_us": bdone[:nb],
+                "build_index": bidx[:nb], "elapsed_us": el.value}
 
 
 def graph_preset(preset, component=0):

@@ -54,6 +54,7 @@ class dv_tuning(C.Structure):
         ("lane_slice_max", C.c_uint32),
         ("autotune", C.c_uint32),
         ("plan_mode", C.c_uint32),
+        ("graphs", C.c_uint32),
     ]
 
 
@@ -89,8 +90,33 @@ class dv_graph_params(C.Structure):
     ]
 
 
+class dv_server_opts(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_uint64),
+        ("max_batch", C.c_uint32),
+        ("max_inflight", C.c_uint32),
+        ("query_cus", C.c_uint32),
+        ("pad", C.c_uint32),
+    ]
+
+
+class dv_request_info(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("status", C.c_int32),
+        ("index_id", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("n", C.c_uint64),
+        ("submit_us", C.c_double),
+        ("start_us", C.c_double),
+        ("done_us", C.c_double),
+    ]
+
+
 P = C.POINTER
 VP = C.c_void_p
+U32P = C.POINTER(C.c_uint32)
+DP = C.POINTER(C.c_double)
 
 # (name, restype, argtypes) of every function include/*.h declares
 SIGNATURES = [
@@ -112,8 +138,25 @@ SIGNATURES = [
     ("dv_range_power", C.c_int, [VP, P(dv_rcrs), C.c_uint32, C.c_uint32, P(dv_rcrs)]),
     ("dv_create_distance_index", C.c_int,
      [VP, P(dv_rcrs), C.c_uint32, C.c_uint32, C.c_uint32, P(dv_rcrs), P(dv_build_stats)]),
+    ("dv_create_distance_index_batched", C.c_int,
+     [VP, P(dv_rcrs), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P(dv_rcrs),
+      P(dv_build_stats)]),
     ("dv_query_batch", C.c_int,
      [VP, P(dv_rcrs), P(C.c_uint32), P(C.c_uint32), P(C.c_uint8), C.c_uint64, C.c_uint32]),
+    ("dv_server_start", C.c_int, [C.c_int, P(dv_server_opts), P(VP)]),
+    ("dv_server_stop", C.c_int, [VP]),
+    ("dv_server_add_index", C.c_int, [VP, P(dv_rcrs), U32P]),
+    ("dv_server_submit_queries", C.c_int,
+     [VP, C.c_uint32, U32P, U32P, C.c_uint64, P(C.c_uint64)]),
+    ("dv_server_submit_build", C.c_int,
+     [VP, P(dv_rcrs), C.c_uint32, C.c_uint32, P(C.c_uint64)]),
+    ("dv_server_poll", C.c_int, [VP, C.c_uint64, P(dv_request_info)]),
+    ("dv_server_wait", C.c_int, [VP, C.c_uint64, P(dv_request_info)]),
+    ("dv_server_results", C.c_int, [VP, C.c_uint64, P(C.c_uint8), C.c_uint64]),
+    ("dv_server_get_index", C.c_int, [VP, C.c_uint32, P(dv_rcrs)]),
+    ("dv_server_replay", C.c_int,
+     [VP, C.c_uint64, U32P, U32P, U32P, DP, P(C.c_uint8), DP, C.c_uint32, P(dv_rcrs),
+      U32P, U32P, DP, DP, U32P, DP]),
     ("dv_graph_preset", C.c_int, [C.c_uint32, C.c_uint32, P(dv_graph_params)]),
     ("dv_generate_graph", C.c_int,
      [P(dv_graph_params), C.c_uint32, P(dv_rcrs), P(P(C.c_uint32))]),

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <vector>
 #include <new>
 #include <string>
 #include <unordered_map>
@@ -108,7 +109,6 @@ DView view(const DevMat &m) { return DView{m.rowmap, m.entries, m.nrows, m.ncols
 
 using dv::Ctx;
 
-struct dv_context : public Ctx {};
 
 namespace {
 
@@ -387,6 +387,7 @@ int dv_finalize(dv_context *c) {
   for (auto &e : c->pev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c->plan_ev) (void)hipEventDestroy(e);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);
   if (c->h_status) (void)hipHostFree(c->h_status);
   if (c->d_status) (void)hipFree(c->d_status);
@@ -424,6 +425,7 @@ int dv_set_tuning(dv_context *c, const dv_tuning *t) {
   if (n.rows_per_group < 1) return C(c)->fail("rows_per_group must be >= 1");
   if (n.lane_slice_max > (1u << 20)) return C(c)->fail("lane_slice_max must be 0 .. 2^20");
   if (n.plan_mode > 2) return C(c)->fail("plan_mode must be 0, 1 or 2");
+  if (n.graphs > 1) return C(c)->fail("graphs must be 0 or 1");
   c->tuning = n;
   return DV_OK;
 }
@@ -588,6 +590,57 @@ int dv_create_distance_index(dv_context *ctx, const dv_rcrs *a, uint32_t dlo, ui
   rc = export_mat(c, dd, loc, out, &st.d2h_ms);
   if (stats) *stats = st;
   return rc;
+}
+
+int dv_create_distance_index_batched(dv_context *ctx, const dv_rcrs *comps, uint32_t ncomp,
+                                     uint32_t dlo, uint32_t dup, uint32_t loc, dv_rcrs *outs,
+                                     dv_build_stats *stats) {
+  if (!ctx || (ncomp && (!comps || !outs))) return DV_ERR_INVALID;
+  Ctx *c = C(ctx);
+  int rc;
+  if (!(dlo <= dup && dup != 0)) return c->fail("create_distance_index: need dlo <= dup and dup != 0");
+  for (uint32_t i = 0; i < ncomp; ++i) {
+    if ((rc = check_desc(c, &comps[i], "component"))) return rc;
+    if (comps[i].nrows != comps[i].ncols) return c->fail("batched: a component is not square");
+  }
+  dv_build_stats st{};
+  std::vector<DevMat> dcomps(ncomp), douts;
+  uint64_t input_bytes = 0;
+  for (uint32_t i = 0; i < ncomp; ++i) {
+    if ((rc = to_dev(c, &comps[i], dcomps[i], &st.h2d_ms))) {
+      for (auto &m : dcomps) c->release(m);
+      return rc;
+    }
+    if (comps[i].location == DV_DEVICE) input_bytes += dv::mat_bytes(dcomps[i]);
+  }
+  c->reset_stats();
+  c->peak_bytes = c->cur_bytes;
+  (void)hipEventRecord(c->ev[2], c->stream);
+  rc = c->dindex_batched(dcomps, dlo, dup, douts);
+  (void)hipEventRecord(c->ev[3], c->stream);
+  for (auto &m : dcomps) c->release(m);
+  if (rc) return rc;
+  (void)hipEventSynchronize(c->ev[3]);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+  st.build_ms = ms;
+  st.alg_bytes = c->stats_alg_bytes;
+  st.n_spgemm = c->stats_n_spgemm;
+  st.n_spadd = c->stats_n_spadd;
+  st.overflow_rows = c->stats_overflow_rows;
+  st.deferred_rows = c->stats_mid_rows + c->stats_big_rows;
+  st.planned = c->stats_planned;
+  st.retried = c->stats_retried;
+  st.peak_device_bytes = c->peak_bytes + input_bytes;
+  for (uint32_t i = 0; i < ncomp; ++i) {
+    if ((rc = export_mat(c, douts[i], loc, &outs[i], &st.d2h_ms))) {
+      for (uint32_t j = 0; j < i; ++j) dv_free(ctx, &outs[j]);
+      for (uint32_t j = i; j < ncomp; ++j) c->release(douts[j]);
+      return rc;
+    }
+  }
+  if (stats) *stats = st;
+  return DV_OK;
 }
 
 int dv_query_batch(dv_context *ctx, const dv_rcrs *m, const uint32_t *qi, const uint32_t *qj,

@@ -147,58 +147,139 @@ uint64_t Ctx::predict_cap(const Plan &p, const PlanSlot &s) const {
   }
 }
 
-// Enqueue every operation of the plan; no host synchronisation.
-int Ctx::plan_enqueue(Plan &p) {
+// Assign buffers to every slot (reusing those whose last reader comes
+// earlier), reserve scratch and counters.  All allocation happens here, so
+// plan_launch only enqueues kernels and can be captured into a graph.
+int Ctx::plan_prepare(Plan &p) {
   const int nops = (int)p.ops.size();
   if (!plan_ctr_reserve(nops)) return fail("device allocation failed (plan counters)", DV_ERR_NOMEM);
   uint64_t nmax = 0;
   for (const PlanOp &o : p.ops) nmax = std::max(nmax, p.slots[o.c].nrows);
-  OpScratch s;
-  if (!op_scratch(nmax, s)) return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
-  fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
-           (uint32_t)(nops * sizeof(OpCounters) / 4), 0u);
-  fill_u32(stream, d_status, 1, 0u);
-  const bool timing = prof && plan_events_reserve(nops);
+  if (!op_scratch(nmax, p.scratch))
+    return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
   for (int i = 0; i < nops; ++i) {
-    const PlanOp &o = p.ops[i];
-    PlanSlot &c = p.slots[o.c];
+    PlanSlot &c = p.slots[p.ops[i].c];
     c.cap = predict_cap(p, c);
     c.buf = buf_acquire((uint64_t)c.sb * (c.nrows + 1), c.cap);
     if (c.buf < 0) return fail("device allocation failed (plan buffers)", DV_ERR_NOMEM);
     c.rowmap = bufs[c.buf].rowmap;
     c.entries = bufs[c.buf].entries;
     // c.cap stays the prediction (a reused buffer may be larger)
+    if (p.ops[i].kind == P_IDENT) {
+      c.n_entries = 2 * c.nrows;
+      c.nnz = c.nrows;
+    }
+    // buffers whose last reader is this operation go back to the cache
+    for (int x = 0; x < (int)p.slots.size(); ++x) {
+      PlanSlot &sl = p.slots[x];
+      if (!sl.external && sl.buf >= 0 && sl.last_use == i && x != p.out && !sl.released) {
+        bufs[sl.buf].in_use = false;
+        sl.released = true;
+      }
+    }
+  }
+  return DV_OK;
+}
+
+// Enqueue the plan's kernels (no allocation, no synchronisation).
+void Ctx::plan_launch(Plan &p, bool timing) {
+  const int nops = (int)p.ops.size();
+  const OpScratch &s = p.scratch;
+  fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
+           (uint32_t)(nops * sizeof(OpCounters) / 4), 0u);
+  fill_u32(stream, d_status, 1, 0u);
+  for (int i = 0; i < nops; ++i) {
+    const PlanOp &o = p.ops[i];
+    PlanSlot &c = p.slots[o.c];
     OpCounters *ctr = d_plan_ctr + i;
     if (timing) (void)hipEventRecord(plan_ev[4 * i], stream);
     if (o.kind == P_IDENT) {
       identity(stream, c.sb, c.nrows, c.rowmap, c.entries);
-      c.n_entries = 2 * c.nrows;
-      c.nnz = c.nrows;
       if (timing)
         for (int e = 1; e < 4; ++e) (void)hipEventRecord(plan_ev[4 * i + e], stream);
-    } else {
-      const DevMat a = p.slots[o.a].as_mat(), b = p.slots[o.b].as_mat();
-      if (o.kind == P_SPADD) {
-        enq_spadd_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status);
-        if (timing) (void)hipEventRecord(plan_ev[4 * i + 1], stream);
-        if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
-        enq_spadd_write(a, b, c.rowmap, c.entries, c.cap, ctr, d_status);
-      } else {
-        enq_spgemm_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status,
-                         timing ? plan_ev[4 * i + 1] : nullptr);
-        if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
-        enq_spgemm_write(a, b, c.rowmap, c.entries, s, ctr, d_status);
-      }
-      if (timing) (void)hipEventRecord(plan_ev[4 * i + 3], stream);
+      continue;
     }
-    // buffers whose last reader is this operation go back to the cache
-    for (PlanSlot &x : p.slots)
-      if (!x.external && x.buf >= 0 && x.last_use == i && &x != &p.slots[p.out]) {
-        bufs[x.buf].in_use = false;
-        x.released = true;
-      }
+    const DevMat a = p.slots[o.a].as_mat(), b = p.slots[o.b].as_mat();
+    if (o.kind == P_SPADD) {
+      enq_spadd_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status);
+      if (timing) (void)hipEventRecord(plan_ev[4 * i + 1], stream);
+      if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
+      enq_spadd_write(a, b, c.rowmap, c.entries, c.cap, ctr, d_status);
+    } else {
+      enq_spgemm_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status,
+                       timing ? plan_ev[4 * i + 1] : nullptr);
+      if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
+      enq_spgemm_write(a, b, c.rowmap, c.entries, s, ctr, d_status);
+    }
+    if (timing) (void)hipEventRecord(plan_ev[4 * i + 3], stream);
   }
-  return hip(hipGetLastError(), "plan launch") ? DV_OK : DV_ERR_DEVICE;
+}
+
+// A fingerprint of everything a captured plan depends on: operations,
+// buffers, shapes, scratch, counters and the launch tuning.
+static uint64_t plan_key(const Plan &p, const dv_tuning &t, const void *ctr, const void *status) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) {
+    h ^= v;
+    h *= 1099511628211ull;
+  };
+  for (const PlanOp &o : p.ops) {
+    mix(o.kind);
+    mix((uint64_t)(int64_t)o.a);
+    mix((uint64_t)(int64_t)o.b);
+  }
+  for (const PlanSlot &sl : p.slots) {
+    mix((uint64_t)sl.rowmap);
+    mix((uint64_t)sl.entries);
+    mix(sl.cap);
+    mix(sl.nrows);
+    mix(sl.ncols);
+    mix(sl.sb);
+    mix(sl.n_entries);
+  }
+  const OpScratch &s = p.scratch;
+  mix((uint64_t)s.cnt);
+  mix((uint64_t)s.stage);
+  mix((uint64_t)s.tmp);
+  mix(s.stage_entries);
+  const uint32_t *tw = reinterpret_cast<const uint32_t *>(&t);
+  for (size_t i = 0; i < sizeof(dv_tuning) / 4; ++i) mix(tw[i]);
+  mix((uint64_t)ctr);
+  mix((uint64_t)status);
+  return h;
+}
+
+// Enqueue every operation of the plan; no host synchronisation.  With
+// use_graph the launches are captured once into a hipGraph and replayed while
+// the fingerprint stays the same.
+int Ctx::plan_enqueue(Plan &p) {
+  int rc = plan_prepare(p);
+  if (rc) return rc;
+  const int nops = (int)p.ops.size();
+  const bool timing = prof && plan_events_reserve(nops);
+  if (!tuning.graphs || timing) {
+    plan_launch(p, timing);
+    return hip(hipGetLastError(), "plan launch") ? DV_OK : DV_ERR_DEVICE;
+  }
+  const uint64_t key = plan_key(p, tuning, d_plan_ctr, d_status);
+  if (!graph_exec || key != graph_key) {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+    hipGraph_t g = nullptr;
+    if (!hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture"))
+      return DV_ERR_DEVICE;
+    plan_launch(p, false);
+    if (!hip(hipStreamEndCapture(stream, &g), "end capture")) return DV_ERR_DEVICE;
+    const bool ok = hip(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0), "instantiate");
+    (void)hipGraphDestroy(g);
+    if (!ok) {
+      graph_exec = nullptr;
+      return DV_ERR_DEVICE;
+    }
+    graph_key = key;
+    graph_captures += 1;
+  }
+  return hip(hipGraphLaunch(graph_exec, stream), "graph launch") ? DV_OK : DV_ERR_DEVICE;
 }
 
 // Read the plan's counters back (one synchronisation) and fill in the slots.

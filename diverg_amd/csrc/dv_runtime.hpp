@@ -80,6 +80,7 @@ struct Plan {
   std::vector<PlanSlot> slots;
   std::vector<PlanOp> ops;
   int out = -1;
+  OpScratch scratch;  // set by Ctx::plan_prepare
   int add_slot(uint64_t nrows, uint64_t ncols, uint32_t sb);
   int external(const DevMat &m);
   int op(PlanKind kind, int a, int b, uint64_t nrows, uint64_t ncols, uint32_t sb);
@@ -104,7 +105,7 @@ struct Ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t num_cu = 256;
   uint32_t lds_per_block = 65536;
-  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0};
+  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0, 0};
   std::string last_error;
 
   // pool accounting
@@ -130,6 +131,10 @@ struct Ctx {
   OpCounters *d_plan_ctr = nullptr, *h_plan_ctr = nullptr;
   int plan_ctr_cap = 0;
   std::vector<hipEvent_t> plan_ev;
+  // hipGraph of the last planned build (replayed while its fingerprint holds)
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_key = 0;
+  uint64_t graph_captures = 0;
 
   // per-kernel timing (dv_profile)
   bool prof = false;
@@ -198,11 +203,20 @@ struct Ctx {
   uint64_t predict_cap(const Plan &p, const PlanSlot &s) const;
   bool plan_ctr_reserve(int nops);
   bool plan_events_reserve(int nops);
+  int plan_prepare(Plan &p);
+  void plan_launch(Plan &p, bool timing);
   int plan_enqueue(Plan &p);
   int plan_finish(Plan &p, uint32_t &status);
   int plan_run(Plan &p, DevMat &out, const std::function<int(DevMat &)> &exact);
   int power_dev(const DevMat &a, uint32_t k, DevMat &out);
   int dindex_dev(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
+
+  // dv_batch.hip
+  int dindex_batched(const std::vector<DevMat> &comps, uint32_t dlo, uint32_t dup,
+                     std::vector<DevMat> &outs);
 };
 
 }  // namespace dv
+
+// The C ABI's opaque context is the library context.
+struct dv_context : public dv::Ctx {};

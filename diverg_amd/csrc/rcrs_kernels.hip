@@ -760,6 +760,47 @@ __global__ __launch_bounds__(BLOCK) void k_slice_lengths(Mat<S> A, Mat<S> B, uin
   }
 }
 
+// ---------------------------------------------------------------------------
+// Serving: one launch answers a batch of (index, i, j) queries against a
+// table of resident indexes (each with its own rowmap width).
+__global__ __launch_bounds__(BLOCK) void k_query_multi(const IndexDesc *__restrict__ table,
+                                                       uint32_t ntable,
+                                                       const uint3 *__restrict__ q, uint32_t n,
+                                                       uint8_t *__restrict__ out) {
+  for (uint32_t t = blockIdx.x * BLOCK + threadIdx.x; t < n; t += gridDim.x * BLOCK) {
+    const uint3 v = q[t];  // x: index id, y: i, z: j
+    if (v.x >= ntable) {
+      out[t] = 2;
+      continue;
+    }
+    const IndexDesc d = table[v.x];
+    if (v.y >= d.nrows || v.z >= d.ncols) {
+      out[t] = 2;
+      continue;
+    }
+    uint64_t b, e;
+    if (d.sb == 4) {
+      b = static_cast<const uint32_t *>(d.rowmap)[v.y];
+      e = static_cast<const uint32_t *>(d.rowmap)[v.y + 1];
+    } else {
+      b = static_cast<const uint64_t *>(d.rowmap)[v.y];
+      e = static_cast<const uint64_t *>(d.rowmap)[v.y + 1];
+    }
+    uint64_t first = b, len = e - b;
+    while (len > 0) {
+      const uint64_t half = len >> 1;
+      if (d.entries[first + half] < v.z) {
+        first += half + 1;
+        len -= half + 1;
+      } else {
+        len = half;
+      }
+    }
+    if ((first - b) & 1) --first;
+    out[t] = (uint8_t)(first != e && d.entries[first] <= v.z);
+  }
+}
+
 // Input validation for device-resident matrices handed to the library.
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_validate(Mat<S> M, uint64_t nrows, uint64_t ncols,
@@ -777,6 +818,54 @@ __global__ __launch_bounds__(BLOCK) void k_validate(Mat<S> M, uint64_t nrows, ui
     const uint2 v = r[k];
     if (v.x > v.y || v.y >= ncols) *flag = 1u;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Block-diagonal assembly of independent components and the split back.
+// Component rows [row0, row0 + n) of the assembly hold the component's rows
+// with every column shifted by col0 and every rowmap offset by ent0.
+template <typename SI>
+__global__ __launch_bounds__(BLOCK) void k_concat(const SI *__restrict__ rm, const uint32_t *__restrict__ en,
+                                                  uint64_t n, uint64_t n_entries, uint64_t row0,
+                                                  uint64_t ent0, uint32_t col0,
+                                                  uint64_t *__restrict__ out_rm,
+                                                  uint32_t *__restrict__ out_en, int write_last) {
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  const uint64_t t0 = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  for (uint64_t i = t0; i < n + (write_last ? 1 : 0); i += stride) out_rm[row0 + i] = ent0 + rm[i];
+  for (uint64_t k = t0; k < n_entries; k += stride) out_en[ent0 + k] = en[k] + col0;
+}
+
+template <typename SO>
+__global__ __launch_bounds__(BLOCK) void k_extract(const uint64_t *__restrict__ rm,
+                                                   const uint32_t *__restrict__ en, uint64_t row0,
+                                                   uint64_t n, uint32_t col0,
+                                                   SO *__restrict__ out_rm,
+                                                   uint32_t *__restrict__ out_en) {
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  const uint64_t t0 = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const uint64_t base = rm[row0], len = rm[row0 + n] - base;
+  for (uint64_t i = t0; i <= n; i += stride) out_rm[i] = (SO)(rm[row0 + i] - base);
+  for (uint64_t k = t0; k < len; k += stride) out_en[k] = en[base + k] - col0;
+}
+
+// out[c] = rowmap[bounds[c]] for c <= ncomp (component entry offsets).
+__global__ void k_gather_bounds(const uint64_t *__restrict__ rm, const uint64_t *__restrict__ bounds,
+                                uint32_t m, unsigned long long *__restrict__ out) {
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < m; c += gridDim.x * blockDim.x)
+    out[c] = rm[bounds[c]];
+}
+
+// nnz of the ranges entries[e0 .. e1) (sum of e - s + 1) into *out.
+__global__ __launch_bounds__(BLOCK) void k_slice_nnz(const uint2 *__restrict__ r, uint64_t nr,
+                                                     unsigned long long *__restrict__ out) {
+  unsigned long long s = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nr;
+       i += (uint64_t)gridDim.x * BLOCK) {
+    const uint2 v = r[i];
+    s += (unsigned long long)v.y - v.x + 1;
+  }
+  block_add_u64(s, out);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *__restrict__ p, uint64_t n,
@@ -943,6 +1032,51 @@ void query(hipStream_t s, uint32_t sb, DView m, const uint32_t *qi, const uint32
   else
     hipLaunchKernelGGL(k_query<uint64_t>, dim3(grid_for(n)), dim3(BLOCK), 0, s,
                        mat<uint64_t>(m), m.nrows, m.ncols, qi, qj, out, n, bad);
+}
+
+void concat_component(hipStream_t s, uint32_t sb, const void *rm, const uint32_t *en, uint64_t n,
+                      uint64_t n_entries, uint64_t row0, uint64_t ent0, uint32_t col0,
+                      uint64_t *out_rm, uint32_t *out_en, bool write_last) {
+  const uint64_t work = std::max<uint64_t>(n + 1, n_entries);
+  if (sb == 4)
+    hipLaunchKernelGGL(k_concat<uint32_t>, dim3(grid_for(work)), dim3(BLOCK), 0, s,
+                       static_cast<const uint32_t *>(rm), en, n, n_entries, row0, ent0, col0,
+                       out_rm, out_en, write_last ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_concat<uint64_t>, dim3(grid_for(work)), dim3(BLOCK), 0, s,
+                       static_cast<const uint64_t *>(rm), en, n, n_entries, row0, ent0, col0,
+                       out_rm, out_en, write_last ? 1 : 0);
+}
+
+void extract_component(hipStream_t s, uint32_t sb, const uint64_t *rm, const uint32_t *en,
+                       uint64_t row0, uint64_t n, uint64_t n_entries, uint32_t col0,
+                       void *out_rm, uint32_t *out_en) {
+  const uint64_t work = std::max<uint64_t>(n + 1, n_entries);
+  if (sb == 4)
+    hipLaunchKernelGGL(k_extract<uint32_t>, dim3(grid_for(work)), dim3(BLOCK), 0, s, rm, en, row0,
+                       n, col0, static_cast<uint32_t *>(out_rm), out_en);
+  else
+    hipLaunchKernelGGL(k_extract<uint64_t>, dim3(grid_for(work)), dim3(BLOCK), 0, s, rm, en, row0,
+                       n, col0, static_cast<uint64_t *>(out_rm), out_en);
+}
+
+void gather_bounds(hipStream_t s, const uint64_t *rm, const uint64_t *bounds, uint32_t m,
+                   unsigned long long *out) {
+  hipLaunchKernelGGL(k_gather_bounds, dim3(1), dim3(BLOCK), 0, s, rm, bounds, m, out);
+}
+
+void slice_nnz(hipStream_t s, const uint32_t *entries, uint64_t n_entries,
+               unsigned long long *out) {
+  if (n_entries < 2) return;
+  hipLaunchKernelGGL(k_slice_nnz, dim3(grid_for(n_entries / 2, BLOCK * 8, 4096)), dim3(BLOCK), 0,
+                     s, reinterpret_cast<const uint2 *>(entries), n_entries / 2, out);
+}
+
+void query_multi(hipStream_t s, const IndexDesc *table, uint32_t ntable, const void *q, uint32_t n,
+                 uint8_t *out) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_query_multi, dim3(grid_for(n, BLOCK, 1024)), dim3(BLOCK), 0, s, table,
+                     ntable, static_cast<const uint3 *>(q), n, out);
 }
 
 void validate_matrix(hipStream_t s, uint32_t sb, DView m, uint64_t n_entries, uint32_t *flag) {

@@ -92,6 +92,38 @@ void spgemm_emit_staged(hipStream_t s, uint32_t sb, uint64_t nrows,
 void query(hipStream_t s, uint32_t sb, DView m, const uint32_t *qi,
            const uint32_t *qj, uint8_t *out, uint64_t n, uint32_t *bad);
 
+// Component `c` of a block-diagonal assembly (uint64 rowmap): its rows go to
+// [row0, row0 + n) with rowmap offsets + ent0 and columns + col0;
+// write_last also writes rowmap[row0 + n].
+void concat_component(hipStream_t s, uint32_t sb, const void *rm, const uint32_t *en, uint64_t n,
+                      uint64_t n_entries, uint64_t row0, uint64_t ent0, uint32_t col0,
+                      uint64_t *out_rm, uint32_t *out_en, bool write_last);
+// The inverse: rows [row0, row0 + n) of the assembly, columns - col0, into a
+// component matrix with a size_bytes = sb rowmap.
+void extract_component(hipStream_t s, uint32_t sb, const uint64_t *rm, const uint32_t *en,
+                       uint64_t row0, uint64_t n, uint64_t n_entries, uint32_t col0,
+                       void *out_rm, uint32_t *out_en);
+// out[c] = rm[bounds[c]] for c < m.
+void gather_bounds(hipStream_t s, const uint64_t *rm, const uint64_t *bounds, uint32_t m,
+                   unsigned long long *out);
+// *out += sum of (e - s + 1) over entries[0 .. n_entries).
+void slice_nnz(hipStream_t s, const uint32_t *entries, uint64_t n_entries,
+               unsigned long long *out);
+
+// A resident index as the serving query kernel sees it.
+struct IndexDesc {
+  const void *rowmap;
+  const uint32_t *entries;
+  uint64_t nrows;
+  uint64_t ncols;
+  uint32_t sb;
+  uint32_t pad;
+};
+// out[t] = D_q(i, j) for the packed queries q[t] = (index id, i, j) (12
+// bytes each); 2 for an unknown index or (i, j) outside it.
+void query_multi(hipStream_t s, const IndexDesc *table, uint32_t ntable, const void *q, uint32_t n,
+                 uint8_t *out);
+
 // Sets *flag when rowmap[0] != 0, rowmap[n] != n_entries, a row length is
 // odd or negative, or a range is not s <= e < ncols.
 void validate_matrix(hipStream_t s, uint32_t sb, DView m, uint64_t n_entries, uint32_t *flag);

@@ -115,7 +115,10 @@ void *dv_stream(dv_context *ctx);
  *                   (exact fallback on overflow); 1: one readback per
  *                   operation, as the reference's drivers; 2: as 0 with a
  *                   capacity prediction too small to hold any product, which
- *                   always takes the fallback (for tests) */
+ *                   always takes the fallback (for tests)
+ *   graphs          1: the launches of a planned build are captured once into
+ *                   a hipGraph and replayed while the build's operations,
+ *                   buffers and tuning stay the same (not while profiling) */
 typedef struct dv_tuning {
   uint32_t lds_words;
   uint32_t stage_entries;
@@ -125,6 +128,7 @@ typedef struct dv_tuning {
   uint32_t lane_slice_max;
   uint32_t autotune;
   uint32_t plan_mode;
+  uint32_t graphs;
 } dv_tuning;
 int dv_get_tuning(const dv_context *ctx, dv_tuning *t);
 int dv_set_tuning(dv_context *ctx, const dv_tuning *t);
@@ -183,6 +187,17 @@ int dv_create_distance_index(dv_context *ctx, const dv_rcrs *a, uint32_t dlo,
                              uint32_t dup, uint32_t out_location, dv_rcrs *d,
                              dv_build_stats *stats);
 
+/* Distance indexes of `ncomp` independent graph components in one planned
+ * build over their block-diagonal assembly: each power step is one launch
+ * sequence covering every component (replayed from a hipGraph with
+ * dv_tuning.graphs).  outs[c] is bit-identical to dv_create_distance_index
+ * on comps[c] (the block assembly of test_dindex.cpp:111-173).  The
+ * components together may hold at most UINT32_MAX - 1 loci.  No counterpart
+ * entry point in the reference, which builds one graph per call. */
+int dv_create_distance_index_batched(dv_context *ctx, const dv_rcrs *comps, uint32_t ncomp,
+                                     uint32_t dlo, uint32_t dup, uint32_t out_location,
+                                     dv_rcrs *outs, dv_build_stats *stats);
+
 /* out[q] = D(qi[q], qj[q]) for q < n.  CRSMatrixBase::operator()(i, j)
  * (crs_matrix.hpp:1197-1218) via binary_search(..., RangeGroup)
  * (crs_matrix.hpp:149-169).  qi, qj, out are host or device pointers
@@ -192,6 +207,70 @@ int dv_create_distance_index(dv_context *ctx, const dv_rcrs *a, uint32_t dlo,
 int dv_query_batch(dv_context *ctx, const dv_rcrs *m, const uint32_t *qi,
                    const uint32_t *qj, uint8_t *out, uint64_t n,
                    uint32_t query_location);
+
+/* --- serving --------------------------------------------------------------
+ * A serving loop over HBM-resident indexes (no counterpart in the reference,
+ * whose queries are host binary searches, crs_matrix.hpp:149-169).
+ * Continuous batching: queries that arrive while earlier batches are on the
+ * GPU are packed into the next launch (up to max_inflight batches in flight),
+ * on a query stream that may be pinned to a few CUs so builds cannot delay it.
+ * Index builds run on a second stream; a build is admitted when its predicted
+ * HBM footprint fits the budget, and its index joins the query table when it
+ * completes.  Answers equal dv_query_batch on the same index. */
+typedef struct dv_server dv_server;
+
+typedef struct dv_server_opts {
+  uint64_t hbm_budget_bytes; /* indexes + running builds; 0: 75% of free HBM */
+  uint32_t max_batch;        /* queries per launch; 0: 65536 */
+  uint32_t max_inflight;     /* batches in flight; 0: 2 */
+  uint32_t query_cus;        /* CUs the query stream is restricted to (0: all) */
+  uint32_t pad;
+} dv_server_opts;
+
+typedef struct dv_request_info {
+  uint32_t kind;      /* 0 queries, 1 build */
+  int32_t status;     /* dv_status of the finished request; DV_ERR_BUSY while pending */
+  uint32_t index_id;  /* queries: the index asked; build: the index produced */
+  uint32_t pad;
+  uint64_t n;         /* queries in the request */
+  double submit_us;   /* server clock, microseconds */
+  double start_us;    /* first launch (queries) / admission (build) */
+  double done_us;
+} dv_request_info;
+
+int dv_server_start(int device, const dv_server_opts *opts, dv_server **srv);
+int dv_server_stop(dv_server *srv);
+/* Copy an index to HBM and make it queryable; *index_id names it. */
+int dv_server_add_index(dv_server *srv, const dv_rcrs *index, uint32_t *index_id);
+/* Queue n locus pairs against one index; returns at once. */
+int dv_server_submit_queries(dv_server *srv, uint32_t index_id, const uint32_t *qi,
+                             const uint32_t *qj, uint64_t n, uint64_t *ticket);
+/* Queue a distance-index build of `a` (host or device resident; copied before
+ * the call returns); DV_ERR_BUDGET if it can never fit the budget. */
+int dv_server_submit_build(dv_server *srv, const dv_rcrs *a, uint32_t dlo, uint32_t dup,
+                           uint64_t *ticket);
+/* DV_OK and *info when the request is finished, DV_ERR_BUSY while pending. */
+int dv_server_poll(dv_server *srv, uint64_t ticket, dv_request_info *info);
+int dv_server_wait(dv_server *srv, uint64_t ticket, dv_request_info *info);
+/* Answers of a finished query request (n bytes: 1 hit, 0 miss, 2 out of range). */
+int dv_server_results(dv_server *srv, uint64_t ticket, uint8_t *out, uint64_t n);
+/* Copy an index the server holds (e.g. a finished build) to the host. */
+int dv_server_get_index(dv_server *srv, uint32_t index_id, dv_rcrs *out);
+
+/* Load generator: replays a query stream against the server in real time.
+ * Query q (index qidx[q], pair (qi[q], qj[q])) arrives at arrival_us[q]
+ * (ascending, microseconds from the start); build b of builds[b] arrives at
+ * build_us[b].  Queries are submitted as they fall due, in arrival order;
+ * the call returns when every query and build has finished.  answers[q] gets
+ * the answer, latency_us[q] the time from arrival to the answer being on the
+ * host; build_done_us[b] the completion time of build b and build_index[b]
+ * the index it produced.  *elapsed_us is the replay's wall time. */
+int dv_server_replay(dv_server *srv, uint64_t n, const uint32_t *qidx, const uint32_t *qi,
+                     const uint32_t *qj, const double *arrival_us, uint8_t *answers,
+                     double *latency_us, uint32_t nbuilds, const dv_rcrs *builds,
+                     const uint32_t *build_dlo, const uint32_t *build_dup,
+                     const double *build_us, double *build_done_us, uint32_t *build_index,
+                     double *elapsed_us);
 
 #ifdef __cplusplus
 }
