@@ -55,6 +55,7 @@ class dv_tuning(C.Structure):
         ("autotune", C.c_uint32),
         ("plan_mode", C.c_uint32),
         ("graphs", C.c_uint32),
+        ("summary_span", C.c_uint32),
     ]
 
 

@@ -30,9 +30,11 @@ int Ctx::identity_dev(uint64_t n, uint32_t sb, DevMat &out) {
 bool Ctx::op_scratch(uint64_t n, OpScratch &s) {
   const uint64_t n1 = n ? n : 1;
   const uint32_t KS = tuning.stage_entries;
-  void *buf[6];
-  const uint64_t sz[6] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * scan_tmp_words(n)};
-  if (!scratch_carve(sz, buf, 6)) return false;
+  void *buf[7];
+  const uint64_t sz[7] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * lookback_words(n),
+                          8 * summary_words(n1)};
+  if (!scratch_carve(sz, buf, 7)) return false;
+  s.summ = buf[6];
   s.cnt = static_cast<uint32_t *>(buf[0]);
   s.stage = static_cast<uint32_t *>(buf[1]);
   s.ovf = static_cast<uint32_t *>(buf[2]);
@@ -60,7 +62,8 @@ void Ctx::enq_spadd_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint
                           const OpScratch &s, OpCounters *ctr, uint64_t cap,
                           unsigned int *status) {
   spadd_count(stream, a.sb, view(a), view(b), s.cnt, status);
-  scan_counts(stream, c_sb, s.cnt, a.nrows, c_rowmap, s.tmp, ctr, cap, status);
+  scan_lookback(stream, c_sb, s.cnt, a.nrows, c_rowmap, s.tmp, ctr, cap, status, nullptr, 0,
+                nullptr);
 }
 
 void Ctx::enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
@@ -72,20 +75,23 @@ void Ctx::enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap
 
 // ---------------------------------------------------------------------------
 // SpGEMM halves: the lane tier, the two group tiers over the rows it hands
-// over, the row-pointer scan; then the staged copy and the direct rewrite of
-// rows whose output exceeds the stage.
+// over, the row-pointer scan (which also copies the staged rows when C's
+// entries buffer is given); then the staged copy if not done yet and the
+// direct rewrite of rows whose output exceeds the stage.
 void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                            const OpScratch &s, OpCounters *ctr, uint64_t cap,
-                           unsigned int *status, hipEvent_t after_lane) {
+                           unsigned int *status, hipEvent_t after_lane, uint32_t *c_entries,
+                           bool summaries) {
   const uint64_t n = a.nrows;
   if (n) {
+    if (summaries) block_summary(stream, b.sb, view(b), s.summ, status);
     // persistent lane-tier grid: few blocks keep the per-block counter atomics
     // (nnz, slice ranges) uncontended
     const uint32_t lane_grid =
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)num_cu * 8));
     spgemm_lane(stream, a.sb, view(a), view(b), s.stage_entries, tuning.lane_slice_max,
                 tuning.small_slice_max, lane_grid, s.cnt, s.stage, s.ovf, s.mid, s.big, ctr,
-                status);
+                status, summaries ? s.summ : nullptr, tuning.summary_span);
     if (after_lane) (void)hipEventRecord(after_lane, stream);
     GroupLaunch g = group_launch(tuning.lanes_small, n);
     g.rows = s.mid;
@@ -104,15 +110,17 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
   } else if (after_lane) {
     (void)hipEventRecord(after_lane, stream);
   }
-  scan_counts(stream, c_sb, s.cnt, n, c_rowmap, s.tmp, ctr, cap, status);
+  scan_lookback(stream, c_sb, s.cnt, n, c_rowmap, s.tmp, ctr, cap, status, s.stage,
+                s.stage_entries, c_entries);
 }
 
 void Ctx::enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
                            uint32_t *c_entries, const OpScratch &s, OpCounters *ctr,
-                           const unsigned int *status) {
+                           const unsigned int *status, bool staged_copied) {
   const uint64_t n = a.nrows;
   if (!n) return;
-  spgemm_emit_staged(stream, a.sb, n, c_rowmap, s.stage, s.stage_entries, c_entries, status);
+  if (!staged_copied)
+    spgemm_emit_staged(stream, a.sb, n, c_rowmap, s.stage, s.stage_entries, c_entries, status);
   GroupLaunch g = group_launch(64, n);
   g.rows = s.ovf;
   g.n_dev = &ctr->overflow;
@@ -149,6 +157,10 @@ int Ctx::spadd_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   if (!read_counters(h, &st)) {
     release(c);
     return DV_ERR_DEVICE;
+  }
+  if (st & 4) {
+    release(c);
+    return fail("spadd: row-pointer scan look-back did not complete", DV_ERR_DEVICE);
   }
   if (st & 2) {
     release(c);
@@ -202,6 +214,10 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
     release(c);
     return DV_ERR_DEVICE;
   }
+  if (st & 4) {
+    release(c);
+    return fail("spgemm: row-pointer scan look-back did not complete", DV_ERR_DEVICE);
+  }
   if (st & 2) {
     release(c);
     return fail("spgemm: entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
@@ -214,7 +230,7 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
     return fail("device allocation failed (spgemm entries)", DV_ERR_NOMEM);
   }
   prof_mark(3);
-  enq_spgemm_write(a, b, c.rowmap, c.entries, s, d_ctr, d_status);
+  enq_spgemm_write(a, b, c.rowmap, c.entries, s, d_ctr, d_status, false);
   prof_mark(4);
   if (!hip(hipGetLastError(), "spgemm write launch")) {
     release(c);

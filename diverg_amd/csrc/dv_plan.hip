@@ -44,6 +44,10 @@ int Plan::op(PlanKind kind, int a, int b, uint64_t nrows, uint64_t ncols, uint32
   const int c = add_slot(nrows, ncols, sb);
   slots[c].producer = (int)ops.size();
   slots[c].kind = kind;
+  // how many rows a range of the result may cover: I and the input 1,
+  // I + A one more, and a product the sum of its operands' spans less one
+  if (kind == P_SPADD) slots[c].span = std::max(slots[a].span, slots[b].span) + 1;
+  if (kind == P_SPGEMM) slots[c].span = slots[a].span + slots[b].span - 1;
   ops.push_back(PlanOp{kind, a, b, c});
   if (a >= 0) slots[a].last_use = (int)ops.size() - 1;
   if (b >= 0) slots[b].last_use = (int)ops.size() - 1;
@@ -206,10 +210,12 @@ void Ctx::plan_launch(Plan &p, bool timing) {
       if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
       enq_spadd_write(a, b, c.rowmap, c.entries, c.cap, ctr, d_status);
     } else {
+      const bool summaries =
+          tuning.summary_span && p.slots[o.a].span >= tuning.summary_span;
       enq_spgemm_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status,
-                       timing ? plan_ev[4 * i + 1] : nullptr);
+                       timing ? plan_ev[4 * i + 1] : nullptr, c.entries, summaries);
       if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
-      enq_spgemm_write(a, b, c.rowmap, c.entries, s, ctr, d_status);
+      enq_spgemm_write(a, b, c.rowmap, c.entries, s, ctr, d_status, true);
     }
     if (timing) (void)hipEventRecord(plan_ev[4 * i + 3], stream);
   }
@@ -358,6 +364,10 @@ int Ctx::plan_run(Plan &p, DevMat &out, const std::function<int(DevMat &)> &exac
   if (rc) {
     release_all();
     return rc;
+  }
+  if (st & 4) {
+    release_all();
+    return fail("row-pointer scan look-back did not complete", DV_ERR_DEVICE);
   }
   if (st & 2) {
     release_all();

@@ -35,7 +35,8 @@ struct OpScratch {
   uint32_t *ovf = nullptr;    // rows whose output exceeds the stage
   uint32_t *mid = nullptr;    // rows for the medium groups
   uint32_t *big = nullptr;    // rows for the 64-lane groups
-  unsigned long long *tmp = nullptr;  // scan tile sums
+  unsigned long long *tmp = nullptr;  // scan tile words
+  void *summ = nullptr;               // block summaries of the right operand
   uint32_t stage_entries = 0;
 };
 
@@ -52,6 +53,7 @@ struct PlanSlot {
   uint64_t cap = 0;         // entries capacity
   uint64_t n_entries = 0;   // known after the readback (static for I and inputs)
   uint64_t nnz = 0;
+  uint64_t span = 1;        // rows an A range of this matrix may cover (static bound)
   int producer = -1;        // index of the op writing the slot
   int last_use = -1;        // index of the last op reading the slot
   int buf = -1;             // context buffer backing the slot
@@ -105,7 +107,7 @@ struct Ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t num_cu = 256;
   uint32_t lds_per_block = 65536;
-  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0, 0};
+  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0, 0, 64};
   std::string last_error;
 
   // pool accounting
@@ -184,13 +186,15 @@ struct Ctx {
   void enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
                        uint32_t *c_entries, uint64_t cap, OpCounters *ctr,
                        const unsigned int *status);
-  // after_lane (optional) is recorded right after the lane-tier kernel
+  // after_lane (optional) is recorded right after the lane-tier kernel;
+  // with c_entries the scan also copies the staged rows into C
   void enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                         const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status,
-                        hipEvent_t after_lane = nullptr);
+                        hipEvent_t after_lane = nullptr, uint32_t *c_entries = nullptr,
+                        bool summaries = false);
   void enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
                         uint32_t *c_entries, const OpScratch &s, OpCounters *ctr,
-                        const unsigned int *status);
+                        const unsigned int *status, bool staged_copied);
   int identity_dev(uint64_t n, uint32_t sb, DevMat &out);
   int spadd_dev(const DevMat &a, const DevMat &b, DevMat &c);
   int spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c);

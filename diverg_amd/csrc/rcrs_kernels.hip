@@ -221,13 +221,81 @@ __device__ __forceinline__ bool insert(Runs &r, uint32_t s, uint32_t e) {
   return done;
 }
 
+// Fold one range into the running interval [cl, ch] / the slots.
+__device__ __forceinline__ void fold(uint32_t vs, uint32_t ve, uint32_t &cl, uint32_t &ch,
+                                     Runs &r, bool &ok) {
+  if (vs > ve) return;
+  if (vs <= ch + 1u && ve + 1u >= cl) {
+    cl = min(cl, vs);
+    ch = max(ch, ve);
+  } else {
+    if (cl <= ch && ok && !insert(r, cl, ch)) {
+      compact(r);
+      ok = insert(r, cl, ch);
+    }
+    cl = vs;
+    ch = ve;
+  }
+}
+
+// Fold the flat run of ranges en[p0 .. p1) with LANE_BATCH loads in flight.
+__device__ __forceinline__ void fold_segment(const uint32_t *__restrict__ en, uint64_t p0,
+                                             uint64_t p1, uint32_t &cl, uint32_t &ch, Runs &r,
+                                             bool &ok, unsigned long long &gathered) {
+  gathered += (p1 - p0) >> 1;
+  for (uint64_t p = p0; ok && p < p1; p += 2 * LANE_BATCH) {
+    uint2 v[LANE_BATCH];
+#pragma unroll
+    for (int k = 0; k < LANE_BATCH; ++k)
+      v[k] = p + 2 * k < p1 ? ld_range(en, p + 2 * k) : make_uint2(1u, 0u);
+#pragma unroll
+    for (int k = 0; k < LANE_BATCH; ++k) fold(v[k].x, v[k].y, cl, ch, r, ok);
+  }
+}
+
+// Block summaries: summ[b] describes B rows [32b, 32b + 32): (lo, hi) when the
+// union of their ranges is the single interval [lo, hi] (each range, in row
+// order, touches the union of the ranges before it), (0xFFFFFFFF, 0) when the
+// rows are empty, (1, 0) otherwise.
+constexpr int SUMM_ROWS = 32;
+constexpr uint32_t SUMM_EMPTY = 0xFFFFFFFFu;
+
+template <typename S>
+__global__ __launch_bounds__(BLOCK) void k_block_summary(Mat<S> B, uint64_t nrows,
+                                                         uint2 *__restrict__ summ,
+                                                         const unsigned int *__restrict__ status) {
+  if (*status) return;
+  const uint64_t nb = (nrows + SUMM_ROWS - 1) / SUMM_ROWS;
+  for (uint64_t b = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; b < nb;
+       b += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t r0 = b * SUMM_ROWS, r1 = min(r0 + SUMM_ROWS, nrows);
+    const uint64_t p1 = B.rm[r1];
+    uint32_t lo = SUMM_EMPTY, hi = 0;
+    bool valid = true;
+    for (uint64_t p = B.rm[r0]; valid && p < p1; p += 2) {
+      const uint2 v = ld_range(B.en, p);
+      if (lo > hi) {
+        lo = v.x;
+        hi = v.y;
+      } else if (v.x <= hi + 1u && v.y + 1u >= lo) {
+        lo = min(lo, v.x);
+        hi = max(hi, v.y);
+      } else {
+        valid = false;
+      }
+    }
+    summ[b] = valid ? make_uint2(lo, hi) : make_uint2(1u, 0u);
+  }
+}
+
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     Mat<S> A, Mat<S> B, uint64_t nrows, uint32_t KS, uint32_t lane_max, uint32_t mid_max,
     uint32_t *__restrict__ cnt, uint32_t *__restrict__ stage,
     uint32_t *__restrict__ ovf_list, uint32_t *__restrict__ mid_list,
     uint32_t *__restrict__ big_list, OpCounters *__restrict__ ctr,
-    const unsigned int *__restrict__ status) {
+    const unsigned int *__restrict__ status, const uint2 *__restrict__ summ,
+    uint32_t summ_min) {
   if (*status) return;  // an earlier step overflowed: operands are not valid
   unsigned long long nnz_acc = 0, slice_acc = 0;
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows;
@@ -240,7 +308,6 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
       uint64_t len = 0;
       for (uint64_t ai = a0; ai < a1; ai += 2)
         len += ((uint64_t)B.rm[(uint64_t)A.en[ai + 1] + 1] - B.rm[A.en[ai]]) >> 1;
-      slice_acc += len;
       bool ok = len <= lane_max;
       Runs r;
 #pragma unroll
@@ -253,29 +320,26 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
       // slots and starts a new running interval.
       uint32_t cl = 0xFFFFFFFFu, ch = 0u;
       for (uint64_t ai = a0; ok && ai < a1; ai += 2) {
-        const uint64_t p0 = B.rm[A.en[ai]], p1 = B.rm[(uint64_t)A.en[ai + 1] + 1];
-        // LANE_BATCH independent loads in flight, then fold them in order
-        for (uint64_t p = p0; ok && p < p1; p += 2 * LANE_BATCH) {
-          uint2 v[LANE_BATCH];
-#pragma unroll
-          for (int k = 0; k < LANE_BATCH; ++k)
-            v[k] = p + 2 * k < p1 ? ld_range(B.en, p + 2 * k) : make_uint2(1u, 0u);
-#pragma unroll
-          for (int k = 0; k < LANE_BATCH; ++k) {
-            const uint32_t vs = v[k].x, ve = v[k].y;
-            if (vs > ve) continue;
-            if (vs <= ch + 1u && ve + 1u >= cl) {
-              cl = min(cl, vs);
-              ch = max(ch, ve);
-            } else {
-              if (cl <= ch && ok && !insert(r, cl, ch)) {
-                compact(r);
-                ok = insert(r, cl, ch);
-              }
-              cl = vs;
-              ch = ve;
+        const uint64_t s = A.en[ai], e = A.en[ai + 1];
+        if (summ && e - s + 1 >= summ_min) {
+          // head rows, whole 32-row blocks by their summaries, tail rows
+          // (summ_min >= 64 leaves at least one whole block in between)
+          const uint64_t h1 = (s + SUMM_ROWS - 1) / SUMM_ROWS * SUMM_ROWS;
+          const uint64_t t0 = (e + 1) / SUMM_ROWS * SUMM_ROWS;
+          fold_segment(B.en, B.rm[s], B.rm[h1], cl, ch, r, ok, slice_acc);
+          for (uint64_t bb = h1 / SUMM_ROWS; ok && bb < t0 / SUMM_ROWS; ++bb) {
+            const uint2 sm = summ[bb];
+            if (sm.x <= sm.y) {
+              fold(sm.x, sm.y, cl, ch, r, ok);
+              slice_acc += 1;
+            } else if (sm.x != SUMM_EMPTY) {
+              fold_segment(B.en, B.rm[bb * SUMM_ROWS], B.rm[(bb + 1) * SUMM_ROWS], cl, ch, r, ok,
+                           slice_acc);
             }
           }
+          fold_segment(B.en, B.rm[t0], B.rm[e + 1], cl, ch, r, ok, slice_acc);
+        } else {
+          fold_segment(B.en, B.rm[s], B.rm[e + 1], cl, ch, r, ok, slice_acc);
         }
       }
       if (ok && cl <= ch && !insert(r, cl, ch)) {
@@ -525,71 +589,93 @@ __device__ unsigned long long block_scan_incl(unsigned long long v,
   return v + off;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scan_tiles(const uint32_t *__restrict__ cnt,
-                                                      uint64_t n,
-                                                      unsigned long long *__restrict__ tile_sum) {
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
-  unsigned long long s = 0;
-#pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k)
-    if (base + k < n) s += cnt[base + k];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  __shared__ unsigned long long red[BLOCK / 64];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+// ---------------------------------------------------------------------------
+// Single-pass row-pointer scan with decoupled look-back.  Tiles of SCAN_TILE
+// counts take tickets in launch order (so every predecessor of a tile is
+// already running and publishes its aggregate without waiting), publish their
+// aggregate, walk back over predecessors until an inclusive prefix is found,
+// and publish their own inclusive prefix.  Tile words: bits 62-63 flag
+// (0 none, 1 aggregate, 2 inclusive prefix), bits 0-61 value.  A bounded
+// spin: a look-back that waits too long sets bit 2 of *status and goes on.
+// EMIT: rows whose count fits the stage are copied into C here; a tile whose
+// entries would pass `cap` writes none of them (the total then flags bit 0).
+constexpr unsigned long long LB_AGG = 1ull << 62, LB_INC = 2ull << 62,
+                             LB_VAL = (1ull << 62) - 1;
+
+template <typename S, bool EMIT>
+__global__ __launch_bounds__(BLOCK) void k_scan_lookback(
+    const uint32_t *__restrict__ cnt, uint64_t n, S *__restrict__ rm,
+    unsigned long long *__restrict__ tile_state, unsigned int *__restrict__ ticket,
+    const uint32_t *__restrict__ stage, uint32_t KS, uint32_t *__restrict__ c_en,
+    OpCounters *__restrict__ ctr, unsigned long long cap, uint32_t sb,
+    unsigned int *__restrict__ status) {
+  __shared__ unsigned int s_tile;
+  __shared__ unsigned long long s_excl;
+  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
-    tile_sum[blockIdx.x] = t;
-  }
-}
-
-// Exclusive scan of the tile sums; the grand total goes to ctr->total and,
-// when it exceeds the capacity of the output entries (or a 32-bit rowmap),
-// sets bit 0 (capacity) or bit 1 (rowmap width) of *status.
-__global__ __launch_bounds__(BLOCK) void k_scan_spine(unsigned long long *__restrict__ tile_sum,
-                                                      uint64_t ntiles,
-                                                      OpCounters *__restrict__ ctr,
-                                                      unsigned long long cap, uint32_t sb,
-                                                      unsigned int *__restrict__ status) {
-  unsigned long long carry = 0;
-  for (uint64_t base = 0; base < ntiles; base += BLOCK) {
-    const uint64_t i = base + threadIdx.x;
-    const unsigned long long v = i < ntiles ? tile_sum[i] : 0ull;
-    unsigned long long tot;
-    const unsigned long long incl = block_scan_incl(v, tot);
-    if (i < ntiles) tile_sum[i] = carry + incl - v;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) {
-    ctr->total = carry;
-    if (carry > cap) atomicOr(status, 1u);
-    if (sb == 4 && carry > 0xFFFFFFFFull) atomicOr(status, 2u);
-  }
-}
-
-template <typename S>
-__global__ __launch_bounds__(BLOCK) void k_scan_write(const uint32_t *__restrict__ cnt,
-                                                      uint64_t n,
-                                                      const unsigned long long *__restrict__ tile_off,
-                                                      S *__restrict__ rm) {
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  const uint64_t tile = s_tile;
+  const uint64_t base = tile * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
   uint32_t v[SCAN_ITEMS];
-  unsigned long long s = 0;
+  unsigned long long tsum = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     v[k] = base + k < n ? cnt[base + k] : 0u;
-    s += v[k];
+    tsum += v[k];
   }
-  unsigned long long tot;
-  unsigned long long run = block_scan_incl(s, tot) - s + tile_off[blockIdx.x];
+  unsigned long long agg;
+  const unsigned long long incl = block_scan_incl(tsum, agg);
+  if (threadIdx.x == 0) {
+    unsigned long long excl = 0;
+    if (tile == 0) {
+      __hip_atomic_store(&tile_state[0], LB_INC | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&tile_state[tile], LB_AGG | agg, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      int64_t j = (int64_t)tile - 1;
+      uint32_t spins = 0;
+      while (j >= 0) {
+        const unsigned long long w =
+            __hip_atomic_load(&tile_state[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long f = w & ~LB_VAL;
+        if (f == 0) {
+          if (++spins > (1u << 24)) {  // bounded wait
+            atomicOr(status, 4u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+          continue;
+        }
+        excl += w & LB_VAL;
+        if (f == LB_INC) break;
+        --j;
+      }
+      __hip_atomic_store(&tile_state[tile], LB_INC | (excl + agg), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_excl = excl;
+    const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tile + 1 == ntiles) {
+      const unsigned long long total = excl + agg;
+      ctr->total = total;
+      if (total > cap) atomicOr(status, 1u);
+      if (sb == 4 && total > 0xFFFFFFFFull) atomicOr(status, 2u);
+    }
+  }
+  __syncthreads();
+  const unsigned long long texcl = s_excl;
+  unsigned long long run = texcl + incl - tsum;
+  if (tile == 0 && threadIdx.x == 0) rm[0] = 0;
+  const bool fits = texcl + agg <= cap && (sb == 8 || texcl + agg <= 0xFFFFFFFFull);
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    run += v[k];
-    if (base + k < n) rm[base + k + 1] = (S)run;
+    const uint64_t row = base + k;
+    if (row < n) {
+      if (EMIT && fits && v[k] <= KS)
+        for (uint32_t e = 0; e < v[k]; ++e) c_en[run + e] = stage[(uint64_t)e * n + row];
+      run += v[k];
+      rm[row + 1] = (S)run;
+    }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) rm[0] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -931,24 +1017,33 @@ void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *en
                        static_cast<uint64_t *>(rowmap), entries);
 }
 
-uint64_t scan_tmp_words(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+uint64_t lookback_words(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
-void scan_counts(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
-                 unsigned long long *tmp, OpCounters *ctr, uint64_t cap, unsigned int *status) {
+void scan_lookback(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
+                   unsigned long long *tile_state, OpCounters *ctr, uint64_t cap,
+                   unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
+                   uint32_t *c_entries) {
+  const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  // tile words and the ticket (the last word) start at zero
+  fill_u32(s, reinterpret_cast<uint32_t *>(tile_state), 2 * (ntiles + 1), 0u);
   if (n == 0) {
     fill_u32(s, static_cast<uint32_t *>(rowmap), sb / 4, 0u);
     return;
   }
-  const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-  hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, tmp);
-  hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(BLOCK), 0, s, tmp, ntiles, ctr,
-                     (unsigned long long)cap, sb, status);
-  if (sb == 4)
-    hipLaunchKernelGGL(k_scan_write<uint32_t>, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt,
-                       n, tmp, static_cast<uint32_t *>(rowmap));
-  else
-    hipLaunchKernelGGL(k_scan_write<uint64_t>, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt,
-                       n, tmp, static_cast<uint64_t *>(rowmap));
+  auto *ticket = reinterpret_cast<unsigned int *>(tile_state + ntiles);
+  const bool emit = c_entries != nullptr;
+#define DV_LB(S_, E_)                                                                        \
+  hipLaunchKernelGGL((k_scan_lookback<S_, E_>), dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, \
+                     n, static_cast<S_ *>(rowmap), tile_state, ticket, stage, stage_entries,     \
+                     c_entries, ctr, (unsigned long long)cap, sb, status)
+  if (sb == 4) {
+    if (emit) DV_LB(uint32_t, true);
+    else DV_LB(uint32_t, false);
+  } else {
+    if (emit) DV_LB(uint64_t, true);
+    else DV_LB(uint64_t, false);
+  }
+#undef DV_LB
 }
 
 void spadd_count(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt,
@@ -985,16 +1080,30 @@ void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t max_entries, OpC
 void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, uint32_t lane_max,
                  uint32_t mid_max, uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
-                 const unsigned int *status) {
+                 const unsigned int *status, const void *summ, uint32_t summ_min) {
   if (!a.nrows) return;
+  const uint2 *sm = static_cast<const uint2 *>(summ);
   if (sb == 4)
     hipLaunchKernelGGL(k_spgemm_lane<uint32_t>, dim3(grid), dim3(BLOCK), 0, s, mat<uint32_t>(a),
                        mat<uint32_t>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list,
-                       mid_list, big_list, ctr, status);
+                       mid_list, big_list, ctr, status, sm, summ_min);
   else
     hipLaunchKernelGGL(k_spgemm_lane<uint64_t>, dim3(grid), dim3(BLOCK), 0, s, mat<uint64_t>(a),
                        mat<uint64_t>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list,
-                       mid_list, big_list, ctr, status);
+                       mid_list, big_list, ctr, status, sm, summ_min);
+}
+
+uint64_t summary_words(uint64_t nrows) { return (nrows + SUMM_ROWS - 1) / SUMM_ROWS; }
+
+void block_summary(hipStream_t s, uint32_t sb, DView b, void *summ, const unsigned int *status) {
+  const uint64_t nb = summary_words(b.nrows);
+  if (!nb) return;
+  if (sb == 4)
+    hipLaunchKernelGGL(k_block_summary<uint32_t>, dim3(grid_for(nb)), dim3(BLOCK), 0, s,
+                       mat<uint32_t>(b), b.nrows, static_cast<uint2 *>(summ), status);
+  else
+    hipLaunchKernelGGL(k_block_summary<uint64_t>, dim3(grid_for(nb)), dim3(BLOCK), 0, s,
+                       mat<uint64_t>(b), b.nrows, static_cast<uint2 *>(summ), status);
 }
 
 void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunch &g) {

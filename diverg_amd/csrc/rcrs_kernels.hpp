@@ -53,13 +53,17 @@ void zero_counters(hipStream_t s, OpCounters *c);
 
 void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *entries);
 
-// rowmap[0] = 0, rowmap[i+1] = cnt[0] + ... + cnt[i]; total -> ctr->total.
-// A total above `cap` sets bit 0 of *status, above a 32-bit rowmap bit 1.
-// `tmp` needs scan_tmp_words(n) uint64 words.
-uint64_t scan_tmp_words(uint64_t n);
-void scan_counts(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n,
-                 void *rowmap, unsigned long long *tmp, OpCounters *ctr, uint64_t cap,
-                 unsigned int *status);
+// Row-pointer scan: rowmap[0] = 0, rowmap[i+1] = cnt[0] + ... + cnt[i];
+// total -> ctr->total; a total above `cap` sets bit 0 of *status, above a
+// 32-bit rowmap bit 1.  One pass (decoupled look-back over tiles) and, when
+// c_entries is given, the copy of staged rows into C (column-major stage of
+// stage_entries per row).  tile_state needs lookback_words(n) words; it is
+// cleared by the call.  A look-back that does not complete sets bit 2.
+uint64_t lookback_words(uint64_t n);
+void scan_lookback(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
+                   unsigned long long *tile_state, OpCounters *ctr, uint64_t cap,
+                   unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
+                   uint32_t *c_entries);
 
 // Kernels that read operands take `status` and do nothing once it is set.
 void spadd_count(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt,
@@ -80,7 +84,13 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
                  uint32_t mid_max,
                  uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
-                 const unsigned int *status);
+                 const unsigned int *status, const void *summ, uint32_t summ_min);
+// Per-32-row block summaries of B (8 bytes each, summary_words(nrows) of
+// them): the single interval the block's ranges union to, when they do.  The
+// lane tier then folds an A range of at least summ_min (>= 64) rows as head
+// rows, whole blocks by their summaries, and tail rows.
+uint64_t summary_words(uint64_t nrows);
+void block_summary(hipStream_t s, uint32_t sb, DView b, void *summ, const unsigned int *status);
 // Group tiers (counting or direct write), see GroupLaunch.
 void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunch &g);
 // Copy staged rows (count <= stage) to their place in C.

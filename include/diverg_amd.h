@@ -118,7 +118,10 @@ void *dv_stream(dv_context *ctx);
  *                   always takes the fallback (for tests)
  *   graphs          1: the launches of a planned build are captured once into
  *                   a hipGraph and replayed while the build's operations,
- *                   buffers and tuning stay the same (not while profiling) */
+ *                   buffers and tuning stay the same (not while profiling)
+ *   summary_span    planned SpGEMMs whose left operand's ranges span at least
+ *                   this many rows (>= 64; 0: never) read whole 32-row blocks
+ *                   of the right operand through per-block union summaries */
 typedef struct dv_tuning {
   uint32_t lds_words;
   uint32_t stage_entries;
@@ -129,6 +132,7 @@ typedef struct dv_tuning {
   uint32_t autotune;
   uint32_t plan_mode;
   uint32_t graphs;
+  uint32_t summary_span;
 } dv_tuning;
 int dv_get_tuning(const dv_context *ctx, dv_tuning *t);
 int dv_set_tuning(dv_context *ctx, const dv_tuning *t);

@@ -130,6 +130,19 @@ def test_power_noncanonical_input(ctx):
         assert_identical(ctx.range_power(a, k), orc.power(a, k), f"A^{k} non-canonical")
 
 
+@pytest.mark.parametrize("summary_span", [0, 64, 200])
+def test_distance_index_block_summaries(ctx, summary_span):
+    """Block summaries on and off, on a graph whose bubbles make some 32-row
+    blocks fail the single-interval test (they fall back to their ranges)."""
+    ctx.set_tuning(summary_span=summary_span)
+    a = dv.generate_graph(3, backbone_bp=60000, seed=8, snp_rate=0.05, indel_rate=0.01,
+                          sv_rate=2e-3, sv_min=10, sv_max=300)
+    print(f"dindex summaries span={summary_span}", flush=True)
+    for dlo, dup in [(0, 300), (40, 700)]:
+        assert_identical(ctx.create_distance_index(a, dlo, dup), orc.dindex(a, dlo, dup),
+                         f"D[{dlo},{dup}] summary_span={summary_span}")
+
+
 @pytest.mark.parametrize("size_bytes", [4, 8])
 @pytest.mark.parametrize("dlo,dup", [(150, 450), (0, 250), (7, 7), (1, 8), (5, 5), (0, 1)])
 @pytest.mark.parametrize("plan_mode", [0, 1, 2])
