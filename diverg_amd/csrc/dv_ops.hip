@@ -83,6 +83,8 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
                            unsigned int *status, hipEvent_t after_lane, uint32_t *c_entries,
                            bool summaries) {
   const uint64_t n = a.nrows;
+  // the summary buffer of the scratch holds summary_words(n + 1) words
+  summaries = summaries && b.nrows <= n + 1 && tuning.summary_span >= 64;
   if (n) {
     if (summaries) block_summary(stream, b.sb, view(b), s.summ, status);
     // persistent lane-tier grid: few blocks keep the per-block counter atomics
