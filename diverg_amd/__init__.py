@@ -209,6 +209,12 @@ class Context:
         self._check(N.lib().dv_get_tuning(self._h, C.byref(t)), "dv_get_tuning")
         return {f: getattr(t, f) for f, _ in N.dv_tuning._fields_}
 
+    def last_tuning(self):
+        """The launch shape the last index build ran with (autotuned or not)."""
+        t = N.dv_tuning()
+        self._check(N.lib().dv_last_tuning(self._h, C.byref(t)), "dv_last_tuning")
+        return {f: getattr(t, f) for f, _ in N.dv_tuning._fields_}
+
     def set_tuning(self, **kw):
         """Update the named fields of the launch tuning (dv_tuning)."""
         t = N.dv_tuning()

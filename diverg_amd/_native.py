@@ -128,6 +128,7 @@ SIGNATURES = [
     ("dv_last_error", C.c_char_p, [VP]),
     ("dv_stream", VP, [VP]),
     ("dv_get_tuning", C.c_int, [VP, P(dv_tuning)]),
+    ("dv_last_tuning", C.c_int, [VP, P(dv_tuning)]),
     ("dv_set_tuning", C.c_int, [VP, P(dv_tuning)]),
     ("dv_profile", C.c_int, [VP, C.c_int]),
     ("dv_profile_read", C.c_int, [VP, P(dv_kernel_time)]),

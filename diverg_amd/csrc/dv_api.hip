@@ -412,6 +412,12 @@ int dv_get_tuning(const dv_context *c, dv_tuning *t) {
   return DV_OK;
 }
 
+int dv_last_tuning(const dv_context *c, dv_tuning *t) {
+  if (!c || !t) return DV_ERR_INVALID;
+  *t = c->last_tuning;
+  return DV_OK;
+}
+
 int dv_set_tuning(dv_context *c, const dv_tuning *t) {
   if (!c || !t) return DV_ERR_INVALID;
   const dv_tuning n = *t;
@@ -426,6 +432,7 @@ int dv_set_tuning(dv_context *c, const dv_tuning *t) {
   if (n.lane_slice_max > (1u << 20)) return C(c)->fail("lane_slice_max must be 0 .. 2^20");
   if (n.plan_mode > 2) return C(c)->fail("plan_mode must be 0, 1 or 2");
   if (n.graphs > 1) return C(c)->fail("graphs must be 0 or 1");
+  if (n.autotune > 1) return C(c)->fail("autotune must be 0 or 1");
   if (n.summary_span != 0 && n.summary_span < 64) return C(c)->fail("summary_span must be 0 or >= 64");
   c->tuning = n;
   return DV_OK;

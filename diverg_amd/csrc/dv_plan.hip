@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 
 #include "dv_runtime.hpp"
@@ -444,16 +445,111 @@ bool Ctx::plan_events_reserve(int nops) {
   return true;
 }
 
+int Ctx::dindex_planned(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
+  Plan p;
+  p.out = p.dindex(p.external(a), dlo, dup);
+  return plan_run(p, d, [&](DevMat &o) { return dindex_exact(a, dlo, dup, o); });
+}
+
+// Launch-shape autotuner, in place of the reference's SparseConfig / ExecGrid
+// heuristics (range_sparse_base.hpp:99-533): coordinate descent over the
+// launch parameters, each candidate timed as a whole planned build of this
+// exact shape (best of two runs), the winner cached per build shape.  Every
+// candidate computes the same index; only its speed differs.
+int Ctx::autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning &best) {
+  const dv_tuning user = tuning;
+  best = user;
+  const std::array<uint64_t, 6> key{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup};
+  auto copy_tuned = [](dv_tuning &to, const dv_tuning &from) {
+    to.lds_words = from.lds_words;
+    to.stage_entries = from.stage_entries;
+    to.rows_per_group = from.rows_per_group;
+    to.lanes_small = from.lanes_small;
+    to.small_slice_max = from.small_slice_max;
+    to.lane_slice_max = from.lane_slice_max;
+    to.summary_span = from.summary_span;
+  };
+  auto hit = tuned.find(key);
+  if (hit != tuned.end()) {
+    copy_tuned(best, hit->second);
+    return DV_OK;
+  }
+  auto timed = [&](const dv_tuning &t, double &ms) -> int {
+    tuning = t;
+    tuning.graphs = 0;
+    tuning.autotune = 0;
+    ms = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {
+      DevMat o;
+      if (!hip(hipStreamSynchronize(stream), "autotune synchronize")) return DV_ERR_DEVICE;
+      const auto t0 = std::chrono::steady_clock::now();
+      const int rc = dindex_planned(a, dlo, dup, o);
+      const bool planned = stats_planned != 0;
+      const double dt =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      release(o);
+      if (rc) return rc;
+      autotune_builds += 1;
+      if (planned) ms = std::min(ms, dt);
+    }
+    return DV_OK;
+  };
+  const uint32_t lds_max = (uint32_t)std::min<uint64_t>(1024, lds_per_block / 32);
+  const std::vector<std::pair<uint32_t dv_tuning::*, std::vector<uint32_t>>> dims = {
+      {&dv_tuning::stage_entries, {4, 8, 16}},
+      {&dv_tuning::lanes_small, {8, 16, 32}},
+      {&dv_tuning::small_slice_max, {32, 64, 128, 256}},
+      {&dv_tuning::lane_slice_max, {1024, 4096, 16384}},
+      {&dv_tuning::summary_span, {0, 64, 256}},
+      {&dv_tuning::lds_words, {lds_max / 4, lds_max / 2, lds_max}},
+      {&dv_tuning::rows_per_group, {1, 4, 16}},
+  };
+  double best_ms = 0;
+  int rc = timed(best, best_ms);
+  for (size_t d = 0; !rc && d < dims.size(); ++d) {
+    const uint32_t keep = best.*(dims[d].first);
+    for (uint32_t v : dims[d].second) {
+      if (v == keep || v == 0 && dims[d].first != &dv_tuning::summary_span) continue;
+      dv_tuning cand = best;
+      cand.*(dims[d].first) = v;
+      double ms = 0;
+      rc = timed(cand, ms);
+      if (rc) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = cand;
+      }
+    }
+  }
+  tuning = user;
+  if (rc) {
+    best = user;
+    return rc;
+  }
+  best.graphs = user.graphs;
+  best.autotune = user.autotune;
+  tuned[key] = best;
+  return DV_OK;
+}
+
 int Ctx::dindex_dev(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   if (!(dlo <= dup && dup != 0)) return fail("create_distance_index: need dlo <= dup and dup != 0");
   if (a.nrows != a.ncols) return fail("create_distance_index: matrix is not square");
+  last_tuning = tuning;
   if (tuning.plan_mode == 1) {
     reset_stats();
     return dindex_exact(a, dlo, dup, d);
   }
-  Plan p;
-  p.out = p.dindex(p.external(a), dlo, dup);
-  return plan_run(p, d, [&](DevMat &o) { return dindex_exact(a, dlo, dup, o); });
+  if (!tuning.autotune || tuning.plan_mode != 0) return dindex_planned(a, dlo, dup, d);
+  dv_tuning best;
+  int rc = autotune_dindex(a, dlo, dup, best);
+  if (rc) return rc;
+  const dv_tuning user = tuning;
+  tuning = best;
+  last_tuning = best;
+  rc = dindex_planned(a, dlo, dup, d);
+  tuning = user;
+  return rc;
 }
 
 int Ctx::power_dev(const DevMat &a, uint32_t k, DevMat &out) {

@@ -3,8 +3,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <functional>
+#include <map>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -137,6 +139,11 @@ struct Ctx {
   hipGraphExec_t graph_exec = nullptr;
   uint64_t graph_key = 0;
   uint64_t graph_captures = 0;
+  // autotuned launch shapes per build shape (rows, entries, nnz, size type,
+  // d1, d2) and the tuning the last index build ran with
+  std::map<std::array<uint64_t, 6>, dv_tuning> tuned;
+  dv_tuning last_tuning{};
+  uint64_t autotune_builds = 0;
 
   // per-kernel timing (dv_profile)
   bool prof = false;
@@ -214,6 +221,8 @@ struct Ctx {
   int plan_run(Plan &p, DevMat &out, const std::function<int(DevMat &)> &exact);
   int power_dev(const DevMat &a, uint32_t k, DevMat &out);
   int dindex_dev(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
+  int dindex_planned(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
+  int autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning &best);
 
   // dv_batch.hip
   int dindex_batched(const std::vector<DevMat> &comps, uint32_t dlo, uint32_t dup,

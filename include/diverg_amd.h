@@ -108,8 +108,13 @@ void *dv_stream(dv_context *ctx);
  *   small_slice_max medium rows: at most this many B-slice ranges
  *   lane_slice_max  lane-per-row tier: at most this many slice ranges
  *                   (0 sends every non-empty row to the groups)
- *   autotune        1: dv_create_distance_index* picks lanes_small and
- *                   small_slice_max per build from timed probes
+ *   autotune        1: the first planned dv_create_distance_index* build of
+ *                   each shape (rows, entries, nnz, size type, d1, d2) times
+ *                   candidate launch shapes -- coordinate descent over
+ *                   stage_entries, lanes_small, small_slice_max,
+ *                   lane_slice_max, summary_span, lds_words, rows_per_group,
+ *                   each a whole build, best of two -- and later builds of
+ *                   that shape reuse the winner (dv_last_tuning shows it)
  *   plan_mode       0: index builds and powers are enqueued as one sequence
  *                   with predicted output capacities and a single readback
  *                   (exact fallback on overflow); 1: one readback per
@@ -136,6 +141,9 @@ typedef struct dv_tuning {
 } dv_tuning;
 int dv_get_tuning(const dv_context *ctx, dv_tuning *t);
 int dv_set_tuning(dv_context *ctx, const dv_tuning *t);
+/* The launch shape the last dv_create_distance_index* call ran with (the
+ * autotuned one when autotune is on).  No counterpart in the reference. */
+int dv_last_tuning(const dv_context *ctx, dv_tuning *t);
 
 /* Per-kernel timing with HIP events on the context stream (no counterpart in
  * the reference, whose DIVERG_STATS timers print symbolic/numeric times after

@@ -130,6 +130,19 @@ def test_power_noncanonical_input(ctx):
         assert_identical(ctx.range_power(a, k), orc.power(a, k), f"A^{k} non-canonical")
 
 
+def test_distance_index_autotuned(ctx, cfg1):
+    """The autotuner tries many launch shapes; every one must give the same
+    index, and the cached winner is reused for the same shape."""
+    want = orc.dindex(cfg1, 150, 450)
+    ctx.set_tuning(autotune=1)
+    print("dindex autotune cfg1 [150,450]", flush=True)
+    for _ in range(2):
+        assert_identical(ctx.create_distance_index(cfg1, 150, 450), want, "autotuned D")
+    t = ctx.last_tuning()
+    assert t["autotune"] == 1
+    assert t["lanes_small"] in (8, 16, 32) and t["stage_entries"] in (4, 8, 16)
+
+
 @pytest.mark.parametrize("summary_span", [0, 64, 200])
 def test_distance_index_block_summaries(ctx, summary_span):
     """Block summaries on and off, on a graph whose bubbles make some 32-row
