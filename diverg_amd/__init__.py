@@ -16,14 +16,15 @@ in libdiverg_amd.so; there is no Python or CPU fallback -- a missing library
 raises `NativeMissing`.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
 from . import _native as N
 from ._native import NativeMissing  # noqa: F401
 
-__all__ = ["RangeCRS", "DeviceRCRS", "Context", "Server", "DiVerGError", "generate_graph",
-           "graph_preset", "identical", "NativeMissing"]
+__all__ = ["RangeCRS", "DeviceRCRS", "Context", "Server", "Graph", "DiVerGError",
+           "generate_graph", "graph_preset", "identical", "NativeMissing"]
 
 
 class DiVerGError(RuntimeError):
@@ -310,6 +311,40 @@ class Context:
             return res, BuildStats({f: getattr(st, f) for f, _ in N.dv_build_stats._fields_})
         return res
 
+    def merge_distance_index(self, d1, d2, location="host"):
+        """util::merge_distance_index (dindex.hpp:435-456, 556-579): the union
+        of two indices of one shape, through the device SpAdd."""
+        keep = []
+        out = N.dv_rcrs()
+        self._check(N.lib().dv_merge_distance_index(self._h, C.byref(d1._desc(keep)),
+                                                    C.byref(d2._desc(keep)),
+                                                    self._loc(location), C.byref(out)),
+                    "dv_merge_distance_index")
+        return self._out(out, location)
+
+    def save(self, m, path):
+        """Write m in the reference's serialize() layout (crs_matrix.hpp:1392-1409,
+        1986-2000)."""
+        keep = []
+        self._check(N.lib().dv_save(self._h, C.byref(m._desc(keep)), os.fsencode(path)),
+                    "dv_save")
+
+    def load(self, path, size_bytes=0, location="host"):
+        """Read a matrix written by save() or by the reference's serialize()."""
+        out = N.dv_rcrs()
+        self._check(N.lib().dv_load(self._h, os.fsencode(path), size_bytes,
+                                    self._loc(location), C.byref(out)), "dv_load")
+        return self._out(out, location)
+
+    def adjacency_matrix(self, graph, lower=0, upper=0, size_bytes=8, location="host"):
+        """Character-level adjacency of nodes [lower, upper) of `graph` (upper 0:
+        all), built on the device (adjacency.hpp:170-228, 249-262)."""
+        out = N.dv_rcrs()
+        self._check(N.lib().dv_adjacency_matrix(self._h, graph._h, lower, upper, size_bytes,
+                                                self._loc(location), C.byref(out)),
+                    "dv_adjacency_matrix")
+        return self._out(out, location)
+
     def query(self, m, qi, qj):
         """Batched D(i, j); raises DiVerGError(DV_ERR_RANGE) for pairs outside."""
         keep = []
@@ -448,6 +483,91 @@ class Server:
         self._check(rc, "dv_server_replay")
         return {"answers": ans, "latency_us": lat, "build_done_us": bdone[:nb],
                 "build_index": bidx[:nb], "elapsed_us": el.value}
+
+
+class Graph:
+    """A sequence graph (dv_graph): nodes with lengths in bases and directed
+    edges.  Load GFA with Graph.load_gfa(path) or build from arrays."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def load_gfa(cls, path, version=0):
+        h = C.c_void_p()
+        rc = N.lib().dv_graph_load_gfa(os.fsencode(path), version, C.byref(h))
+        if rc:
+            raise DiVerGError(rc, f"dv_graph_load_gfa({path!r}) failed")
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, node_len, edges_from=(), edges_to=()):
+        nl = np.ascontiguousarray(node_len, dtype=np.uint32)
+        ef = np.ascontiguousarray(edges_from, dtype=np.uint32)
+        et = np.ascontiguousarray(edges_to, dtype=np.uint32)
+        if len(ef) != len(et):
+            raise ValueError("edges_from and edges_to differ in length")
+        h = C.c_void_p()
+        u32 = C.POINTER(C.c_uint32)
+        rc = N.lib().dv_graph_from_arrays(len(nl), nl.ctypes.data_as(u32), len(ef),
+                                          ef.ctypes.data_as(u32), et.ctypes.data_as(u32),
+                                          C.byref(h))
+        if rc:
+            raise DiVerGError(rc, "dv_graph_from_arrays failed")
+        return cls(h)
+
+    def close(self):
+        if self._h:
+            N.lib().dv_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = N.dv_graph_info()
+        rc = N.lib().dv_graph_info_get(self._h, C.byref(i))
+        if rc:
+            raise DiVerGError(rc, "dv_graph_info_get failed")
+        return {f: getattr(i, f) for f, _ in N.dv_graph_info._fields_}
+
+    def arrays(self):
+        """(node_len, charorder, edge_off, edge_to) as numpy arrays."""
+        inf = self.info()
+        n, e = inf["nodes"], inf["edges"]
+        nl = np.zeros(n, np.uint32)
+        co = np.zeros(n + 1, np.uint64)
+        eo = np.zeros(n + 1, np.uint64)
+        et = np.zeros(max(e, 1), np.uint32)
+        u32, u64 = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)
+        rc = N.lib().dv_graph_arrays(self._h, nl.ctypes.data_as(u32), co.ctypes.data_as(u64),
+                                     eo.ctypes.data_as(u64), et.ctypes.data_as(u32))
+        if rc:
+            raise DiVerGError(rc, "dv_graph_arrays failed")
+        return nl, co, eo, et[:e]
+
+    def node_name(self, k):
+        s = N.lib().dv_graph_node_name(self._h, k)
+        return None if s is None else s.decode()
+
+    def sort(self, order="min-breaks", seed=0):
+        rc = N.lib().dv_graph_sort(self._h, N.DV_ORDERS[order], seed)
+        if rc:
+            raise DiVerGError(rc, "dv_graph_sort failed")
+
+    def components(self):
+        """Block boundaries [0, b1, ..., n] with no edge between blocks."""
+        cnt = C.c_uint64()
+        N.lib().dv_graph_components(self._h, None, 0, C.byref(cnt))
+        arr = np.zeros(cnt.value + 1, np.uint64)
+        rc = N.lib().dv_graph_components(self._h, arr.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         len(arr), C.byref(cnt))
+        if rc:
+            raise DiVerGError(rc, "dv_graph_components failed")
+        return arr
 
 
 def graph_preset(preset, component=0):

@@ -1,4 +1,5 @@
-"""ctypes declarations of the C ABI (include/diverg_amd.h, diverg_amd_gen.h)."""
+"""ctypes declarations of the C ABI (include/diverg_amd.h, diverg_amd_gen.h,
+diverg_amd_graph.h)."""
 import ctypes as C
 import os
 
@@ -10,8 +11,10 @@ DV_HOST, DV_DEVICE = 0, 1
 STATUS = {
     0: "ok", 1: "invalid argument", 2: "overflow", 3: "out of memory",
     4: "device error", 5: "HBM budget exceeded", 6: "index out of range",
-    7: "busy", 8: "not found",
+    7: "busy", 8: "not found", 9: "I/O error",
 }
+
+DV_ORDERS = {"native": 0, "topological": 1, "min-breaks": 2, "cuthill-mckee": 3, "random": 4}
 
 
 class dv_rcrs(C.Structure):
@@ -41,6 +44,17 @@ class dv_build_stats(C.Structure):
         ("deferred_rows", C.c_uint64),
         ("planned", C.c_uint32),
         ("retried", C.c_uint32),
+    ]
+
+
+class dv_graph_info(C.Structure):
+    _fields_ = [
+        ("nodes", C.c_uint64),
+        ("edges", C.c_uint64),
+        ("loci", C.c_uint64),
+        ("components", C.c_uint64),
+        ("breaks", C.c_uint64),
+        ("bandwidth", C.c_uint64),
     ]
 
 
@@ -163,6 +177,20 @@ SIGNATURES = [
     ("dv_generate_graph", C.c_int,
      [P(dv_graph_params), C.c_uint32, P(dv_rcrs), P(P(C.c_uint32))]),
     ("dv_free_host", None, [VP]),
+    ("dv_merge_distance_index", C.c_int, [VP, P(dv_rcrs), P(dv_rcrs), C.c_uint32, P(dv_rcrs)]),
+    ("dv_save", C.c_int, [VP, P(dv_rcrs), C.c_char_p]),
+    ("dv_load", C.c_int, [VP, C.c_char_p, C.c_uint32, C.c_uint32, P(dv_rcrs)]),
+    ("dv_graph_load_gfa", C.c_int, [C.c_char_p, C.c_uint32, P(VP)]),
+    ("dv_graph_from_arrays", C.c_int,
+     [C.c_uint64, U32P, C.c_uint64, U32P, U32P, P(VP)]),
+    ("dv_graph_free", None, [VP]),
+    ("dv_graph_info_get", C.c_int, [VP, P(dv_graph_info)]),
+    ("dv_graph_arrays", C.c_int, [VP, U32P, P(C.c_uint64), P(C.c_uint64), U32P]),
+    ("dv_graph_node_name", C.c_char_p, [VP, C.c_uint64]),
+    ("dv_graph_sort", C.c_int, [VP, C.c_uint32, C.c_uint64]),
+    ("dv_graph_components", C.c_int, [VP, P(C.c_uint64), C.c_uint64, P(C.c_uint64)]),
+    ("dv_adjacency_matrix", C.c_int,
+     [VP, VP, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, P(dv_rcrs)]),
 ]
 
 _lib = None

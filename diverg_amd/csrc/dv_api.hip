@@ -12,6 +12,8 @@
 #include <string>
 #include <unordered_map>
 
+#include "../../include/diverg_amd_graph.h"
+#include "dv_graph.hpp"
 #include "dv_internal.hpp"
 #include "dv_runtime.hpp"
 #include "rcrs_kernels.hpp"
@@ -34,6 +36,8 @@ int Ctx::fail(const std::string &msg, int code) {
   last_error = msg;
   return code;
 }
+
+int set_error(dv_context *ctx, const std::string &msg, int code) { return ctx->fail(msg, code); }
 
 void *Ctx::dalloc(uint64_t bytes) {
   if (bytes == 0) bytes = 8;
@@ -326,6 +330,7 @@ const char *dv_status_string(int s) {
     case DV_ERR_RANGE: return "index out of range";
     case DV_ERR_BUSY: return "busy";
     case DV_ERR_NOT_FOUND: return "not found";
+    case DV_ERR_IO: return "I/O error";
     default: return "unknown status";
   }
 }
@@ -522,6 +527,27 @@ int dv_range_spadd(dv_context *ctx, const dv_rcrs *a, const dv_rcrs *b, uint32_t
   c->release(db);
   if (rc) return rc;
   return export_mat(c, dc, loc, out, nullptr);
+}
+
+int dv_adjacency_matrix(dv_context *ctx, const dv_graph *g, uint64_t lower, uint64_t upper,
+                        uint32_t sb, uint32_t loc, dv_rcrs *out) {
+  if (!ctx || !out) return DV_ERR_INVALID;
+  Ctx *c = C(ctx);
+  if (!g) return c->fail("adjacency_matrix: graph is NULL");
+  if (!valid_sb(sb)) return c->fail("adjacency_matrix: size_bytes must be 4 or 8");
+  if (loc != DV_HOST && loc != DV_DEVICE) return c->fail("adjacency_matrix: bad location");
+  DevMat m;
+  const int rc = c->adjacency_dev(*reinterpret_cast<const dv::Graph *>(g), lower, upper, sb, m);
+  if (rc) return rc;
+  return export_mat(c, m, loc, out, nullptr);
+}
+
+int dv_merge_distance_index(dv_context *ctx, const dv_rcrs *d1, const dv_rcrs *d2, uint32_t loc,
+                            dv_rcrs *out) {
+  if (!ctx || !out) return DV_ERR_INVALID;
+  if (d1 && d2 && (d1->nrows != d2->nrows || d1->ncols != d2->ncols))
+    return C(ctx)->fail("merge_distance_index: the indices differ in shape");
+  return dv_range_spadd(ctx, d1, d2, loc, out);
 }
 
 int dv_range_spgemm(dv_context *ctx, const dv_rcrs *a, const dv_rcrs *b, uint32_t loc,

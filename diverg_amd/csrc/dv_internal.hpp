@@ -4,11 +4,16 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "../../include/diverg_amd.h"
 #include "../../include/diverg_amd_gen.h"
 
 namespace dv {
+
+// Set the context's last error message; returns `code` (defined with the
+// context in dv_api.hip).
+int set_error(dv_context *ctx, const std::string &msg, int code);
 
 // Allocation record behind dv_rcrs::owner for library-allocated matrices.
 struct Owner {

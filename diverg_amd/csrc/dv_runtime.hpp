@@ -16,6 +16,8 @@
 
 namespace dv {
 
+struct Graph;
+
 // A device-resident rCRS matrix.  `owned` matrices were allocated from the
 // context's stream-ordered pool; others alias caller memory.
 struct DevMat {
@@ -223,6 +225,9 @@ struct Ctx {
   int dindex_dev(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
   int dindex_planned(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
   int autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning &best);
+
+  // dv_graph.hip
+  int adjacency_dev(const Graph &g, uint64_t lower, uint64_t upper, uint32_t sb, DevMat &out);
 
   // dv_batch.hip
   int dindex_batched(const std::vector<DevMat> &comps, uint32_t dlo, uint32_t dup,

@@ -11,6 +11,8 @@
 #ifndef DIVERG_AMD_DINDEX_HPP__
 #define DIVERG_AMD_DINDEX_HPP__
 
+#include <cassert>
+
 #include "range_sparse.hpp"
 
 namespace diverg {
@@ -26,6 +28,24 @@ namespace diverg {
                                             nullptr ),
                   "create_distance_index" );
       return amd::adopt< TRCRSMatrix >( out );
+    }
+
+    /**
+     *  util::merge_distance_index< TMutableCRSMatrix >( d1, d2 ) for the Range
+     *  group (dindex.hpp:556-579): the union of two indices of one shape.  The
+     *  reference takes range_spadd for in-core RangeDynamic operands
+     *  (dindex.hpp:435-456); so does this (on the device).
+     */
+    template< typename TMutableCRSMatrix, typename TCRSMatrix >
+    inline TMutableCRSMatrix
+    merge_distance_index( TCRSMatrix& dindex1, TCRSMatrix& dindex2 )
+    {
+      assert( dindex1.numRows() == dindex2.numRows() );  // dindex.hpp:307-308
+      assert( dindex1.numCols() == dindex2.numCols() );
+      dv_rcrs d1 = amd::describe( dindex1 ), d2 = amd::describe( dindex2 ), out{};
+      amd::check( dv_merge_distance_index( amd::context(), &d1, &d2, DV_HOST, &out ),
+                  "merge_distance_index" );
+      return amd::adopt< TMutableCRSMatrix >( out );
     }
   }  // namespace util
 }  // namespace diverg

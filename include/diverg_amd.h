@@ -48,7 +48,8 @@ typedef enum dv_status {
   DV_ERR_BUDGET = 5,    /* serving: request exceeds the HBM budget */
   DV_ERR_RANGE = 6,     /* query (i, j) outside the matrix (crs_matrix.hpp:1222-1243) */
   DV_ERR_BUSY = 7,      /* serving: no free slot; retry */
-  DV_ERR_NOT_FOUND = 8  /* serving: unknown ticket or index id */
+  DV_ERR_NOT_FOUND = 8, /* serving: unknown ticket or index id */
+  DV_ERR_IO = 9         /* file cannot be opened, read or written, or is malformed */
 } dv_status;
 
 typedef enum dv_location { DV_HOST = 0, DV_DEVICE = 1 } dv_location;
@@ -219,6 +220,30 @@ int dv_create_distance_index_batched(dv_context *ctx, const dv_rcrs *comps, uint
 int dv_query_batch(dv_context *ctx, const dv_rcrs *m, const uint32_t *qi,
                    const uint32_t *qj, uint8_t *out, uint64_t n,
                    uint32_t query_location);
+
+/* Merge two distance indices of the same shape: the union of their cells.
+ * util::merge_distance_index<TMutable>(d1, d2) for the Range group
+ * (dindex.hpp:556-579): for in-core RangeDynamic operands the reference takes
+ * range_spadd (dindex.hpp:435-456), and so does this entry point (the same
+ * kernels as dv_range_spadd).  Its streaming variant for buffered or
+ * compressed operands (dindex.hpp:290-420) gives the same arrays on canonical
+ * indices.  nnz is recounted from the merged entries, as crs_matrix::nnz
+ * (crs_matrix.hpp:175-189). */
+int dv_merge_distance_index(dv_context *ctx, const dv_rcrs *d1, const dv_rcrs *d2,
+                            uint32_t out_location, dv_rcrs *out);
+
+/* --- persistence ------------------------------------------------------------
+ * The byte layout of CRSMatrix<RangeDynamic,bool,uint32_t,S>::serialize /
+ * load (crs_matrix.hpp:1392-1409, 1986-2000; vectors as utils.hpp:191-289):
+ * u64 count + uint32 entries, u64 count + S rowmap, u64 ncols, u64 nnz.
+ * Files are interchangeable with the reference's.  dv_save accepts host or
+ * device matrices (ctx may be NULL for a host matrix).  dv_load validates the
+ * structure (rowmap from 0, even, non-decreasing, ending at the entry count;
+ * entries below ncols) before returning it; size_bytes 0 takes the size type
+ * from the file length. */
+int dv_save(dv_context *ctx, const dv_rcrs *m, const char *path);
+int dv_load(dv_context *ctx, const char *path, uint32_t size_bytes, uint32_t location,
+            dv_rcrs *out);
 
 /* --- serving --------------------------------------------------------------
  * A serving loop over HBM-resident indexes (no counterpart in the reference,

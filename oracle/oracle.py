@@ -65,6 +65,9 @@ def lib():
                            C.POINTER(C.c_uint8), C.c_uint64]),
             ("orc_walk_rows", [P, C.POINTER(C.c_uint32), C.c_uint64,
                                C.c_uint32, C.c_uint32, P]),
+            ("orc_adjacency", [C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                               C.POINTER(C.c_uint32), C.c_uint64, C.c_uint64, P]),
+            ("orc_merge_stream", [P, P, P]),
         ]:
             f = getattr(l, name)
             f.argtypes = args
@@ -178,3 +181,28 @@ def num_threads():
 
 def set_threads(t):
     lib().orc_set_threads(int(t))
+
+
+def adjacency(node_len, edge_off, edge_to, lower=0, upper=0):
+    """Character-level adjacency of nodes [lower, upper) (adjacency.hpp:170-228
+    then crs_matrix.hpp:197-242)."""
+    nl = np.ascontiguousarray(node_len, dtype=np.uint32)
+    eo = np.ascontiguousarray(edge_off, dtype=np.uint64)
+    et = np.ascontiguousarray(edge_to, dtype=np.uint32)
+    if et.size == 0:
+        et = np.zeros(1, dtype=np.uint32)
+    s = _Rcrs()
+    _check(lib().orc_adjacency(len(nl), nl.ctypes.data_as(C.POINTER(C.c_uint32)),
+                               eo.ctypes.data_as(C.POINTER(C.c_uint64)),
+                               et.ctypes.data_as(C.POINTER(C.c_uint32)), lower, upper,
+                               C.byref(s)), "adjacency")
+    return _out(s)
+
+
+def merge_stream(a, b):
+    """Streaming distance-index merge (dindex.hpp:290-420)."""
+    keep = []
+    s = _Rcrs()
+    _check(lib().orc_merge_stream(C.byref(_in(a, keep)), C.byref(_in(b, keep)), C.byref(s)),
+           "merge_stream")
+    return _out(s)

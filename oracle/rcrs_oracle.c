@@ -22,6 +22,13 @@
  *                     brute-force membership.
  *   orc_walk_rows     independent forward BFS by exact walk length; used to
  *                     spot-check rows of full-size indexes.
+ *   orc_adjacency     dense adjacency built from its definition on small
+ *                     graphs, nnz = loci - nodes + edges
+ *                     (adjacency.hpp:192-194), and the block-diagonal
+ *                     assembly of components (test_dindex.cpp:84-173).
+ *   orc_merge_stream  union of sets on canonical indices (the merge tests of
+ *                     test_crsmatrix.cpp:911-1165) and equality with
+ *                     orc_spadd there.
  * The checks live in tests/test_oracle.py.
  */
 #include <stdint.h>
@@ -636,6 +643,136 @@ int orc_walk_rows(const orc_rcrs *a, const uint32_t *rows, uint64_t nr,
   free(res);
   free(rlen);
   return rc;
+}
+
+/* Character-level adjacency of nodes [lower, upper), restating
+ * util::adjacency_matrix (adjacency.hpp:170-228): row l -> l + 1 inside a
+ * node; the last locus of a node -> the first locus of each out-neighbour,
+ * sorted (std::sort, duplicates kept); columns relative to the first locus of
+ * node `lower`.  The Basic rows are then converted to ranges as
+ * append(..., RangeGroup, BasicGroup) does (crs_matrix.hpp:197-242): a
+ * column extends the open range only when it equals the range's end + 1.
+ * nnz is the number of Basic entries.  Edges must stay inside the range. */
+int orc_adjacency(uint64_t n_nodes, const uint32_t *node_len, const uint64_t *eoff,
+                  const uint32_t *etgt, uint64_t lower, uint64_t upper, orc_rcrs *out) {
+  if (upper == 0) upper = n_nodes;
+  if (lower > upper || upper > n_nodes) return ORC_EINVAL;
+  uint64_t *start = (uint64_t *)malloc(sizeof(uint64_t) * (upper - lower + 1));
+  if (!start) return ORC_ENOMEM;
+  start[0] = 0;
+  for (uint64_t k = lower; k < upper; ++k) start[k - lower + 1] = start[k - lower] + node_len[k];
+  const uint64_t n = start[upper - lower];
+  uint64_t basic = 0, max_deg = 0;
+  for (uint64_t k = lower; k < upper; ++k) {
+    basic += node_len[k] - 1 + (eoff[k + 1] - eoff[k]);
+    if (eoff[k + 1] - eoff[k] > max_deg) max_deg = eoff[k + 1] - eoff[k];
+    for (uint64_t q = eoff[k]; q < eoff[k + 1]; ++q)
+      if (etgt[q] < lower || etgt[q] >= upper) {
+        free(start);
+        return ORC_EINVAL;
+      }
+  }
+  uint32_t *cols = (uint32_t *)malloc(sizeof(uint32_t) * (max_deg ? max_deg : 1));
+  if (!cols || alloc_rcrs(out, n, n, 2 * basic)) {
+    free(start);
+    free(cols);
+    return ORC_ENOMEM;
+  }
+  uint64_t w = 0, row = 0;
+  for (uint64_t k = lower; k < upper; ++k) {
+    const uint64_t s0 = start[k - lower];
+    for (uint64_t j = 0; j + 1 < node_len[k]; ++j) {  /* inside the node */
+      out->entries[w++] = (uint32_t)(s0 + j + 1);
+      out->entries[w++] = (uint32_t)(s0 + j + 1);
+      out->rowmap[++row] = w;
+    }
+    const uint64_t deg = eoff[k + 1] - eoff[k];
+    for (uint64_t q = 0; q < deg; ++q) cols[q] = (uint32_t)start[etgt[eoff[k] + q] - lower];
+    qsort(cols, deg, sizeof(uint32_t), cmp_u32);
+    int pending = 0;
+    uint32_t pend = 0;
+    for (uint64_t q = 0; q < deg; ++q) {
+      if (pending && cols[q] == pend + 1) {
+        ++pend;
+      } else {
+        if (pending) out->entries[w++] = pend;
+        out->entries[w++] = cols[q];
+        pend = cols[q];
+        pending = 1;
+      }
+    }
+    if (pending) out->entries[w++] = pend;
+    out->rowmap[++row] = w;
+  }
+  out->nnz = basic;
+  free(start);
+  free(cols);
+  return ORC_OK;
+}
+
+/* The streaming merge of two Range-group distance indices, restating
+ * merge_distance_index(..., RangeGroup, std::true_type) (dindex.hpp:290-420):
+ * per row, while both have ranges, take the one with the smaller (start,
+ * end), then absorb ranges from either side while they touch (u + 1 >= l);
+ * when one side is exhausted the rest of the other is copied verbatim.  nnz
+ * is the sum of the emitted range lengths. */
+int orc_merge_stream(const orc_rcrs *a, const orc_rcrs *b, orc_rcrs *c) {
+  if (a->nrows != b->nrows || a->ncols != b->ncols) return ORC_EINVAL;
+  const uint64_t cap = a->rowmap[a->nrows] + b->rowmap[b->nrows];
+  if (alloc_rcrs(c, a->nrows, a->ncols, cap)) return ORC_ENOMEM;
+  const uint32_t *e1 = a->entries, *e2 = b->entries;
+  uint64_t c1 = 0, c2 = 0, w = 0, nnz = 0;
+  for (uint64_t r = 0; r < a->nrows; ++r) {
+    const uint64_t end1 = a->rowmap[r + 1], end2 = b->rowmap[r + 1];
+    while (c1 < end1) {
+      if (c2 >= end2) {
+        while (c1 < end1) {
+          const uint32_t l = e1[c1++], u = e1[c1++];
+          c->entries[w++] = l;
+          c->entries[w++] = u;
+          nnz += (uint64_t)u - l + 1;
+        }
+        break;
+      }
+      uint32_t l, u;
+      if (e1[c1] < e2[c2] || (e1[c1] == e2[c2] && e1[c1 + 1] < e2[c2 + 1])) {
+        l = e1[c1];
+        u = e1[c1 + 1];
+        c1 += 2;
+      } else {
+        l = e2[c2];
+        u = e2[c2 + 1];
+        c2 += 2;
+      }
+      for (;;) {
+        const uint64_t l1 = c1 < end1 ? e1[c1] : UINT32_MAX, u1 = c1 < end1 ? e1[c1 + 1] : UINT32_MAX;
+        const uint64_t l2 = c2 < end2 ? e2[c2] : UINT32_MAX, u2 = c2 < end2 ? e2[c2 + 1] : UINT32_MAX;
+        if ((uint64_t)u + 1 >= l1) {
+          if (l1 < l) l = (uint32_t)l1;
+          if (u1 > u) u = (uint32_t)u1;
+          c1 += 2;
+        } else if ((uint64_t)u + 1 >= l2) {
+          if (l2 < l) l = (uint32_t)l2;
+          if (u2 > u) u = (uint32_t)u2;
+          c2 += 2;
+        } else {
+          break;
+        }
+      }
+      c->entries[w++] = l;
+      c->entries[w++] = u;
+      nnz += (uint64_t)u - l + 1;
+    }
+    while (c2 < end2) {
+      const uint32_t l = e2[c2++], u = e2[c2++];
+      c->entries[w++] = l;
+      c->entries[w++] = u;
+      nnz += (uint64_t)u - l + 1;
+    }
+    c->rowmap[r + 1] = w;
+  }
+  c->nnz = nnz;
+  return ORC_OK;
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }

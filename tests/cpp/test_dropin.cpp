@@ -1,15 +1,19 @@
 // Drop-in check of the C++ wrapper headers: the reference's call sites
 // (diverg::util::create_distance_index, diverg::range_spgemm / range_spadd /
-// range_power, CRSMatrix<RangeDynamic,...>::operator(), serialize/load) compiled
+// range_power, CRSMatrix<RangeDynamic,...>::operator(), serialize/load,
+// util::range_adjacency_matrix, upper_node_rank_in_component,
+// util::merge_distance_index) compiled
 // unchanged against diverg_amd/include and compared with the CPU oracle, in the
 // manner of check_identical (test_dindex.cpp:34-53).  Needs a GPU.
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <sstream>
+#include <string>
 #include <stdexcept>
 #include <vector>
 
+#include <diverg/adjacency.hpp>
 #include <diverg/crs_matrix.hpp>
 #include <diverg/dindex.hpp>
 #include <diverg/range_sparse.hpp>
@@ -29,6 +33,8 @@ int orc_spgemm(const orc_rcrs *a, const orc_rcrs *b, orc_rcrs *c);
 int orc_spadd(const orc_rcrs *a, const orc_rcrs *b, orc_rcrs *c);
 int orc_power(const orc_rcrs *a, uint32_t k, orc_rcrs *c);
 int orc_identity(uint64_t n, orc_rcrs *out);
+int orc_adjacency(uint64_t n_nodes, const uint32_t *node_len, const uint64_t *eoff,
+                  const uint32_t *etgt, uint64_t lower, uint64_t upper, orc_rcrs *out);
 void orc_free(orc_rcrs *m);
 }
 
@@ -137,10 +143,59 @@ static void scenario(const char *name) {
   orc_free(&e);
 }
 
-int main() {
+// test_dindex.cpp:55-240 on the reference's multi.gfa: the adjacency of the
+// whole graph and of each component (walked with upper_node_rank_in_component)
+// and distance indices of it, against the oracle.
+template <typename TSize>
+static void graph_scenario(const std::string &gfa, unsigned dmin, unsigned dmax) {
+  std::printf("scenario: adjacency and distance index of %s, size type %zu bytes\n",
+              gfa.c_str(), sizeof(TSize));
+  diverg::amd::SeqGraph graph;
+  diverg::amd::load_gfa(graph, gfa);
+  const uint64_t n = graph.get_node_count();
+  std::vector<uint32_t> len(n);
+  std::vector<uint64_t> eoff(n + 1);
+  dv_graph_info info = graph.info();
+  std::vector<uint32_t> etgt(info.edges ? info.edges : 1);
+  dv_graph_arrays(graph.handle(), len.data(), nullptr, eoff.data(), etgt.data());
+  auto ra = diverg::util::range_adjacency_matrix<rcrs_t<TSize>>(graph);
+  orc_rcrs e;
+  REQUIRE(orc_adjacency(n, len.data(), eoff.data(), etgt.data(), 0, 0, &e) == 0);
+  check_identical(ra, e);
+  orc_free(&e);
+  REQUIRE(ra.numRows() == diverg::util::total_nof_loci(graph));
+  std::size_t checked = 0;
+  for (uint64_t lower = 1; lower <= n;) {
+    const uint64_t upper = std::get<0>(diverg::util::upper_node_rank_in_component(graph, lower));
+    auto block = diverg::util::range_adjacency_matrix<rcrs_t<TSize>>(graph, lower, upper);
+    REQUIRE(orc_adjacency(n, len.data(), eoff.data(), etgt.data(), lower - 1, upper - 1, &e) == 0);
+    check_identical(block, e);
+    orc_free(&e);
+    ++checked;
+    lower = upper;
+  }
+  REQUIRE(checked == std::get<2>(diverg::util::upper_node_rank_in_component(graph, 1)));
+  REQUIRE(checked >= 2);
+  OracleView<TSize> ov(ra);
+  auto d = diverg::util::create_distance_index(ra, dmin, dmax);
+  REQUIRE(orc_dindex(&ov.o, dmin, dmax, &e) == 0);
+  check_identical(d, e);
+  // merging the windows [dmin, mid] and [mid + 1, dmax] gives [dmin, dmax]
+  const unsigned mid = (dmin + dmax) / 2;
+  auto d1 = diverg::util::create_distance_index(ra, dmin, mid);
+  auto d2 = diverg::util::create_distance_index(ra, mid + 1, dmax);
+  auto merged = diverg::util::merge_distance_index<rcrs_t<TSize>>(d1, d2);
+  check_identical(merged, e);
+  orc_free(&e);
+}
+
+int main(int argc, char **argv) {
   diverg::ScopedRuntime runtime;
   scenario<uint64_t>("uint64_t");
   scenario<uint32_t>("uint32_t");
+  const std::string gfa = argc > 1 ? argv[1] : "tests/golden/gfa/multi.gfa";
+  graph_scenario<uint64_t>(gfa, 1, 8);   // test_dindex.cpp:180-206
+  graph_scenario<uint32_t>(gfa, 1, 3);   // test_dindex.cpp:208-240
   std::printf("%s (%d failures)\n", failures ? "FAILED" : "passed", failures);
   return failures ? 1 : 0;
 }

@@ -14,7 +14,8 @@ BIN = os.path.join(HERE, "cpp", "test_dropin")
 
 def test_cpp_dropin():
     assert os.path.exists(BIN), "build it with __graft_entry__.build()"
-    res = subprocess.run([BIN], capture_output=True, text=True, timeout=100)
+    gfa = os.path.join(HERE, "golden", "gfa", "multi.gfa")
+    res = subprocess.run([BIN, gfa], capture_output=True, text=True, timeout=100)
     print(res.stdout[-2000:], res.stderr[-2000:], flush=True)
     assert res.returncode == 0
     assert "passed (0 failures)" in res.stdout
