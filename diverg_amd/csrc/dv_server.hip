@@ -5,10 +5,8 @@
 // are queued into the next batch whenever a batch slot is free (so queries
 // that arrive while earlier batches are on the GPU ride the next launch) and
 // completes batches as their events fire; one build thread admits and runs
-// builds.  Queries run on their own stream, optionally restricted to a few
-// CUs (hipExtStreamCreateWithCUMask) so that long build kernels, whose
-// persistent grids fill the remaining CUs, never hold a query batch back.
-// All streams keep the default priority.
+// builds.  Queries run on their own stream (their own hardware queue), builds
+// on another.  All streams keep the default priority.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -317,6 +315,8 @@ extern "C" {
 int dv_server_start(int device, const dv_server_opts *opts, dv_server **out) {
   if (!out) return DV_ERR_INVALID;
   *out = nullptr;
+  // CU-masked query streams are switched off (query_cus must be 0)
+  if (opts && opts->query_cus) return DV_ERR_INVALID;
   auto *s = new (std::nothrow) dv_server();
   if (!s) return DV_ERR_NOMEM;
   if (opts) s->opts = *opts;
@@ -332,22 +332,6 @@ int dv_server_start(int device, const dv_server_opts *opts, dv_server **out) {
     return rc;
   }
   s->qstream = s->qctx->stream;
-  if (s->opts.query_cus) {
-    // restrict the query stream to query_cus CUs spread over the chip
-    const uint32_t ncu = s->qctx->num_cu;
-    std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
-    const uint32_t want = std::min(s->opts.query_cus, ncu);
-    for (uint32_t k = 0; k < want; ++k) {
-      const uint32_t cu = (uint32_t)((uint64_t)k * ncu / want);
-      mask[cu / 32] |= 1u << (cu % 32);
-    }
-    hipStream_t st = nullptr;
-    if (hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) == hipSuccess) {
-      s->qstream = st;
-    } else {
-      (void)hipGetLastError();
-    }
-  }
   size_t free_b = 0, total_b = 0;
   (void)hipSetDevice(device);
   (void)hipMemGetInfo(&free_b, &total_b);

@@ -250,7 +250,7 @@ int dv_load(dv_context *ctx, const char *path, uint32_t size_bytes, uint32_t loc
  * whose queries are host binary searches, crs_matrix.hpp:149-169).
  * Continuous batching: queries that arrive while earlier batches are on the
  * GPU are packed into the next launch (up to max_inflight batches in flight),
- * on a query stream that may be pinned to a few CUs so builds cannot delay it.
+ * on a query stream of its own.
  * Index builds run on a second stream; a build is admitted when its predicted
  * HBM footprint fits the budget, and its index joins the query table when it
  * completes.  Answers equal dv_query_batch on the same index. */
@@ -260,7 +260,8 @@ typedef struct dv_server_opts {
   uint64_t hbm_budget_bytes; /* indexes + running builds; 0: 75% of free HBM */
   uint32_t max_batch;        /* queries per launch; 0: 65536 */
   uint32_t max_inflight;     /* batches in flight; 0: 2 */
-  uint32_t query_cus;        /* CUs the query stream is restricted to (0: all) */
+  uint32_t query_cus;        /* reserved, must be 0 (CU-masked query streams are
+                                switched off; nonzero is DV_ERR_INVALID) */
   uint32_t pad;
 } dv_server_opts;
 
