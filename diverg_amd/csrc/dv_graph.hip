@@ -8,7 +8,7 @@
 //
 // The host computes each row count's ingredients and the exact output size
 // (runs per node), so the build needs no readback for allocation: count per
-// locus -> look-back scan into the rowmap -> write.  Threads take chunks of
+// locus -> row-pointer scan -> write.  Threads take chunks of
 // ADJ_CHUNK consecutive loci, find their node once by binary search over the
 // node starts, then walk forward.
 #include <hip/hip_runtime.h>
@@ -139,7 +139,7 @@ int Ctx::adjacency_dev(const Graph &g, uint64_t lower, uint64_t upper, uint32_t 
   auto *d_eoff = static_cast<uint64_t *>(dalloc(8 * (m + 1)));
   auto *d_etgt = static_cast<uint32_t *>(dalloc(4 * std::max<size_t>(1, etgt.size())));
   void *buf[2] = {nullptr, nullptr};
-  const uint64_t sz[2] = {4 * std::max<uint64_t>(1, n), 8 * lookback_words(n)};
+  const uint64_t sz[2] = {4 * std::max<uint64_t>(1, n), 8 * scan_words(n)};
   auto cleanup = [&]() {
     dfree(d_start);
     dfree(d_eoff);
@@ -172,7 +172,7 @@ int Ctx::adjacency_dev(const Graph &g, uint64_t lower, uint64_t upper, uint32_t 
         hipLaunchKernelGGL((k_adjacency<false, uint64_t>), dim3(adj_grid(n)), dim3(ADJ_BLOCK), 0,
                            stream, d_start, m, d_eoff, d_etgt, n, cnt, nullptr, nullptr, d_status);
     }
-    scan_lookback(stream, sb, cnt, n, out.rowmap, tiles, d_ctr, out.n_entries, d_status, nullptr,
+    scan_rowmap(stream, sb, cnt, n, out.rowmap, tiles, d_ctr, out.n_entries, d_status, nullptr,
                   0, nullptr);
     if (n) {
       if (sb == 4)

@@ -31,7 +31,7 @@ bool Ctx::op_scratch(uint64_t n, OpScratch &s) {
   const uint64_t n1 = n ? n : 1;
   const uint32_t KS = tuning.stage_entries;
   void *buf[7];
-  const uint64_t sz[7] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * lookback_words(n),
+  const uint64_t sz[7] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * scan_words(n),
                           8 * summary_words(n1)};
   if (!scratch_carve(sz, buf, 7)) return false;
   s.summ = buf[6];
@@ -62,7 +62,7 @@ void Ctx::enq_spadd_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint
                           const OpScratch &s, OpCounters *ctr, uint64_t cap,
                           unsigned int *status) {
   spadd_count(stream, a.sb, view(a), view(b), s.cnt, status);
-  scan_lookback(stream, c_sb, s.cnt, a.nrows, c_rowmap, s.tmp, ctr, cap, status, nullptr, 0,
+  scan_rowmap(stream, c_sb, s.cnt, a.nrows, c_rowmap, s.tmp, ctr, cap, status, nullptr, 0,
                 nullptr);
 }
 
@@ -84,7 +84,7 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
                            bool summaries) {
   const uint64_t n = a.nrows;
   // the summary buffer of the scratch holds summary_words(n + 1) words
-  summaries = summaries && b.nrows <= n + 1 && tuning.summary_span >= 64;
+  summaries = summaries && b.nrows <= n + 1 && tuning.summary_span >= 2;
   if (n) {
     if (summaries) block_summary(stream, b.sb, view(b), s.summ, status);
     // persistent lane-tier grid: few blocks keep the per-block counter atomics
@@ -112,7 +112,7 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
   } else if (after_lane) {
     (void)hipEventRecord(after_lane, stream);
   }
-  scan_lookback(stream, c_sb, s.cnt, n, c_rowmap, s.tmp, ctr, cap, status, s.stage,
+  scan_rowmap(stream, c_sb, s.cnt, n, c_rowmap, s.tmp, ctr, cap, status, s.stage,
                 s.stage_entries, c_entries);
 }
 
@@ -159,10 +159,6 @@ int Ctx::spadd_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   if (!read_counters(h, &st)) {
     release(c);
     return DV_ERR_DEVICE;
-  }
-  if (st & 4) {
-    release(c);
-    return fail("spadd: row-pointer scan look-back did not complete", DV_ERR_DEVICE);
   }
   if (st & 2) {
     release(c);
@@ -215,10 +211,6 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   if (!hip(hipGetLastError(), "spgemm count launch") || !read_counters(h, &st)) {
     release(c);
     return DV_ERR_DEVICE;
-  }
-  if (st & 4) {
-    release(c);
-    return fail("spgemm: row-pointer scan look-back did not complete", DV_ERR_DEVICE);
   }
   if (st & 2) {
     release(c);

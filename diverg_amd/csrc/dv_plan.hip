@@ -366,10 +366,6 @@ int Ctx::plan_run(Plan &p, DevMat &out, const std::function<int(DevMat &)> &exac
     release_all();
     return rc;
   }
-  if (st & 4) {
-    release_all();
-    return fail("row-pointer scan look-back did not complete", DV_ERR_DEVICE);
-  }
   if (st & 2) {
     release_all();
     return fail("entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
@@ -500,7 +496,7 @@ int Ctx::autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning 
       {&dv_tuning::lanes_small, {8, 16, 32}},
       {&dv_tuning::small_slice_max, {32, 64, 128, 256}},
       {&dv_tuning::lane_slice_max, {1024, 4096, 16384}},
-      {&dv_tuning::summary_span, {0, 64, 256}},
+      {&dv_tuning::summary_span, {0, 16, 64}},
       {&dv_tuning::lds_words, {lds_max / 4, lds_max / 2, lds_max}},
       {&dv_tuning::rows_per_group, {1, 4, 16}},
   };

@@ -253,38 +253,78 @@ __device__ __forceinline__ void fold_segment(const uint32_t *__restrict__ en, ui
   }
 }
 
-// Block summaries: summ[b] describes B rows [32b, 32b + 32): (lo, hi) when the
-// union of their ranges is the single interval [lo, hi] (each range, in row
-// order, touches the union of the ranges before it), (0xFFFFFFFF, 0) when the
-// rows are empty, (1, 0) otherwise.
+// Interval summaries of the right operand, per 32-row block k (rows
+// 32k .. 32k+31): pre[r] is the union of rows 32k .. r and suf[r] the union
+// of rows r .. 32k+31, each (lo, hi) when that union is known to be one
+// interval, (0xFFFFFFFF, 0) when the rows are empty, (1, 0) otherwise.  The
+// whole block's union is pre[32k + 31].  A union is "known to be one
+// interval" when the pieces joined by the scan below touch; the grouping can
+// miss a union that is an interval (it then reads as unknown and the lane
+// tier reads those rows' ranges), never claim one that is not.
 constexpr int SUMM_ROWS = 32;
 constexpr uint32_t SUMM_EMPTY = 0xFFFFFFFFu;
 
+struct Ivl {
+  uint32_t lo, hi, st;  // st: 0 empty, 1 one interval, 2 unknown
+};
+
+__device__ __forceinline__ Ivl ivl_join(Ivl a, Ivl b) {
+  if (a.st == 0) return b;
+  if (b.st == 0) return a;
+  if (a.st == 2 || b.st == 2) return Ivl{1u, 0u, 2u};
+  if (b.lo <= a.hi + 1u && b.hi + 1u >= a.lo) return Ivl{min(a.lo, b.lo), max(a.hi, b.hi), 1u};
+  return Ivl{1u, 0u, 2u};
+}
+
+__device__ __forceinline__ uint2 ivl_enc(Ivl v) {
+  return v.st == 1 ? make_uint2(v.lo, v.hi) : v.st == 0 ? make_uint2(SUMM_EMPTY, 0u) : make_uint2(1u, 0u);
+}
+
 template <typename S>
-__global__ __launch_bounds__(BLOCK) void k_block_summary(Mat<S> B, uint64_t nrows,
-                                                         uint2 *__restrict__ summ,
-                                                         const unsigned int *__restrict__ status) {
+__global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t nrows,
+                                                            uint2 *__restrict__ pre,
+                                                            uint2 *__restrict__ suf,
+                                                            const unsigned int *__restrict__ status) {
   if (*status) return;
-  const uint64_t nb = (nrows + SUMM_ROWS - 1) / SUMM_ROWS;
-  for (uint64_t b = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; b < nb;
-       b += (uint64_t)gridDim.x * BLOCK) {
-    const uint64_t r0 = b * SUMM_ROWS, r1 = min(r0 + SUMM_ROWS, nrows);
-    const uint64_t p1 = B.rm[r1];
-    uint32_t lo = SUMM_EMPTY, hi = 0;
-    bool valid = true;
-    for (uint64_t p = B.rm[r0]; valid && p < p1; p += 2) {
-      const uint2 v = ld_range(B.en, p);
-      if (lo > hi) {
-        lo = v.x;
-        hi = v.y;
-      } else if (v.x <= hi + 1u && v.y + 1u >= lo) {
-        lo = min(lo, v.x);
-        hi = max(hi, v.y);
-      } else {
-        valid = false;
+  const int l32 = threadIdx.x & 31;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows;
+       base += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t r = base + threadIdx.x;
+    Ivl v{1u, 0u, 0u};
+    if (r < nrows) {
+      const uint64_t p1 = B.rm[r + 1];
+      for (uint64_t p = B.rm[r]; p < p1 && v.st != 2; p += 2) {
+        const uint2 x = ld_range(B.en, p);
+        v = ivl_join(v, Ivl{x.x, x.y, 1u});
       }
     }
-    summ[b] = valid ? make_uint2(lo, hi) : make_uint2(1u, 0u);
+    Ivl f = v, g = v;
+#pragma unroll
+    for (int d = 1; d < SUMM_ROWS; d <<= 1) {
+      const Ivl o{__shfl_up(f.lo, d, SUMM_ROWS), __shfl_up(f.hi, d, SUMM_ROWS),
+                  __shfl_up(f.st, d, SUMM_ROWS)};
+      if (l32 >= d) f = ivl_join(o, f);
+      const Ivl u{__shfl_down(g.lo, d, SUMM_ROWS), __shfl_down(g.hi, d, SUMM_ROWS),
+                  __shfl_down(g.st, d, SUMM_ROWS)};
+      if (l32 + d < SUMM_ROWS) g = ivl_join(g, u);
+    }
+    if (r < nrows) {
+      pre[r] = ivl_enc(f);
+      suf[r] = ivl_enc(g);
+    }
+  }
+}
+
+// Fold the summary of rows [r0, r1): its interval, nothing when empty, the
+// rows' ranges when unknown.
+__device__ __forceinline__ void fold_summary(uint2 sm, const uint32_t *__restrict__ en,
+                                             uint64_t p0, uint64_t p1, uint32_t &cl, uint32_t &ch,
+                                             Runs &r, bool &ok, unsigned long long &gathered) {
+  if (sm.x <= sm.y) {
+    fold(sm.x, sm.y, cl, ch, r, ok);
+    gathered += 1;
+  } else if (sm.x != SUMM_EMPTY) {
+    fold_segment(en, p0, p1, cl, ch, r, ok, gathered);
   }
 }
 
@@ -294,8 +334,8 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     uint32_t *__restrict__ cnt, uint32_t *__restrict__ stage,
     uint32_t *__restrict__ ovf_list, uint32_t *__restrict__ mid_list,
     uint32_t *__restrict__ big_list, OpCounters *__restrict__ ctr,
-    const unsigned int *__restrict__ status, const uint2 *__restrict__ summ,
-    uint32_t summ_min) {
+    const unsigned int *__restrict__ status, const uint2 *__restrict__ pre,
+    const uint2 *__restrict__ suf, uint32_t summ_min) {
   if (*status) return;  // an earlier step overflowed: operands are not valid
   unsigned long long nnz_acc = 0, slice_acc = 0;
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows;
@@ -321,23 +361,18 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
       uint32_t cl = 0xFFFFFFFFu, ch = 0u;
       for (uint64_t ai = a0; ok && ai < a1; ai += 2) {
         const uint64_t s = A.en[ai], e = A.en[ai + 1];
-        if (summ && e - s + 1 >= summ_min) {
-          // head rows, whole 32-row blocks by their summaries, tail rows
-          // (summ_min >= 64 leaves at least one whole block in between)
+        if (pre && (s / SUMM_ROWS) != (e / SUMM_ROWS) && e - s + 1 >= summ_min) {
+          // rows [s, h1) by the suffix summary of s, whole blocks by their
+          // prefix summary at the block's last row, rows [t0, e] by the
+          // prefix summary of e
           const uint64_t h1 = (s + SUMM_ROWS - 1) / SUMM_ROWS * SUMM_ROWS;
           const uint64_t t0 = (e + 1) / SUMM_ROWS * SUMM_ROWS;
-          fold_segment(B.en, B.rm[s], B.rm[h1], cl, ch, r, ok, slice_acc);
-          for (uint64_t bb = h1 / SUMM_ROWS; ok && bb < t0 / SUMM_ROWS; ++bb) {
-            const uint2 sm = summ[bb];
-            if (sm.x <= sm.y) {
-              fold(sm.x, sm.y, cl, ch, r, ok);
-              slice_acc += 1;
-            } else if (sm.x != SUMM_EMPTY) {
-              fold_segment(B.en, B.rm[bb * SUMM_ROWS], B.rm[(bb + 1) * SUMM_ROWS], cl, ch, r, ok,
-                           slice_acc);
-            }
-          }
-          fold_segment(B.en, B.rm[t0], B.rm[e + 1], cl, ch, r, ok, slice_acc);
+          if (s < h1) fold_summary(suf[s], B.en, B.rm[s], B.rm[h1], cl, ch, r, ok, slice_acc);
+          for (uint64_t bb = h1; ok && bb < t0; bb += SUMM_ROWS)
+            fold_summary(pre[bb + SUMM_ROWS - 1], B.en, B.rm[bb], B.rm[bb + SUMM_ROWS], cl, ch, r,
+                         ok, slice_acc);
+          if (ok && t0 <= e)
+            fold_summary(pre[e], B.en, B.rm[t0], B.rm[e + 1], cl, ch, r, ok, slice_acc);
         } else {
           fold_segment(B.en, B.rm[s], B.rm[e + 1], cl, ch, r, ok, slice_acc);
         }
@@ -590,90 +625,108 @@ __device__ unsigned long long block_scan_incl(unsigned long long v,
 }
 
 // ---------------------------------------------------------------------------
-// Single-pass row-pointer scan with decoupled look-back.  Tiles of SCAN_TILE
-// counts take tickets in launch order (so every predecessor of a tile is
-// already running and publishes its aggregate without waiting), publish their
-// aggregate, walk back over predecessors until an inclusive prefix is found,
-// and publish their own inclusive prefix.  Tile words: bits 62-63 flag
-// (0 none, 1 aggregate, 2 inclusive prefix), bits 0-61 value.  A bounded
-// spin: a look-back that waits too long sets bit 2 of *status and goes on.
-// EMIT: rows whose count fits the stage are copied into C here; a tile whose
-// entries would pass `cap` writes none of them (the total then flags bit 0).
-constexpr unsigned long long LB_AGG = 1ull << 62, LB_INC = 2ull << 62,
-                             LB_VAL = (1ull << 62) - 1;
+// Row-pointer scan in two launches with no inter-block waiting: (1) each tile
+// of SCAN_TILE rows sums its counts; (2) each tile adds up the sums of all
+// tiles before it (a block reduction over at most a few thousand words, from
+// L2), scans its own rows and writes them.  Loads and stores of counts and
+// row pointers go through LDS so that global accesses are coalesced; the
+// per-thread run of SCAN_ITEMS consecutive rows is read from a padded LDS
+// layout (one pad word per 16) without bank conflicts.
+// EMIT: rows whose count fits the stage are copied into C here (stage is
+// column-major, so the copy walks rows across lanes); a tile whose entries
+// would pass `cap` writes none of them (the total then flags bit 0).
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v) {
+  __shared__ unsigned long long red[BLOCK / 64];
+  v = wave_sum_u64(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned long long t = 0;
+#pragma unroll
+  for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_tile_sums(const uint32_t *__restrict__ cnt, uint64_t n,
+                                                     unsigned long long *__restrict__ tsum) {
+  const uint64_t t0 = (uint64_t)blockIdx.x * SCAN_TILE;
+  unsigned long long v = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const uint64_t i = t0 + (uint64_t)k * BLOCK + threadIdx.x;
+    if (i < n) v += cnt[i];
+  }
+  v = block_sum_u64(v);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = v;
+}
+
+constexpr int SCAN_PAD = SCAN_TILE + SCAN_TILE / 16;
+__device__ __forceinline__ uint32_t scan_pad(uint32_t i) { return i + (i >> 4); }
 
 template <typename S, bool EMIT>
-__global__ __launch_bounds__(BLOCK) void k_scan_lookback(
+__global__ __launch_bounds__(BLOCK) void k_scan_rows(
     const uint32_t *__restrict__ cnt, uint64_t n, S *__restrict__ rm,
-    unsigned long long *__restrict__ tile_state, unsigned int *__restrict__ ticket,
-    const uint32_t *__restrict__ stage, uint32_t KS, uint32_t *__restrict__ c_en,
-    OpCounters *__restrict__ ctr, unsigned long long cap, uint32_t sb,
-    unsigned int *__restrict__ status) {
-  __shared__ unsigned int s_tile;
-  __shared__ unsigned long long s_excl;
-  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint64_t tile = s_tile;
-  const uint64_t base = tile * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
-  uint32_t v[SCAN_ITEMS];
-  unsigned long long tsum = 0;
+    const unsigned long long *__restrict__ tsum, const uint32_t *__restrict__ stage, uint32_t KS,
+    uint32_t *__restrict__ c_en, OpCounters *__restrict__ ctr, unsigned long long cap,
+    uint32_t sb, unsigned int *__restrict__ status) {
+  __shared__ uint32_t s_cnt[SCAN_PAD];
+  __shared__ unsigned long long s_ex[SCAN_PAD];
+  const uint64_t tile = blockIdx.x;
+  const uint64_t t0 = tile * SCAN_TILE;
+  // coalesced load of the tile's counts
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    v[k] = base + k < n ? cnt[base + k] : 0u;
-    tsum += v[k];
+    const uint32_t i = (uint32_t)k * BLOCK + threadIdx.x;
+    s_cnt[scan_pad(i)] = t0 + i < n ? cnt[t0 + i] : 0u;
+  }
+  // rows before this tile
+  unsigned long long e = 0;
+  for (uint64_t j = threadIdx.x; j < tile; j += BLOCK) e += tsum[j];
+  const unsigned long long texcl = block_sum_u64(e);  // (its barrier also orders s_cnt)
+  // this thread's SCAN_ITEMS consecutive rows
+  const uint32_t r0 = threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  unsigned long long tsum_t = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = s_cnt[scan_pad(r0 + k)];
+    tsum_t += v[k];
   }
   unsigned long long agg;
-  const unsigned long long incl = block_scan_incl(tsum, agg);
-  if (threadIdx.x == 0) {
-    unsigned long long excl = 0;
-    if (tile == 0) {
-      __hip_atomic_store(&tile_state[0], LB_INC | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __hip_atomic_store(&tile_state[tile], LB_AGG | agg, __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      int64_t j = (int64_t)tile - 1;
-      uint32_t spins = 0;
-      while (j >= 0) {
-        const unsigned long long w =
-            __hip_atomic_load(&tile_state[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long f = w & ~LB_VAL;
-        if (f == 0) {
-          if (++spins > (1u << 24)) {  // bounded wait
-            atomicOr(status, 4u);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(2);
-          continue;
-        }
-        excl += w & LB_VAL;
-        if (f == LB_INC) break;
-        --j;
-      }
-      __hip_atomic_store(&tile_state[tile], LB_INC | (excl + agg), __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-    s_excl = excl;
-    const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (tile + 1 == ntiles) {
-      const unsigned long long total = excl + agg;
-      ctr->total = total;
-      if (total > cap) atomicOr(status, 1u);
-      if (sb == 4 && total > 0xFFFFFFFFull) atomicOr(status, 2u);
-    }
-  }
-  __syncthreads();
-  const unsigned long long texcl = s_excl;
-  unsigned long long run = texcl + incl - tsum;
-  if (tile == 0 && threadIdx.x == 0) rm[0] = 0;
-  const bool fits = texcl + agg <= cap && (sb == 8 || texcl + agg <= 0xFFFFFFFFull);
+  const unsigned long long incl = block_scan_incl(tsum_t, agg);
+  unsigned long long run = texcl + incl - tsum_t;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    const uint64_t row = base + k;
+    s_ex[scan_pad(r0 + k)] = run;
+    run += v[k];
+  }
+  if (tile + 1 == gridDim.x && threadIdx.x == 0) {
+    const unsigned long long total = texcl + agg;
+    ctr->total = total;
+    if (total > cap) atomicOr(status, 1u);
+    if (sb == 4 && total > 0xFFFFFFFFull) atomicOr(status, 2u);
+  }
+  if (tile == 0 && threadIdx.x == 0) rm[0] = 0;
+  __syncthreads();
+  const bool fits = texcl + agg <= cap && (sb == 8 || texcl + agg <= 0xFFFFFFFFull);
+  // coalesced stores of the row pointers (and the staged entries)
+#pragma unroll 4
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const uint32_t i = (uint32_t)k * BLOCK + threadIdx.x;
+    const uint64_t row = t0 + i;
     if (row < n) {
-      if (EMIT && fits && v[k] <= KS)
-        for (uint32_t e = 0; e < v[k]; ++e) c_en[run + e] = stage[(uint64_t)e * n + row];
-      run += v[k];
-      rm[row + 1] = (S)run;
+      const uint32_t c = s_cnt[scan_pad(i)];
+      const unsigned long long ex = s_ex[scan_pad(i)];
+      rm[row + 1] = (S)(ex + c);
+      if (EMIT && fits && c <= KS)
+        for (uint32_t q = 0; q < c; ++q) c_en[ex + q] = stage[(uint64_t)q * n + row];
     }
   }
 }
@@ -1017,33 +1070,31 @@ void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *en
                        static_cast<uint64_t *>(rowmap), entries);
 }
 
-uint64_t lookback_words(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+uint64_t scan_words(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
-void scan_lookback(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
-                   unsigned long long *tile_state, OpCounters *ctr, uint64_t cap,
-                   unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
-                   uint32_t *c_entries) {
-  const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-  // tile words and the ticket (the last word) start at zero
-  fill_u32(s, reinterpret_cast<uint32_t *>(tile_state), 2 * (ntiles + 1), 0u);
+void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
+                 unsigned long long *tile_sums, OpCounters *ctr, uint64_t cap,
+                 unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
+                 uint32_t *c_entries) {
   if (n == 0) {
     fill_u32(s, static_cast<uint32_t *>(rowmap), sb / 4, 0u);
     return;
   }
-  auto *ticket = reinterpret_cast<unsigned int *>(tile_state + ntiles);
+  const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_tile_sums, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, tile_sums);
   const bool emit = c_entries != nullptr;
-#define DV_LB(S_, E_)                                                                        \
-  hipLaunchKernelGGL((k_scan_lookback<S_, E_>), dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, \
-                     n, static_cast<S_ *>(rowmap), tile_state, ticket, stage, stage_entries,     \
-                     c_entries, ctr, (unsigned long long)cap, sb, status)
+#define DV_SR(S_, E_)                                                                       \
+  hipLaunchKernelGGL((k_scan_rows<S_, E_>), dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, \
+                     static_cast<S_ *>(rowmap), tile_sums, stage, stage_entries, c_entries,     \
+                     ctr, (unsigned long long)cap, sb, status)
   if (sb == 4) {
-    if (emit) DV_LB(uint32_t, true);
-    else DV_LB(uint32_t, false);
+    if (emit) DV_SR(uint32_t, true);
+    else DV_SR(uint32_t, false);
   } else {
-    if (emit) DV_LB(uint64_t, true);
-    else DV_LB(uint64_t, false);
+    if (emit) DV_SR(uint64_t, true);
+    else DV_SR(uint64_t, false);
   }
-#undef DV_LB
+#undef DV_SR
 }
 
 void spadd_count(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt,
@@ -1082,28 +1133,29 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
                  const unsigned int *status, const void *summ, uint32_t summ_min) {
   if (!a.nrows) return;
-  const uint2 *sm = static_cast<const uint2 *>(summ);
+  const uint2 *pre = static_cast<const uint2 *>(summ);
+  const uint2 *suf = pre ? pre + b.nrows : nullptr;
   if (sb == 4)
     hipLaunchKernelGGL(k_spgemm_lane<uint32_t>, dim3(grid), dim3(BLOCK), 0, s, mat<uint32_t>(a),
                        mat<uint32_t>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list,
-                       mid_list, big_list, ctr, status, sm, summ_min);
+                       mid_list, big_list, ctr, status, pre, suf, summ_min);
   else
     hipLaunchKernelGGL(k_spgemm_lane<uint64_t>, dim3(grid), dim3(BLOCK), 0, s, mat<uint64_t>(a),
                        mat<uint64_t>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list,
-                       mid_list, big_list, ctr, status, sm, summ_min);
+                       mid_list, big_list, ctr, status, pre, suf, summ_min);
 }
 
-uint64_t summary_words(uint64_t nrows) { return (nrows + SUMM_ROWS - 1) / SUMM_ROWS; }
+uint64_t summary_words(uint64_t nrows) { return 2 * nrows; }
 
 void block_summary(hipStream_t s, uint32_t sb, DView b, void *summ, const unsigned int *status) {
-  const uint64_t nb = summary_words(b.nrows);
-  if (!nb) return;
+  if (!b.nrows) return;
+  uint2 *pre = static_cast<uint2 *>(summ), *suf = pre + b.nrows;
   if (sb == 4)
-    hipLaunchKernelGGL(k_block_summary<uint32_t>, dim3(grid_for(nb)), dim3(BLOCK), 0, s,
-                       mat<uint32_t>(b), b.nrows, static_cast<uint2 *>(summ), status);
+    hipLaunchKernelGGL(k_interval_summary<uint32_t>, dim3(grid_for(b.nrows)), dim3(BLOCK), 0, s,
+                       mat<uint32_t>(b), b.nrows, pre, suf, status);
   else
-    hipLaunchKernelGGL(k_block_summary<uint64_t>, dim3(grid_for(nb)), dim3(BLOCK), 0, s,
-                       mat<uint64_t>(b), b.nrows, static_cast<uint2 *>(summ), status);
+    hipLaunchKernelGGL(k_interval_summary<uint64_t>, dim3(grid_for(b.nrows)), dim3(BLOCK), 0, s,
+                       mat<uint64_t>(b), b.nrows, pre, suf, status);
 }
 
 void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunch &g) {

@@ -55,15 +55,14 @@ void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *en
 
 // Row-pointer scan: rowmap[0] = 0, rowmap[i+1] = cnt[0] + ... + cnt[i];
 // total -> ctr->total; a total above `cap` sets bit 0 of *status, above a
-// 32-bit rowmap bit 1.  One pass (decoupled look-back over tiles) and, when
+// 32-bit rowmap bit 1.  Two launches (tile sums, then rows) and, when
 // c_entries is given, the copy of staged rows into C (column-major stage of
-// stage_entries per row).  tile_state needs lookback_words(n) words; it is
-// cleared by the call.  A look-back that does not complete sets bit 2.
-uint64_t lookback_words(uint64_t n);
-void scan_lookback(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
-                   unsigned long long *tile_state, OpCounters *ctr, uint64_t cap,
-                   unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
-                   uint32_t *c_entries);
+// stage_entries per row).  tile_sums needs scan_words(n) words.
+uint64_t scan_words(uint64_t n);
+void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
+                 unsigned long long *tile_sums, OpCounters *ctr, uint64_t cap,
+                 unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
+                 uint32_t *c_entries);
 
 // Kernels that read operands take `status` and do nothing once it is set.
 void spadd_count(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt,
@@ -85,10 +84,12 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
                  uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
                  const unsigned int *status, const void *summ, uint32_t summ_min);
-// Per-32-row block summaries of B (8 bytes each, summary_words(nrows) of
-// them): the single interval the block's ranges union to, when they do.  The
-// lane tier then folds an A range of at least summ_min (>= 64) rows as head
-// rows, whole blocks by their summaries, and tail rows.
+// Interval summaries of B in 32-row blocks: per row, the union of the rows
+// from its block's start to it (prefix) and from it to its block's end
+// (suffix), as one interval when known to be one; summary_words(nrows) uint2
+// words (prefixes, then suffixes).  The lane tier folds an A range of at least
+// summ_min rows that crosses a block boundary as a suffix, whole blocks and a
+// prefix: 2 + span / 32 reads instead of the rows' ranges.
 uint64_t summary_words(uint64_t nrows);
 void block_summary(hipStream_t s, uint32_t sb, DView b, void *summ, const unsigned int *status);
 // Group tiers (counting or direct write), see GroupLaunch.

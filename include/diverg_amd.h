@@ -125,9 +125,13 @@ void *dv_stream(dv_context *ctx);
  *   graphs          1: the launches of a planned build are captured once into
  *                   a hipGraph and replayed while the build's operations,
  *                   buffers and tuning stay the same (not while profiling)
- *   summary_span    planned SpGEMMs whose left operand's ranges span at least
- *                   this many rows (>= 64; 0: never) read whole 32-row blocks
- *                   of the right operand through per-block union summaries */
+ *   summary_span    planned SpGEMMs whose left operand's ranges may span at
+ *                   least this many rows (>= 2; 0: never) first summarise the
+ *                   right operand in 32-row blocks (per row, the union of the
+ *                   rows from its block's start / to its block's end when it
+ *                   is one interval); a left range of at least this many rows
+ *                   that crosses a block boundary is then read as one suffix,
+ *                   whole blocks and one prefix instead of row by row */
 typedef struct dv_tuning {
   uint32_t lds_words;
   uint32_t stage_entries;

@@ -143,7 +143,7 @@ def test_distance_index_autotuned(ctx, cfg1):
     assert t["lanes_small"] in (8, 16, 32) and t["stage_entries"] in (4, 8, 16)
 
 
-@pytest.mark.parametrize("summary_span", [0, 64, 200])
+@pytest.mark.parametrize("summary_span", [0, 2, 16, 64, 200])
 def test_distance_index_block_summaries(ctx, summary_span):
     """Block summaries on and off, on a graph whose bubbles make some 32-row
     blocks fail the single-interval test (they fall back to their ranges)."""
