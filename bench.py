@@ -57,6 +57,22 @@ def hip_sync():
         raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
 
 
+def measured_traffic(args):
+    """HBM bytes per k_spgemm_lane launch from the committed rocprofv3 counter
+    runs (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy's readings), for the
+    workload they were taken on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = t.get("workload", {})
+    if (w.get("preset"), w.get("d1"), w.get("d2")) != (args.preset, args.dlo, args.dup):
+        return None
+    return t.get("traffic_bytes_per_launch")
+
+
 def cpu_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -148,7 +164,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": measured_traffic(args),
         "launches": cnt["launches"],
         "mean_launch_ms": round(mean_ms, 4),
         "alg_bytes_per_launch": int(alg_per_launch),

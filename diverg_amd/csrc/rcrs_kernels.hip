@@ -1124,7 +1124,8 @@ void spadd_write(hipStream_t s, uint32_t sb, DView a, DView b, const void *c_row
 void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t max_entries, OpCounters *ctr,
                const unsigned int *status) {
   if (max_entries < 2) return;
-  hipLaunchKernelGGL(k_range_nnz, dim3(grid_for(max_entries / 2, BLOCK * 8, 8192)), dim3(BLOCK),
+  // few blocks: each adds its sum to one counter with one atomic
+  hipLaunchKernelGGL(k_range_nnz, dim3(grid_for(max_entries / 2, BLOCK * 8, 1024)), dim3(BLOCK),
                      0, s, reinterpret_cast<const uint2 *>(entries), ctr, status);
 }
 
