@@ -12,6 +12,10 @@ Further legs (fields of the same JSON line; --legs selects them):
   chromosome    the 5e7 bp graph with structural variants (uint64 rowmap,
                 [200, 1000]): build time, peak HBM, rows checked against
                 the oracle's independent walk-length BFS
+  queries       batched locus-pair queries on the HBM-resident chromosome
+                index (pairs drawn as in the serving workload, batches of
+                65536 resident in HBM): queries/s, per-batch p50/p99 latency,
+                the CPU lookup's rate on the same pairs, answers bit-exact
   serving       (not in the default legs: dv_server_start faulted in its
                 first GPU run and stays off the GPU until a cause is shown)
                 the chromosome index (and the 5e6 bp one) resident in HBM, a
@@ -90,7 +94,7 @@ def main():
     ap.add_argument("--dlo", type=int, default=300)
     ap.add_argument("--dup", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="plain,chromosome,batched",
+    ap.add_argument("--legs", default="plain,chromosome,queries,batched",
                     help="comma-separated legs to run (plain is always run); the serving leg "
                          "is off by default: dv_server_start faulted in its first GPU run")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
@@ -232,8 +236,10 @@ def main():
         "kernels_ms": {k: round(v["total_ms"] / args.steps, 4) for k, v in prof.items()},
     }
     legs = set(args.legs.split(","))
-    if "chromosome" in legs or "serving" in legs:
+    if "chromosome" in legs or "serving" in legs or "queries" in legs:
         out["chromosome"], chrom = leg_chromosome(ctx)
+        if "queries" in legs:
+            out["queries"] = leg_queries(ctx, chrom)
         if "serving" in legs:
             out["serving"] = leg_serving(chrom, result_host, args)
         del chrom
@@ -282,6 +288,95 @@ def leg_chromosome(ctx):
     }
     log(f"chromosome build {res['build_s']}s, rows bit-exact {ok}")
     return res, (a, vpos, dh)
+
+
+class DeviceBuffer:
+    """Raw device memory through the HIP runtime (for query arrays)."""
+
+    def __init__(self, nbytes):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.ptr = ctypes.c_void_p()
+        if self.hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(max(1, nbytes))) != 0:
+            raise RuntimeError("hipMalloc failed")
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        if self.hip.hipMemcpy(self.ptr, arr.ctypes.data_as(ctypes.c_void_p),
+                              ctypes.c_size_t(arr.nbytes), 1) != 0:
+            raise RuntimeError("hipMemcpy H2D failed")
+
+    def download(self, arr):
+        if self.hip.hipMemcpy(arr.ctypes.data_as(ctypes.c_void_p), self.ptr,
+                              ctypes.c_size_t(arr.nbytes), 2) != 0:
+            raise RuntimeError("hipMemcpy D2H failed")
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+def leg_queries(ctx, chrom, batch=65536, nbatch=256):
+    """Batched locus-pair queries on the HBM-resident chromosome index: pairs
+    drawn as the serving workload draws them (sources uniform; half the
+    targets 0-1000 bp downstream on the backbone, half uniform), in batches
+    already resident in HBM; per-batch latency p50/p99 and throughput;
+    every answer checked against the CPU lookup (crs_matrix.hpp:149-169)."""
+    from oracle import oracle as orc
+    a, vpos, dh = chrom
+    rng = np.random.default_rng(0xD1E5A)
+    n = batch * nbatch
+    src = rng.integers(0, a.nrows, n, dtype=np.int64)
+    down = np.minimum(vpos[src].astype(np.int64) + rng.integers(0, 1001, n), int(vpos[-1]))
+    near_t = np.searchsorted(vpos, down, side="left").astype(np.int64)
+    qi = src.astype(np.uint32)
+    qj = np.where(rng.random(n) < 0.5, near_t, rng.integers(0, a.nrows, n)).astype(np.uint32)
+    d = ctx.to_device(dh)
+    bi, bj, bo = DeviceBuffer(4 * n), DeviceBuffer(4 * n), DeviceBuffer(n)
+    bi.upload(qi)
+    bj.upload(qj)
+    ctx.query_device(d, bi.ptr.value, bj.ptr.value, bo.ptr.value, batch)  # warm
+    lat = np.zeros(nbatch)
+    hip_sync()
+    t0 = time.perf_counter()
+    for b in range(nbatch):
+        t = time.perf_counter()
+        ctx.query_device(d, bi.ptr.value + 4 * b * batch, bj.ptr.value + 4 * b * batch,
+                         bo.ptr.value + b * batch, batch)
+        lat[b] = time.perf_counter() - t
+    span = time.perf_counter() - t0
+    # one launch over every pair: the device's own query rate
+    hip_sync()
+    t = time.perf_counter()
+    ctx.query_device(d, bi.ptr.value, bj.ptr.value, bo.ptr.value, n)
+    big_s = time.perf_counter() - t
+    got = np.zeros(n, np.uint8)
+    bo.download(got)
+    for x in (bi, bj, bo):
+        x.free()
+    d.free()
+    orc.set_threads(cpu_threads())
+    t = time.perf_counter()
+    want, rc = orc.query(dh, qi, qj)
+    cpu_s = time.perf_counter() - t
+    res = {
+        "workload": "chromosome index resident in HBM; pairs as the serving workload (sources "
+                    "uniform, half the targets 0-1000 bp downstream on the backbone, half "
+                    f"uniform); {nbatch} batches of {batch} pairs resident in HBM",
+        "queries": n,
+        "batched_queries_per_s": round(n / span, 1),
+        "batch_latency_p50_us": round(float(np.percentile(lat, 50)) * 1e6, 2),
+        "batch_latency_p99_us": round(float(np.percentile(lat, 99)) * 1e6, 2),
+        "one_launch_queries_per_s": round(n / big_s, 1),
+        "cpu_queries_per_s": round(n / cpu_s, 1),
+        "cpu_threads": cpu_threads(),
+        "hits": int(got.sum()),
+        "bit_exact": bool(rc == 0 and np.array_equal(got, want)),
+    }
+    log(f"queries: {res['batched_queries_per_s']:.3g}/s in batches, p50 "
+        f"{res['batch_latency_p50_us']} us p99 {res['batch_latency_p99_us']} us, "
+        f"one launch {res['one_launch_queries_per_s']:.3g}/s, exact {res['bit_exact']}")
+    return res
 
 
 def leg_serving(chrom, idx2, args):

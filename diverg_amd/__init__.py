@@ -345,6 +345,17 @@ class Context:
                     "dv_adjacency_matrix")
         return self._out(out, location)
 
+    def query_device(self, m, qi_ptr, qj_ptr, out_ptr, n):
+        """Batched D(i, j) with the pairs and answers in device memory (raw
+        device pointers); blocks until the answers are written."""
+        keep = []
+        rc = N.lib().dv_query_batch(self._h, C.byref(m._desc(keep)),
+                                    C.cast(C.c_void_p(qi_ptr), C.POINTER(C.c_uint32)),
+                                    C.cast(C.c_void_p(qj_ptr), C.POINTER(C.c_uint32)),
+                                    C.cast(C.c_void_p(out_ptr), C.POINTER(C.c_uint8)), n,
+                                    N.DV_DEVICE)
+        self._check(rc, "dv_query_batch")
+
     def query(self, m, qi, qj):
         """Batched D(i, j); raises DiVerGError(DV_ERR_RANGE) for pairs outside."""
         keep = []

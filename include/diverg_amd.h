@@ -14,7 +14,8 @@
  *    CRSMatrix<RangeDynamic,bool,uint32_t,S>, crs_matrix.hpp:1873-2004).
  *    `size_bytes` is sizeof(S): 4 (uint32_t rowmap) or 8 (uint64_t rowmap).
  *  - Inputs may live on the host or on the device (`location`); they are not
- *    modified.  Outputs are allocated by the library at the requested
+ *    modified.  Device inputs are validated on the device before use, except
+ *    descriptors returned by this library (`owner` set).  Outputs are allocated by the library at the requested
  *    location and released with dv_free().
  *  - `nnz` is the number of set cells.  With size_bytes == 4 it is reduced
  *    modulo 2^32, which is what the reference's uint32_t accumulator yields
