@@ -257,27 +257,88 @@ __device__ __forceinline__ void fold_segment(const uint32_t *__restrict__ en, ui
 // 32k .. 32k+31): pre[r] is the union of rows 32k .. r and suf[r] the union
 // of rows r .. 32k+31, each (lo, hi) when that union is known to be one
 // interval, (0xFFFFFFFF, 0) when the rows are empty, (1, 0) otherwise.  The
-// whole block's union is pre[32k + 31].  A union is "known to be one
-// interval" when the pieces joined by the scan below touch; the grouping can
-// miss a union that is an interval (it then reads as unknown and the lane
-// tier reads those rows' ranges), never claim one that is not.
+// whole block's union is pre[32k + 31].
+// Partial unions in the scan are kept exactly as up to two disjoint
+// intervals, so a row with a gap (a bubble's alternate locus reaches past the
+// reference locus) does not spoil the block when neighbouring rows fill the
+// gap; a partial union of three or more pieces becomes "unknown" (the lane
+// tier then reads those rows' ranges).  A summary is never claimed for a
+// union that is not one interval.
 constexpr int SUMM_ROWS = 32;
 constexpr uint32_t SUMM_EMPTY = 0xFFFFFFFFu;
 
-struct Ivl {
-  uint32_t lo, hi, st;  // st: 0 empty, 1 one interval, 2 unknown
+struct Ivl2 {
+  uint32_t lo0, hi0, lo1, hi1;  // pieces in ascending order; an unused piece is (EMPTY, 0)
+  uint32_t st;                  // 0 empty, 1 the pieces, 2 unknown
 };
 
-__device__ __forceinline__ Ivl ivl_join(Ivl a, Ivl b) {
-  if (a.st == 0) return b;
-  if (b.st == 0) return a;
-  if (a.st == 2 || b.st == 2) return Ivl{1u, 0u, 2u};
-  if (b.lo <= a.hi + 1u && b.hi + 1u >= a.lo) return Ivl{min(a.lo, b.lo), max(a.hi, b.hi), 1u};
-  return Ivl{1u, 0u, 2u};
+__device__ __forceinline__ void cswap_iv(uint32_t &la, uint32_t &ha, uint32_t &lb, uint32_t &hb) {
+  const bool sw = lb < la;
+  const uint32_t l = sw ? lb : la, h = sw ? hb : ha;
+  lb = sw ? la : lb;
+  hb = sw ? ha : hb;
+  la = l;
+  ha = h;
 }
 
-__device__ __forceinline__ uint2 ivl_enc(Ivl v) {
-  return v.st == 1 ? make_uint2(v.lo, v.hi) : v.st == 0 ? make_uint2(SUMM_EMPTY, 0u) : make_uint2(1u, 0u);
+__device__ __forceinline__ Ivl2 ivl_join(const Ivl2 &a, const Ivl2 &b) {
+  if (a.st == 0) return b;
+  if (b.st == 0) return a;
+  Ivl2 u{SUMM_EMPTY, 0u, SUMM_EMPTY, 0u, 2u};
+  if (a.st == 2 || b.st == 2) return u;
+  uint32_t L0 = a.lo0, H0 = a.hi0, L1 = a.lo1, H1 = a.hi1, L2 = b.lo0, H2 = b.hi0, L3 = b.lo1,
+           H3 = b.hi1;
+  cswap_iv(L0, H0, L1, H1);
+  cswap_iv(L2, H2, L3, H3);
+  cswap_iv(L0, H0, L2, H2);
+  cswap_iv(L1, H1, L3, H3);
+  cswap_iv(L1, H1, L2, H2);
+  uint32_t l0 = SUMM_EMPTY, h0 = 0u, l1 = SUMM_EMPTY, h1 = 0u;
+  int k = 0;
+  bool over = false;
+  auto put = [&](uint32_t L, uint32_t H) {
+    if (L > H) return;  // unused piece
+    if (k == 0) {
+      l0 = L;
+      h0 = H;
+      k = 1;
+    } else if (k == 1) {
+      if (L <= h0 + 1u) {
+        h0 = max(h0, H);
+      } else {
+        l1 = L;
+        h1 = H;
+        k = 2;
+      }
+    } else if (L <= h1 + 1u) {
+      h1 = max(h1, H);
+    } else {
+      over = true;
+    }
+  };
+  put(L0, H0);
+  put(L1, H1);
+  put(L2, H2);
+  put(L3, H3);
+  if (over) return u;
+  return Ivl2{l0, h0, l1, h1, 1u};
+}
+
+__device__ __forceinline__ uint2 ivl_enc(const Ivl2 &v) {
+  if (v.st == 1 && v.lo1 > v.hi1) return make_uint2(v.lo0, v.hi0);
+  return v.st == 0 ? make_uint2(SUMM_EMPTY, 0u) : make_uint2(1u, 0u);
+}
+
+__device__ __forceinline__ Ivl2 ivl_shfl_up(const Ivl2 &v, int d) {
+  return Ivl2{__shfl_up(v.lo0, d, SUMM_ROWS), __shfl_up(v.hi0, d, SUMM_ROWS),
+              __shfl_up(v.lo1, d, SUMM_ROWS), __shfl_up(v.hi1, d, SUMM_ROWS),
+              __shfl_up(v.st, d, SUMM_ROWS)};
+}
+
+__device__ __forceinline__ Ivl2 ivl_shfl_down(const Ivl2 &v, int d) {
+  return Ivl2{__shfl_down(v.lo0, d, SUMM_ROWS), __shfl_down(v.hi0, d, SUMM_ROWS),
+              __shfl_down(v.lo1, d, SUMM_ROWS), __shfl_down(v.hi1, d, SUMM_ROWS),
+              __shfl_down(v.st, d, SUMM_ROWS)};
 }
 
 template <typename S>
@@ -290,22 +351,20 @@ __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t n
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows;
        base += (uint64_t)gridDim.x * BLOCK) {
     const uint64_t r = base + threadIdx.x;
-    Ivl v{1u, 0u, 0u};
+    Ivl2 v{SUMM_EMPTY, 0u, SUMM_EMPTY, 0u, 0u};
     if (r < nrows) {
       const uint64_t p1 = B.rm[r + 1];
       for (uint64_t p = B.rm[r]; p < p1 && v.st != 2; p += 2) {
         const uint2 x = ld_range(B.en, p);
-        v = ivl_join(v, Ivl{x.x, x.y, 1u});
+        v = ivl_join(v, Ivl2{x.x, x.y, SUMM_EMPTY, 0u, 1u});
       }
     }
-    Ivl f = v, g = v;
+    Ivl2 f = v, g = v;
 #pragma unroll
     for (int d = 1; d < SUMM_ROWS; d <<= 1) {
-      const Ivl o{__shfl_up(f.lo, d, SUMM_ROWS), __shfl_up(f.hi, d, SUMM_ROWS),
-                  __shfl_up(f.st, d, SUMM_ROWS)};
+      const Ivl2 o = ivl_shfl_up(f, d);
       if (l32 >= d) f = ivl_join(o, f);
-      const Ivl u{__shfl_down(g.lo, d, SUMM_ROWS), __shfl_down(g.hi, d, SUMM_ROWS),
-                  __shfl_down(g.st, d, SUMM_ROWS)};
+      const Ivl2 u = ivl_shfl_down(g, d);
       if (l32 + d < SUMM_ROWS) g = ivl_join(g, u);
     }
     if (r < nrows) {
