@@ -726,6 +726,39 @@ __global__ __launch_bounds__(BLOCK) void k_tile_sums(const uint32_t *__restrict_
   if (threadIdx.x == 0) tsum[blockIdx.x] = v;
 }
 
+// With many tiles, one block turns the tile sums into exclusive prefixes in
+// place (each of its threads sums a contiguous chunk, the chunk sums are
+// scanned, the chunks rewritten), so that a tile reads its prefix instead of
+// adding up all earlier tiles.
+constexpr int SPINE_THREADS = 1024;
+constexpr uint64_t SPINE_MIN_TILES = 2048;
+
+__global__ __launch_bounds__(SPINE_THREADS) void k_scan_spine(unsigned long long *__restrict__ tsum,
+                                                              uint64_t ntiles) {
+  __shared__ unsigned long long wsum[SPINE_THREADS / 64];
+  const uint64_t chunk = (ntiles + SPINE_THREADS - 1) / SPINE_THREADS;
+  const uint64_t b0 = min(ntiles, (uint64_t)threadIdx.x * chunk), b1 = min(ntiles, b0 + chunk);
+  unsigned long long v = 0;
+  for (uint64_t j = b0; j < b1; ++j) v += tsum[j];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned long long x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long t = __shfl_up(x, d, 64);
+    if (lane >= d) x += t;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  unsigned long long off = 0;
+  for (int w = 0; w < wid; ++w) off += wsum[w];
+  unsigned long long ex = off + x - v;
+  for (uint64_t j = b0; j < b1; ++j) {
+    const unsigned long long t = tsum[j];
+    tsum[j] = ex;
+    ex += t;
+  }
+}
+
 constexpr int SCAN_PAD = SCAN_TILE + SCAN_TILE / 16;
 __device__ __forceinline__ uint32_t scan_pad(uint32_t i) { return i + (i >> 4); }
 
@@ -734,7 +767,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(
     const uint32_t *__restrict__ cnt, uint64_t n, S *__restrict__ rm,
     const unsigned long long *__restrict__ tsum, const uint32_t *__restrict__ stage, uint32_t KS,
     uint32_t *__restrict__ c_en, OpCounters *__restrict__ ctr, unsigned long long cap,
-    uint32_t sb, unsigned int *__restrict__ status) {
+    uint32_t sb, unsigned int *__restrict__ status, int prefixed) {
   __shared__ uint32_t s_cnt[SCAN_PAD];
   __shared__ unsigned long long s_ex[SCAN_PAD];
   const uint64_t tile = blockIdx.x;
@@ -745,10 +778,17 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(
     const uint32_t i = (uint32_t)k * BLOCK + threadIdx.x;
     s_cnt[scan_pad(i)] = t0 + i < n ? cnt[t0 + i] : 0u;
   }
-  // rows before this tile
-  unsigned long long e = 0;
-  for (uint64_t j = threadIdx.x; j < tile; j += BLOCK) e += tsum[j];
-  const unsigned long long texcl = block_sum_u64(e);  // (its barrier also orders s_cnt)
+  // rows before this tile: its prefix from the spine, or the sum of the
+  // earlier tiles' sums
+  unsigned long long texcl;
+  if (prefixed) {
+    __syncthreads();  // orders the s_cnt stores before the reads below
+    texcl = tsum[tile];
+  } else {
+    unsigned long long e = 0;
+    for (uint64_t j = threadIdx.x; j < tile; j += BLOCK) e += tsum[j];
+    texcl = block_sum_u64(e);  // (its barrier also orders s_cnt)
+  }
   // this thread's SCAN_ITEMS consecutive rows
   const uint32_t r0 = threadIdx.x * SCAN_ITEMS;
   uint32_t v[SCAN_ITEMS];
@@ -1141,11 +1181,14 @@ void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, vo
   }
   const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   hipLaunchKernelGGL(k_tile_sums, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, tile_sums);
+  const int prefixed = ntiles >= SPINE_MIN_TILES;
+  if (prefixed)
+    hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(SPINE_THREADS), 0, s, tile_sums, ntiles);
   const bool emit = c_entries != nullptr;
 #define DV_SR(S_, E_)                                                                       \
   hipLaunchKernelGGL((k_scan_rows<S_, E_>), dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, \
                      static_cast<S_ *>(rowmap), tile_sums, stage, stage_entries, c_entries,     \
-                     ctr, (unsigned long long)cap, sb, status)
+                     ctr, (unsigned long long)cap, sb, status, prefixed)
   if (sb == 4) {
     if (emit) DV_SR(uint32_t, true);
     else DV_SR(uint32_t, false);
