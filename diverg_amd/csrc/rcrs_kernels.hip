@@ -288,8 +288,8 @@ __device__ __forceinline__ Ivl2 ivl_join(const Ivl2 &a, const Ivl2 &b) {
   if (a.st == 2 || b.st == 2) return u;
   uint32_t L0 = a.lo0, H0 = a.hi0, L1 = a.lo1, H1 = a.hi1, L2 = b.lo0, H2 = b.hi0, L3 = b.lo1,
            H3 = b.hi1;
-  cswap_iv(L0, H0, L1, H1);
-  cswap_iv(L2, H2, L3, H3);
+  // each side's pieces are already ascending (an unused piece starts at
+  // EMPTY): merging the two sorted pairs takes three compare-swaps
   cswap_iv(L0, H0, L2, H2);
   cswap_iv(L1, H1, L3, H3);
   cswap_iv(L1, H1, L2, H2);
@@ -397,17 +397,40 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     const uint2 *__restrict__ suf, uint32_t summ_min) {
   if (*status) return;  // an earlier step overflowed: operands are not valid
   unsigned long long nnz_acc = 0, slice_acc = 0;
-  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows;
-       base += (uint64_t)gridDim.x * BLOCK) {
-    const uint64_t row = base + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  // Software pipeline over this lane's rows: the A row bounds of the row two
+  // visits ahead and the first A range of the next row are loaded while the
+  // current row is folded, so a row's own chain is B row bounds -> B ranges.
+  uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t n_a0 = 0, n_a1 = 0, m_a0 = 0, m_a1 = 0;  // next row, the one after
+  uint2 n_first = make_uint2(1u, 0u);
+  if (row < nrows) {
+    n_a0 = A.rm[row];
+    n_a1 = A.rm[row + 1];
+  }
+  if (row + stride < nrows) {
+    m_a0 = A.rm[row + stride];
+    m_a1 = A.rm[row + stride + 1];
+  }
+  if (n_a0 < n_a1) n_first = ld_range(A.en, n_a0);
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows; base += stride) {
+    row = base + threadIdx.x;
+    const uint64_t a0 = n_a0, a1 = n_a1;
+    const uint2 first = n_first;
+    n_a0 = m_a0;
+    n_a1 = m_a1;
+    n_first = make_uint2(1u, 0u);
+    if (n_a0 < n_a1) n_first = ld_range(A.en, n_a0);
+    m_a0 = m_a1 = 0;
+    if (row + 2 * stride < nrows) {
+      m_a0 = A.rm[row + 2 * stride];
+      m_a1 = A.rm[row + 2 * stride + 1];
+    }
     int cls = 0;  // 0 done here (or no row), 1 -> medium groups, 2 -> 64-lane groups
     bool ovf = false;
     if (row < nrows) {
-      const uint64_t a0 = A.rm[row], a1 = A.rm[row + 1];
-      uint64_t len = 0;
-      for (uint64_t ai = a0; ai < a1; ai += 2)
-        len += ((uint64_t)B.rm[(uint64_t)A.en[ai + 1] + 1] - B.rm[A.en[ai]]) >> 1;
-      bool ok = len <= lane_max;
+      uint64_t len = 0;  // B-slice ranges of the row, the tier choice
+      bool ok = true;
       Runs r;
 #pragma unroll
       for (int t = 0; t < LANE_SLOTS; ++t) {
@@ -418,22 +441,27 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
       // common case in banded slices); any other range moves it into the
       // slots and starts a new running interval.
       uint32_t cl = 0xFFFFFFFFu, ch = 0u;
-      for (uint64_t ai = a0; ok && ai < a1; ai += 2) {
-        const uint64_t s = A.en[ai], e = A.en[ai + 1];
+      for (uint64_t ai = a0; ai < a1; ai += 2) {
+        const uint2 ar = ai == a0 ? first : ld_range(A.en, ai);
+        const uint64_t s = ar.x, e = ar.y;
+        const uint64_t bs = B.rm[s], be = B.rm[e + 1];
+        len += (be - bs) >> 1;
+        if (len > lane_max) ok = false;
+        if (!ok) continue;  // only the slice size is still needed
         if (pre && (s / SUMM_ROWS) != (e / SUMM_ROWS) && e - s + 1 >= summ_min) {
           // rows [s, h1) by the suffix summary of s, whole blocks by their
           // prefix summary at the block's last row, rows [t0, e] by the
           // prefix summary of e
           const uint64_t h1 = (s + SUMM_ROWS - 1) / SUMM_ROWS * SUMM_ROWS;
           const uint64_t t0 = (e + 1) / SUMM_ROWS * SUMM_ROWS;
-          if (s < h1) fold_summary(suf[s], B.en, B.rm[s], B.rm[h1], cl, ch, r, ok, slice_acc);
+          if (s < h1) fold_summary(suf[s], B.en, bs, B.rm[h1], cl, ch, r, ok, slice_acc);
           for (uint64_t bb = h1; ok && bb < t0; bb += SUMM_ROWS)
             fold_summary(pre[bb + SUMM_ROWS - 1], B.en, B.rm[bb], B.rm[bb + SUMM_ROWS], cl, ch, r,
                          ok, slice_acc);
           if (ok && t0 <= e)
-            fold_summary(pre[e], B.en, B.rm[t0], B.rm[e + 1], cl, ch, r, ok, slice_acc);
+            fold_summary(pre[e], B.en, B.rm[t0], be, cl, ch, r, ok, slice_acc);
         } else {
-          fold_segment(B.en, B.rm[s], B.rm[e + 1], cl, ch, r, ok, slice_acc);
+          fold_segment(B.en, bs, be, cl, ch, r, ok, slice_acc);
         }
       }
       if (ok && cl <= ch && !insert(r, cl, ch)) {
