@@ -35,7 +35,7 @@ DView view(const DevMat &m);
 // Scratch of one SpAdd / SpGEMM (carved from the context's arena).
 struct OpScratch {
   uint32_t *cnt = nullptr;    // per-row entry counts
-  uint32_t *stage = nullptr;  // staged entries, column-major
+  uint32_t *stage = nullptr;  // staged entries: range k of row r at uint2 k * n + r
   uint32_t *ovf = nullptr;    // rows whose output exceeds the stage
   uint32_t *mid = nullptr;    // rows for the medium groups
   uint32_t *big = nullptr;    // rows for the 64-lane groups

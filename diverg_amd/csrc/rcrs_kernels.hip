@@ -27,8 +27,9 @@
 // fix-up (range_sparse.hpp:1435-1440) is needed.  Windows wider than the LDS
 // budget are processed in chunks, carrying the top bit across chunks.
 //
-// The counting pass keeps the first `stage` entries of each row (column-major
-// staging, so lane-tier stores coalesce); after the row-pointer scan they are
+// The counting pass keeps the first `stage` entries of each row (range k of
+// row r as one uint2 at stage[k * nrows + r], so lane-tier stores and the
+// later copy coalesce across rows); after the row-pointer scan they are
 // copied into place and only rows with longer output are recomputed, in
 // direct-write mode.
 #include <hip/hip_runtime.h>
@@ -477,8 +478,9 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
         for (int t = 0; t < LANE_SLOTS; ++t) {
           if (r.lo[t] <= r.hi[t]) {
             nnz_acc += (unsigned long long)(r.hi[t] - r.lo[t]) + 1;
-            if (idx < KS) stage[(uint64_t)idx * nrows + row] = r.lo[t];
-            if (idx + 1 < KS) stage[(uint64_t)(idx + 1) * nrows + row] = r.hi[t];
+            if (idx < KS)  // KS is even: the whole range fits
+              reinterpret_cast<uint2 *>(stage)[(uint64_t)(idx >> 1) * nrows + row] =
+                  make_uint2(r.lo[t], r.hi[t]);
             idx += 2;
           }
         }
@@ -610,7 +612,7 @@ __device__ __forceinline__ void spgemm_row(uint64_t row, Mat<S> A, Mat<S> B, uin
           if (DIRECT)
             c_en[c_base + idx] = val;
           else if (idx < KS)
-            stage[(uint64_t)idx * nstage + row] = val;
+            stage[((uint64_t)(idx >> 1) * nstage + row) * 2 + (idx & 1u)] = val;
           ++idx;
         }
       }
@@ -669,7 +671,8 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_group(
   if (!DIRECT) block_add_u64(nnz_acc, &ctr->nnz);
 }
 
-// Copy staged rows (count <= KS) into C; stage is column-major.
+// Copy staged rows (count <= KS) into C; stage holds range k of row r at
+// uint2 index k * nrows + r.
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_spgemm_emit_staged(
     uint64_t nrows, const S *__restrict__ c_rm, const uint32_t *__restrict__ stage,
@@ -680,7 +683,9 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_emit_staged(
     const uint64_t b = c_rm[row], e = c_rm[row + 1];
     const uint64_t c = e - b;
     if (c <= KS)
-      for (uint64_t k = 0; k < c; ++k) c_en[b + k] = stage[k * nrows + row];
+      for (uint64_t k = 0; k < c; k += 2)
+        *reinterpret_cast<uint2 *>(c_en + b + k) =
+            reinterpret_cast<const uint2 *>(stage)[(k >> 1) * nrows + row];
   }
 }
 
@@ -719,8 +724,8 @@ __device__ unsigned long long block_scan_incl(unsigned long long v,
 // row pointers go through LDS so that global accesses are coalesced; the
 // per-thread run of SCAN_ITEMS consecutive rows is read from a padded LDS
 // layout (one pad word per 16) without bank conflicts.
-// EMIT: rows whose count fits the stage are copied into C here (stage is
-// column-major, so the copy walks rows across lanes); a tile whose entries
+// EMIT: rows whose count fits the stage are copied into C here (one uint2
+// per range, rows across lanes); a tile whose entries
 // would pass `cap` writes none of them (the total then flags bit 0).
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
@@ -797,9 +802,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(
     uint32_t *__restrict__ c_en, OpCounters *__restrict__ ctr, unsigned long long cap,
     uint32_t sb, unsigned int *__restrict__ status, int prefixed) {
   __shared__ uint32_t s_cnt[SCAN_PAD];
-  __shared__ unsigned long long s_ex[SCAN_PAD];
+  __shared__ uint32_t s_rel[SCAN_PAD];  // exclusive offsets within the tile
   const uint64_t tile = blockIdx.x;
   const uint64_t t0 = tile * SCAN_TILE;
+  const uint2 *stage2 = reinterpret_cast<const uint2 *>(stage);
   // coalesced load of the tile's counts
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
@@ -828,12 +834,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(
   }
   unsigned long long agg;
   const unsigned long long incl = block_scan_incl(tsum_t, agg);
-  unsigned long long run = texcl + incl - tsum_t;
-#pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    s_ex[scan_pad(r0 + k)] = run;
-    run += v[k];
-  }
+  const unsigned long long lstart = incl - tsum_t;  // within the tile
   if (tile + 1 == gridDim.x && threadIdx.x == 0) {
     const unsigned long long total = texcl + agg;
     ctr->total = total;
@@ -841,8 +842,30 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(
     if (sb == 4 && total > 0xFFFFFFFFull) atomicOr(status, 2u);
   }
   if (tile == 0 && threadIdx.x == 0) rm[0] = 0;
-  __syncthreads();
   const bool fits = texcl + agg <= cap && (sb == 8 || texcl + agg <= 0xFFFFFFFFull);
+  if (agg > 0xFFFFFFFFull) {
+    // a tile of more than 2^32 entries: write from the per-thread runs
+    unsigned long long run = texcl + lstart;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+      const uint64_t row = t0 + r0 + k;
+      if (row < n) {
+        if (EMIT && fits && v[k] <= KS)
+          for (uint32_t q = 0; q < v[k]; q += 2)
+            *reinterpret_cast<uint2 *>(c_en + run + q) = stage2[(uint64_t)(q >> 1) * n + row];
+        run += v[k];
+        rm[row + 1] = (S)run;
+      }
+    }
+    return;
+  }
+  uint32_t rel = (uint32_t)lstart;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    s_rel[scan_pad(r0 + k)] = rel;
+    rel += v[k];
+  }
+  __syncthreads();
   // coalesced stores of the row pointers (and the staged entries)
 #pragma unroll 4
   for (int k = 0; k < SCAN_ITEMS; ++k) {
@@ -850,10 +873,11 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(
     const uint64_t row = t0 + i;
     if (row < n) {
       const uint32_t c = s_cnt[scan_pad(i)];
-      const unsigned long long ex = s_ex[scan_pad(i)];
+      const unsigned long long ex = texcl + s_rel[scan_pad(i)];
       rm[row + 1] = (S)(ex + c);
       if (EMIT && fits && c <= KS)
-        for (uint32_t q = 0; q < c; ++q) c_en[ex + q] = stage[(uint64_t)q * n + row];
+        for (uint32_t q = 0; q < c; q += 2)
+          *reinterpret_cast<uint2 *>(c_en + ex + q) = stage2[(uint64_t)(q >> 1) * n + row];
     }
   }
 }

@@ -56,7 +56,7 @@ void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *en
 // Row-pointer scan: rowmap[0] = 0, rowmap[i+1] = cnt[0] + ... + cnt[i];
 // total -> ctr->total; a total above `cap` sets bit 0 of *status, above a
 // 32-bit rowmap bit 1.  Two launches (tile sums, then rows) and, when
-// c_entries is given, the copy of staged rows into C (column-major stage of
+// c_entries is given, the copy of staged rows into C (stage of
 // stage_entries per row).  tile_sums needs scan_words(n) words.
 uint64_t scan_words(uint64_t n);
 void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
@@ -77,7 +77,7 @@ void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t max_entries,
 // finish (more than lane_max slice ranges, or a union of more than 4 runs)
 // are appended to mid_list (slices of at most mid_max ranges) or big_list.
 // Writes per-row entry counts, the first `stage` entries of each row into
-// stage_buf (column-major, stride nrows), rows with more output to ovf_list,
+// stage_buf (range k of row r at uint2 k * nrows + r), rows with more output to ovf_list,
 // and adds set cells to ctr->nnz.
 void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, uint32_t lane_max,
                  uint32_t mid_max,
