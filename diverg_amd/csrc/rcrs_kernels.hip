@@ -158,6 +158,7 @@ __device__ void block_add_u64(unsigned long long v, unsigned long long *dst) {
 // is handed to the group tiers.
 constexpr int LANE_SLOTS = 4;
 constexpr int LANE_BATCH = 8;  // slice loads issued together per lane
+constexpr int A_BATCH = 2;     // A ranges whose B row bounds load together
 
 struct Runs {
   uint32_t lo[LANE_SLOTS], hi[LANE_SLOTS];  // empty slot: lo = 0xFFFFFFFF, hi = 0
@@ -375,16 +376,56 @@ __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t n
   }
 }
 
-// Fold the summary of rows [r0, r1): its interval, nothing when empty, the
-// rows' ranges when unknown.
-__device__ __forceinline__ void fold_summary(uint2 sm, const uint32_t *__restrict__ en,
-                                             uint64_t p0, uint64_t p1, uint32_t &cl, uint32_t &ch,
-                                             Runs &r, bool &ok, unsigned long long &gathered) {
-  if (sm.x <= sm.y) {
-    fold(sm.x, sm.y, cl, ch, r, ok);
-    gathered += 1;
-  } else if (sm.x != SUMM_EMPTY) {
-    fold_segment(en, p0, p1, cl, ch, r, ok, gathered);
+// Fold the B rows [s, e] of an A range that crosses a block boundary through
+// the summaries: rows [s, h1) by the suffix summary of s, each whole block by
+// the prefix summary of its last row, rows [t0, e] by the prefix summary of e.
+// The summaries are loaded SUMM_BATCH at a time before any is used, so a wide
+// range costs a few round trips, not one per block.  An unknown summary falls
+// back to that piece's ranges.
+constexpr int SUMM_BATCH = 8;
+
+template <typename S>
+__device__ __forceinline__ void fold_summarised(const Mat<S> &B, const uint2 *__restrict__ pre,
+                                                const uint2 *__restrict__ suf, uint64_t s,
+                                                uint64_t e, uint64_t bs, uint64_t be,
+                                                uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
+                                                unsigned long long &gathered) {
+  const uint64_t h1 = (s + SUMM_ROWS - 1) / SUMM_ROWS * SUMM_ROWS;
+  const uint64_t t0 = (e + 1) / SUMM_ROWS * SUMM_ROWS;
+  const uint64_t head = s < h1 ? 1 : 0, tail = t0 <= e ? 1 : 0;
+  const uint64_t nitems = head + (t0 - h1) / SUMM_ROWS + tail;
+  for (uint64_t j0 = 0; ok && j0 < nitems; j0 += SUMM_BATCH) {
+    uint2 sm[SUMM_BATCH];
+#pragma unroll
+    for (int q = 0; q < SUMM_BATCH; ++q) {
+      const uint64_t j = j0 + q;
+      sm[q] = make_uint2(SUMM_EMPTY, 0u);
+      if (j < nitems) {
+        if (j < head) sm[q] = suf[s];
+        else if (tail && j + 1 == nitems) sm[q] = pre[e];
+        else sm[q] = pre[h1 + (j - head) * SUMM_ROWS + SUMM_ROWS - 1];
+      }
+    }
+    uint32_t unknown = 0;
+#pragma unroll
+    for (int q = 0; q < SUMM_BATCH; ++q) {
+      if (j0 + q >= nitems) continue;
+      if (sm[q].x <= sm[q].y) {
+        fold(sm[q].x, sm[q].y, cl, ch, r, ok);
+        gathered += 1;
+      } else if (sm[q].x != SUMM_EMPTY) {
+        unknown |= 1u << q;
+      }
+    }
+    // unknown pieces: their own ranges
+    while (ok && unknown) {
+      const uint64_t j = j0 + (uint64_t)__builtin_ctz(unknown);
+      unknown &= unknown - 1;
+      const uint64_t r0 = j < head ? s : (tail && j + 1 == nitems) ? t0 : h1 + (j - head) * SUMM_ROWS;
+      const uint64_t r1 = j < head ? h1 : (tail && j + 1 == nitems) ? e + 1 : r0 + SUMM_ROWS;
+      fold_segment(B.en, r0 == s ? bs : (uint64_t)B.rm[r0], r1 == e + 1 ? be : (uint64_t)B.rm[r1],
+                   cl, ch, r, ok, gathered);
+    }
   }
 }
 
@@ -442,25 +483,43 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
       // common case in banded slices); any other range moves it into the
       // slots and starts a new running interval.
       uint32_t cl = 0xFFFFFFFFu, ch = 0u;
+      // the first A_BATCH A ranges and their B row bounds are loaded together
+      uint2 arq[A_BATCH];
+      S bsq[A_BATCH], beq[A_BATCH];
+#pragma unroll
+      for (int q = 0; q < A_BATCH; ++q) {
+        const uint64_t ai = a0 + 2 * q;
+        arq[q] = q == 0 ? first : ai < a1 ? ld_range(A.en, ai) : make_uint2(0u, 0u);
+      }
+#pragma unroll
+      for (int q = 0; q < A_BATCH; ++q) {
+        const bool live = a0 + 2 * q < a1;
+        bsq[q] = live ? B.rm[arq[q].x] : (S)0;
+        beq[q] = live ? B.rm[(uint64_t)arq[q].y + 1] : (S)0;
+      }
       for (uint64_t ai = a0; ai < a1; ai += 2) {
-        const uint2 ar = ai == a0 ? first : ld_range(A.en, ai);
+        const uint64_t q = (ai - a0) >> 1;
+        uint2 ar;
+        uint64_t bs, be;
+        if (q < A_BATCH) {
+#pragma unroll
+          for (int t = 0; t < A_BATCH; ++t)
+            if (t == (int)q) {
+              ar = arq[t];
+              bs = bsq[t];
+              be = beq[t];
+            }
+        } else {
+          ar = ld_range(A.en, ai);
+          bs = B.rm[ar.x];
+          be = B.rm[(uint64_t)ar.y + 1];
+        }
         const uint64_t s = ar.x, e = ar.y;
-        const uint64_t bs = B.rm[s], be = B.rm[e + 1];
         len += (be - bs) >> 1;
         if (len > lane_max) ok = false;
         if (!ok) continue;  // only the slice size is still needed
         if (pre && (s / SUMM_ROWS) != (e / SUMM_ROWS) && e - s + 1 >= summ_min) {
-          // rows [s, h1) by the suffix summary of s, whole blocks by their
-          // prefix summary at the block's last row, rows [t0, e] by the
-          // prefix summary of e
-          const uint64_t h1 = (s + SUMM_ROWS - 1) / SUMM_ROWS * SUMM_ROWS;
-          const uint64_t t0 = (e + 1) / SUMM_ROWS * SUMM_ROWS;
-          if (s < h1) fold_summary(suf[s], B.en, bs, B.rm[h1], cl, ch, r, ok, slice_acc);
-          for (uint64_t bb = h1; ok && bb < t0; bb += SUMM_ROWS)
-            fold_summary(pre[bb + SUMM_ROWS - 1], B.en, B.rm[bb], B.rm[bb + SUMM_ROWS], cl, ch, r,
-                         ok, slice_acc);
-          if (ok && t0 <= e)
-            fold_summary(pre[e], B.en, B.rm[t0], be, cl, ch, r, ok, slice_acc);
+          fold_summarised(B, pre, suf, s, e, bs, be, cl, ch, r, ok, slice_acc);
         } else {
           fold_segment(B.en, bs, be, cl, ch, r, ok, slice_acc);
         }
