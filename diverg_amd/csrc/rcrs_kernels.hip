@@ -429,7 +429,7 @@ __device__ __forceinline__ void fold_summarised(const Mat<S> &B, const uint2 *__
   }
 }
 
-template <typename S>
+template <typename S, bool SUMM>
 __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     Mat<S> A, Mat<S> B, uint64_t nrows, uint32_t KS, uint32_t lane_max, uint32_t mid_max,
     uint32_t *__restrict__ cnt, uint32_t *__restrict__ stage,
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
         len += (be - bs) >> 1;
         if (len > lane_max) ok = false;
         if (!ok) continue;  // only the slice size is still needed
-        if (pre && (s / SUMM_ROWS) != (e / SUMM_ROWS) && e - s + 1 >= summ_min) {
+        if (SUMM && (s / SUMM_ROWS) != (e / SUMM_ROWS) && e - s + 1 >= summ_min) {
           fold_summarised(B, pre, suf, s, e, bs, be, cl, ch, r, ok, slice_acc);
         } else {
           fold_segment(B.en, bs, be, cl, ch, r, ok, slice_acc);
@@ -1349,14 +1349,20 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
   if (!a.nrows) return;
   const uint2 *pre = static_cast<const uint2 *>(summ);
   const uint2 *suf = pre ? pre + b.nrows : nullptr;
-  if (sb == 4)
-    hipLaunchKernelGGL(k_spgemm_lane<uint32_t>, dim3(grid), dim3(BLOCK), 0, s, mat<uint32_t>(a),
-                       mat<uint32_t>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list,
-                       mid_list, big_list, ctr, status, pre, suf, summ_min);
-  else
-    hipLaunchKernelGGL(k_spgemm_lane<uint64_t>, dim3(grid), dim3(BLOCK), 0, s, mat<uint64_t>(a),
-                       mat<uint64_t>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list,
-                       mid_list, big_list, ctr, status, pre, suf, summ_min);
+  // without summaries the kernel is built without the summary walk (fewer
+  // registers, more waves per SIMD)
+#define DV_LANE(S_, M_)                                                                        \
+  hipLaunchKernelGGL((k_spgemm_lane<S_, M_>), dim3(grid), dim3(BLOCK), 0, s, mat<S_>(a),       \
+                     mat<S_>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list, \
+                     mid_list, big_list, ctr, status, pre, suf, summ_min)
+  if (sb == 4) {
+    if (pre) DV_LANE(uint32_t, true);
+    else DV_LANE(uint32_t, false);
+  } else {
+    if (pre) DV_LANE(uint64_t, true);
+    else DV_LANE(uint64_t, false);
+  }
+#undef DV_LANE
 }
 
 uint64_t summary_words(uint64_t nrows) { return 2 * nrows; }
