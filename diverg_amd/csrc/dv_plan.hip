@@ -496,7 +496,7 @@ int Ctx::autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning 
       {&dv_tuning::lanes_small, {8, 16, 32}},
       {&dv_tuning::small_slice_max, {32, 64, 128, 256}},
       {&dv_tuning::lane_slice_max, {1024, 4096, 16384}},
-      {&dv_tuning::summary_span, {0, 16, 64}},
+      {&dv_tuning::summary_span, {0, 16, 32, 64}},
       {&dv_tuning::lds_words, {lds_max / 4, lds_max / 2, lds_max}},
       {&dv_tuning::rows_per_group, {1, 4, 16}},
   };
