@@ -70,6 +70,7 @@ class dv_tuning(C.Structure):
         ("plan_mode", C.c_uint32),
         ("graphs", C.c_uint32),
         ("summary_span", C.c_uint32),
+        ("schedule", C.c_uint32),
     ]
 
 

@@ -87,14 +87,42 @@ int Plan::power(int p, uint32_t k) {
   return c;
 }
 
-// dindex.hpp:46-189.
+// SpGEMMs that power(p, k) plans: floor(log2 k) squarings, one product per
+// further set bit, and I.P first when bit 0 is set and P is not an SpGEMM
+// output.
+static uint32_t power_spgemms(uint32_t k, bool p_is_product) {
+  if (k == 0) return 0;
+  uint32_t sq = 0;
+  while ((k >> (sq + 1)) != 0) ++sq;
+  return sq + (uint32_t)__builtin_popcount(k) - 1 + ((k & 1) && !p_is_product ? 1 : 0);
+}
+
+// D = A^d1 . (A + I)^(d2 - d1) (dindex.hpp:46-189), or (A + I)^d2 when d1 = 0.
+// A and A + I commute, so with m = min(d1, d2 - d1) the same matrix is also
+// (A (A + I))^m times A^(d1 - m) or (A + I)^(d2 - d1 - m): M = A (A + I) holds
+// the walks of length 1 or 2, M^m those of length m .. 2m.  Boolean products
+// and canonical range rows are unique, so both orders give the same arrays;
+// the plan takes the one with fewer SpGEMMs (for d2 = 2 d1, as [300, 600],
+// M^300: 12 SpGEMMs instead of 23).
 int Plan::dindex(int a, uint32_t dlo, uint32_t dup) {
   const uint64_t n = slots[a].nrows;
   const uint32_t sb = slots[a].sb;
   const int rai = spadd(identity(n, sb), a);
-  const int rad = dlo != 0 ? power(a, dlo) : -1;
-  const int raid = power(rai, dup - dlo);
-  if (dlo == 0) return raid;
+  if (dlo == 0) return power(rai, dup);
+  const uint32_t b = dup - dlo;
+  const uint32_t classic = power_spgemms(dlo, false) + power_spgemms(b, false) + 1;
+  const uint32_t m = std::min(dlo, b);
+  const uint32_t rest = std::max(dlo, b) - m;
+  const uint32_t paired =
+      m == 0 ? ~0u : 1 + power_spgemms(m, true) + (rest ? power_spgemms(rest, false) + 1 : 0);
+  if (paired < classic && schedule != 1) {
+    const int mm = spgemm(a, rai);
+    const int mp = power(mm, m);
+    if (!rest) return mp;
+    return spgemm(mp, dlo > b ? power(a, rest) : power(rai, rest));
+  }
+  const int rad = power(a, dlo);
+  const int raid = power(rai, b);
   return spgemm(rad, raid);
 }
 
@@ -443,6 +471,7 @@ bool Ctx::plan_events_reserve(int nops) {
 
 int Ctx::dindex_planned(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   Plan p;
+  p.schedule = tuning.schedule;
   p.out = p.dindex(p.external(a), dlo, dup);
   return plan_run(p, d, [&](DevMat &o) { return dindex_exact(a, dlo, dup, o); });
 }

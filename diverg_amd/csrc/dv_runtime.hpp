@@ -95,6 +95,7 @@ struct Plan {
   int spgemm(int a, int b);
   int power(int p, uint32_t k);
   int dindex(int a, uint32_t dlo, uint32_t dup);
+  uint32_t schedule = 0;  // 0: fewer SpGEMMs; 1: the reference's order
 };
 
 // A rowmap + entries buffer pair kept by the context across builds.
@@ -111,7 +112,7 @@ struct Ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t num_cu = 256;
   uint32_t lds_per_block = 65536;
-  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0, 0, 32};
+  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0, 0, 32, 0};
   std::string last_error;
 
   // pool accounting

@@ -132,7 +132,12 @@ void *dv_stream(dv_context *ctx);
  *                   rows from its block's start / to its block's end when it
  *                   is one interval); a left range of at least this many rows
  *                   that crosses a block boundary is then read as one suffix,
- *                   whole blocks and one prefix instead of row by row */
+ *                   whole blocks and one prefix instead of row by row
+ *   schedule        0: a planned index build takes whichever of
+ *                   A^d1 (A + I)^(d2 - d1) and (A (A + I))^m . rest
+ *                   (m = min(d1, d2 - d1)) needs fewer SpGEMMs -- the same
+ *                   index, as the factors commute; 1: always the reference's
+ *                   order (dindex.hpp:131-177) */
 typedef struct dv_tuning {
   uint32_t lds_words;
   uint32_t stage_entries;
@@ -144,6 +149,7 @@ typedef struct dv_tuning {
   uint32_t plan_mode;
   uint32_t graphs;
   uint32_t summary_span;
+  uint32_t schedule;
 } dv_tuning;
 int dv_get_tuning(const dv_context *ctx, dv_tuning *t);
 int dv_set_tuning(dv_context *ctx, const dv_tuning *t);

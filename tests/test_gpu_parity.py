@@ -165,6 +165,20 @@ def test_distance_index_many_scan_tiles(ctx):
     assert_identical(ctx.create_distance_index(a, 5, 24), orc.dindex(a, 5, 24), "D[5,24] 9e6 bp")
 
 
+@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("dlo,dup", [(300, 600), (40, 300), (200, 210), (1, 8), (3, 4)])
+def test_distance_index_schedules(ctx, schedule, dlo, dup):
+    """The paired schedule (A (A + I))^m . rest and the reference's order
+    A^d1 (A + I)^(d2 - d1) give the same index."""
+    a = dv.generate_graph(2, backbone_bp=40000, seed=33, size_bytes=4)
+    ctx.set_tuning(schedule=schedule)
+    print(f"dindex schedule={schedule} [{dlo},{dup}]", flush=True)
+    d, st = ctx.create_distance_index(a, dlo, dup, stats=True)
+    assert_identical(d, orc.dindex(a, dlo, dup), f"D[{dlo},{dup}] schedule {schedule}")
+    if schedule == 0 and (dlo, dup) == (300, 600):
+        assert st["n_spgemm"] == 12  # M = A (A + I), then M^300
+
+
 @pytest.mark.parametrize("size_bytes", [4, 8])
 @pytest.mark.parametrize("dlo,dup", [(150, 450), (0, 250), (7, 7), (1, 8), (5, 5), (0, 1)])
 @pytest.mark.parametrize("plan_mode", [0, 1, 2])
