@@ -269,10 +269,16 @@ __device__ __forceinline__ void fold_segment(const uint32_t *__restrict__ en, ui
 constexpr int SUMM_ROWS = 32;
 constexpr uint32_t SUMM_EMPTY = 0xFFFFFFFFu;
 
+// A partial union: up to two pieces in ascending order, an unused piece is
+// (EMPTY, 0); the empty union has lo0 = EMPTY, an unknown one is (1, 0).
 struct Ivl2 {
-  uint32_t lo0, hi0, lo1, hi1;  // pieces in ascending order; an unused piece is (EMPTY, 0)
-  uint32_t st;                  // 0 empty, 1 the pieces, 2 unknown
+  uint32_t lo0, hi0, lo1, hi1;
 };
+
+__device__ __forceinline__ bool ivl_empty(const Ivl2 &v) { return v.lo0 == SUMM_EMPTY; }
+__device__ __forceinline__ bool ivl_unknown(const Ivl2 &v) {
+  return v.lo0 != SUMM_EMPTY && v.lo0 > v.hi0;
+}
 
 __device__ __forceinline__ void cswap_iv(uint32_t &la, uint32_t &ha, uint32_t &lb, uint32_t &hb) {
   const bool sw = lb < la;
@@ -284,10 +290,13 @@ __device__ __forceinline__ void cswap_iv(uint32_t &la, uint32_t &ha, uint32_t &l
 }
 
 __device__ __forceinline__ Ivl2 ivl_join(const Ivl2 &a, const Ivl2 &b) {
-  if (a.st == 0) return b;
-  if (b.st == 0) return a;
-  Ivl2 u{SUMM_EMPTY, 0u, SUMM_EMPTY, 0u, 2u};
-  if (a.st == 2 || b.st == 2) return u;
+  if (ivl_empty(a)) return b;
+  if (ivl_empty(b)) return a;
+  const Ivl2 u{1u, 0u, SUMM_EMPTY, 0u};
+  if (ivl_unknown(a) || ivl_unknown(b)) return u;
+  // the common case: two single intervals that touch
+  if (a.lo1 == SUMM_EMPTY && b.lo1 == SUMM_EMPTY && b.lo0 <= a.hi0 + 1u && b.hi0 + 1u >= a.lo0)
+    return Ivl2{min(a.lo0, b.lo0), max(a.hi0, b.hi0), SUMM_EMPTY, 0u};
   uint32_t L0 = a.lo0, H0 = a.hi0, L1 = a.lo1, H1 = a.hi1, L2 = b.lo0, H2 = b.hi0, L3 = b.lo1,
            H3 = b.hi1;
   // each side's pieces are already ascending (an unused piece starts at
@@ -323,24 +332,22 @@ __device__ __forceinline__ Ivl2 ivl_join(const Ivl2 &a, const Ivl2 &b) {
   put(L2, H2);
   put(L3, H3);
   if (over) return u;
-  return Ivl2{l0, h0, l1, h1, 1u};
+  return Ivl2{l0, h0, l1, h1};
 }
 
 __device__ __forceinline__ uint2 ivl_enc(const Ivl2 &v) {
-  if (v.st == 1 && v.lo1 > v.hi1) return make_uint2(v.lo0, v.hi0);
-  return v.st == 0 ? make_uint2(SUMM_EMPTY, 0u) : make_uint2(1u, 0u);
+  if (!ivl_empty(v) && !ivl_unknown(v) && v.lo1 == SUMM_EMPTY) return make_uint2(v.lo0, v.hi0);
+  return ivl_empty(v) ? make_uint2(SUMM_EMPTY, 0u) : make_uint2(1u, 0u);
 }
 
 __device__ __forceinline__ Ivl2 ivl_shfl_up(const Ivl2 &v, int d) {
   return Ivl2{__shfl_up(v.lo0, d, SUMM_ROWS), __shfl_up(v.hi0, d, SUMM_ROWS),
-              __shfl_up(v.lo1, d, SUMM_ROWS), __shfl_up(v.hi1, d, SUMM_ROWS),
-              __shfl_up(v.st, d, SUMM_ROWS)};
+              __shfl_up(v.lo1, d, SUMM_ROWS), __shfl_up(v.hi1, d, SUMM_ROWS)};
 }
 
 __device__ __forceinline__ Ivl2 ivl_shfl_down(const Ivl2 &v, int d) {
   return Ivl2{__shfl_down(v.lo0, d, SUMM_ROWS), __shfl_down(v.hi0, d, SUMM_ROWS),
-              __shfl_down(v.lo1, d, SUMM_ROWS), __shfl_down(v.hi1, d, SUMM_ROWS),
-              __shfl_down(v.st, d, SUMM_ROWS)};
+              __shfl_down(v.lo1, d, SUMM_ROWS), __shfl_down(v.hi1, d, SUMM_ROWS)};
 }
 
 template <typename S>
@@ -353,12 +360,12 @@ __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t n
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows;
        base += (uint64_t)gridDim.x * BLOCK) {
     const uint64_t r = base + threadIdx.x;
-    Ivl2 v{SUMM_EMPTY, 0u, SUMM_EMPTY, 0u, 0u};
+    Ivl2 v{SUMM_EMPTY, 0u, SUMM_EMPTY, 0u};
     if (r < nrows) {
       const uint64_t p1 = B.rm[r + 1];
-      for (uint64_t p = B.rm[r]; p < p1 && v.st != 2; p += 2) {
+      for (uint64_t p = B.rm[r]; p < p1 && !ivl_unknown(v); p += 2) {
         const uint2 x = ld_range(B.en, p);
-        v = ivl_join(v, Ivl2{x.x, x.y, SUMM_EMPTY, 0u, 1u});
+        v = ivl_join(v, Ivl2{x.x, x.y, SUMM_EMPTY, 0u});
       }
     }
     Ivl2 f = v, g = v;
