@@ -159,7 +159,6 @@ __device__ void block_add_u64(unsigned long long v, unsigned long long *dst) {
 constexpr int LANE_SLOTS = 4;
 constexpr int LANE_BATCH = 8;  // slice loads issued together per lane
 constexpr int A_BATCH = 2;     // A ranges whose B row bounds load together
-constexpr int WIN_RANGES = 256;  // per-wave LDS window of B ranges (2 KB)
 
 struct Runs {
   uint32_t lo[LANE_SLOTS], hi[LANE_SLOTS];  // empty slot: lo = 0xFFFFFFFF, hi = 0
@@ -254,28 +253,6 @@ __device__ __forceinline__ void fold_segment(const uint32_t *__restrict__ en, ui
 #pragma unroll
     for (int k = 0; k < LANE_BATCH; ++k) fold(v[k].x, v[k].y, cl, ch, r, ok);
   }
-}
-
-// Fold the ranges win[i0 .. i1) held in LDS.
-__device__ __forceinline__ void fold_window(const uint2 *win, uint32_t i0, uint32_t i1,
-                                            uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
-                                            unsigned long long &gathered) {
-  gathered += i1 - i0;
-  for (uint32_t i = i0; ok && i < i1; ++i) {  // LDS latency is short: one at a time
-    const uint2 v = win[i];
-    fold(v.x, v.y, cl, ch, r, ok);
-  }
-}
-
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint64_t)__shfl_xor((unsigned long long)v, d, 64));
-  return v;
-}
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint64_t)__shfl_xor((unsigned long long)v, d, 64));
-  return v;
 }
 
 // Interval summaries of the right operand, per 32-row block k (rows
@@ -468,13 +445,6 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     const unsigned int *__restrict__ status, const uint2 *__restrict__ pre,
     const uint2 *__restrict__ suf, uint32_t summ_min) {
   if (*status) return;  // an earlier step overflowed: operands are not valid
-  // The 64 rows a wave takes at a time are consecutive, so in a banded
-  // product their first A ranges' B slices overlap: the wave loads the union
-  // of those slices once, coalesced, into an LDS window and the lanes fold
-  // from there (a slice that does not fit is read from global memory).
-  __shared__ uint2 s_win[BLOCK / 64][WIN_RANGES];
-  uint2 *win = s_win[threadIdx.x >> 6];
-  const int wl = threadIdx.x & 63;
   unsigned long long nnz_acc = 0, slice_acc = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   // Software pipeline over this lane's rows: the A row bounds of the row two
@@ -507,41 +477,6 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     }
     int cls = 0;  // 0 done here (or no row), 1 -> medium groups, 2 -> 64-lane groups
     bool ovf = false;
-    bool first_flat = false;
-    uint64_t wlo = 0, whi = 0;
-    uint2 arq[A_BATCH];
-    S bsq[A_BATCH], beq[A_BATCH];
-    if (row < nrows) {
-      // the first A_BATCH A ranges and their B row bounds are loaded together
-#pragma unroll
-      for (int q = 0; q < A_BATCH; ++q) {
-        const uint64_t ai = a0 + 2 * q;
-        arq[q] = q == 0 ? first : ai < a1 ? ld_range(A.en, ai) : make_uint2(0u, 0u);
-      }
-#pragma unroll
-      for (int q = 0; q < A_BATCH; ++q) {
-        const bool live = a0 + 2 * q < a1;
-        bsq[q] = live ? B.rm[arq[q].x] : (S)0;
-        beq[q] = live ? B.rm[(uint64_t)arq[q].y + 1] : (S)0;
-      }
-      // the first A range's slice goes through the wave's window unless it
-      // takes the summary walk
-      first_flat = a0 < a1 && !(SUMM && (arq[0].x / SUMM_ROWS) != (arq[0].y / SUMM_ROWS) &&
-                                (uint64_t)arq[0].y - arq[0].x + 1 >= summ_min);
-      if (first_flat) {
-        wlo = bsq[0];
-        whi = beq[0];
-      }
-    }
-    // (every lane of the wave gets here, with or without a row)
-    const uint64_t lo = wave_min_u64(first_flat ? wlo : ~0ull);
-    const uint64_t hi = wave_max_u64(first_flat ? whi : 0ull);
-    const bool use_win = hi > lo && hi - lo <= 2ull * WIN_RANGES;
-    if (use_win) {
-      wave_sync();  // the previous window is no longer read
-      for (uint64_t k = wl; 2 * k < hi - lo; k += 64) win[k] = ld_range(B.en, lo + 2 * k);
-      wave_sync();
-    }
     if (row < nrows) {
       uint64_t len = 0;  // B-slice ranges of the row, the tier choice
       bool ok = true;
@@ -555,6 +490,20 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
       // common case in banded slices); any other range moves it into the
       // slots and starts a new running interval.
       uint32_t cl = 0xFFFFFFFFu, ch = 0u;
+      // the first A_BATCH A ranges and their B row bounds are loaded together
+      uint2 arq[A_BATCH];
+      S bsq[A_BATCH], beq[A_BATCH];
+#pragma unroll
+      for (int q = 0; q < A_BATCH; ++q) {
+        const uint64_t ai = a0 + 2 * q;
+        arq[q] = q == 0 ? first : ai < a1 ? ld_range(A.en, ai) : make_uint2(0u, 0u);
+      }
+#pragma unroll
+      for (int q = 0; q < A_BATCH; ++q) {
+        const bool live = a0 + 2 * q < a1;
+        bsq[q] = live ? B.rm[arq[q].x] : (S)0;
+        beq[q] = live ? B.rm[(uint64_t)arq[q].y + 1] : (S)0;
+      }
       for (uint64_t ai = a0; ai < a1; ai += 2) {
         const uint64_t q = (ai - a0) >> 1;
         uint2 ar;
@@ -578,9 +527,6 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
         if (!ok) continue;  // only the slice size is still needed
         if (SUMM && (s / SUMM_ROWS) != (e / SUMM_ROWS) && e - s + 1 >= summ_min) {
           fold_summarised(B, pre, suf, s, e, bs, be, cl, ch, r, ok, slice_acc);
-        } else if (q == 0 && use_win) {  // first_flat holds here: [bs, be) is in the window
-          fold_window(win, (uint32_t)((bs - lo) >> 1), (uint32_t)((be - lo) >> 1), cl, ch, r, ok,
-                      slice_acc);
         } else {
           fold_segment(B.en, bs, be, cl, ch, r, ok, slice_acc);
         }
