@@ -451,7 +451,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
   // visits ahead and the first A range of the next row are loaded while the
   // current row is folded, so a row's own chain is B row bounds -> B ranges.
   uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-  uint64_t n_a0 = 0, n_a1 = 0, m_a0 = 0, m_a1 = 0;  // next row, the one after
+  S n_a0 = 0, n_a1 = 0, m_a0 = 0, m_a1 = 0;  // next row, the one after
   uint2 n_first = make_uint2(1u, 0u);
   if (row < nrows) {
     n_a0 = A.rm[row];
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
   if (n_a0 < n_a1) n_first = ld_range(A.en, n_a0);
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows; base += stride) {
     row = base + threadIdx.x;
-    const uint64_t a0 = n_a0, a1 = n_a1;
+    const uint64_t a0 = n_a0, a1 = n_a1;  // (S: the pipeline holds row bounds at their width)
     const uint2 first = n_first;
     n_a0 = m_a0;
     n_a1 = m_a1;
