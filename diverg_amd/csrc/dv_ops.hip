@@ -81,7 +81,7 @@ void Ctx::enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap
 void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                            const OpScratch &s, OpCounters *ctr, uint64_t cap,
                            unsigned int *status, hipEvent_t after_lane, uint32_t *c_entries,
-                           bool summaries) {
+                           bool summaries, bool plus_i) {
   const uint64_t n = a.nrows;
   // the summary buffer of the scratch holds summary_words(n + 1) words
   summaries = summaries && b.nrows <= n + 1 && tuning.summary_span >= 2;
@@ -93,9 +93,10 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)num_cu * 8));
     spgemm_lane(stream, a.sb, view(a), view(b), s.stage_entries, tuning.lane_slice_max,
                 tuning.small_slice_max, lane_grid, s.cnt, s.stage, s.ovf, s.mid, s.big, ctr,
-                status, summaries ? s.summ : nullptr, tuning.summary_span);
+                status, summaries ? s.summ : nullptr, tuning.summary_span, plus_i ? 1u : 0u);
     if (after_lane) (void)hipEventRecord(after_lane, stream);
     GroupLaunch g = group_launch(tuning.lanes_small, n);
+    g.plus_i = plus_i ? 1u : 0u;
     g.rows = s.mid;
     g.n_dev = &ctr->n_mid;
     g.cnt = s.cnt;
@@ -118,12 +119,13 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
 
 void Ctx::enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
                            uint32_t *c_entries, const OpScratch &s, OpCounters *ctr,
-                           const unsigned int *status, bool staged_copied) {
+                           const unsigned int *status, bool staged_copied, bool plus_i) {
   const uint64_t n = a.nrows;
   if (!n) return;
   if (!staged_copied)
     spgemm_emit_staged(stream, a.sb, n, c_rowmap, s.stage, s.stage_entries, c_entries, status);
   GroupLaunch g = group_launch(64, n);
+  g.plus_i = plus_i ? 1u : 0u;
   g.rows = s.ovf;
   g.n_dev = &ctr->overflow;
   g.c_rowmap = c_rowmap;

@@ -65,6 +65,13 @@ int Plan::spgemm(int a, int b) {
   return op(P_SPGEMM, a, b, slots[a].nrows, slots[b].ncols, slots[a].sb);
 }
 
+int Plan::spgemm_plus_i(int a, int b) {
+  const int c = op(P_SPGEMM, a, b, slots[a].nrows, slots[b].ncols, slots[a].sb);
+  ops.back().plus_i = true;
+  slots[c].span += 1;  // rows of b + I cover one more column than b's
+  return c;
+}
+
 // C = P^k by binary exponentiation (range_sparse.hpp:2376-2431).
 int Plan::power(int p, uint32_t k) {
   const uint64_t n = slots[p].ncols;
@@ -107,8 +114,12 @@ static uint32_t power_spgemms(uint32_t k, bool p_is_product) {
 int Plan::dindex(int a, uint32_t dlo, uint32_t dup) {
   const uint64_t n = slots[a].nrows;
   const uint32_t sb = slots[a].sb;
-  const int rai = spadd(identity(n, sb), a);
-  if (dlo == 0) return power(rai, dup);
+  int rai_slot = -1;
+  auto rai = [&]() {  // A + I, planned when first needed
+    if (rai_slot < 0) rai_slot = spadd(identity(n, sb), a);
+    return rai_slot;
+  };
+  if (dlo == 0) return power(rai(), dup);
   const uint32_t b = dup - dlo;
   const uint32_t classic = power_spgemms(dlo, false) + power_spgemms(b, false) + 1;
   const uint32_t m = std::min(dlo, b);
@@ -116,13 +127,15 @@ int Plan::dindex(int a, uint32_t dlo, uint32_t dup) {
   const uint32_t paired =
       m == 0 ? ~0u : 1 + power_spgemms(m, true) + (rest ? power_spgemms(rest, false) + 1 : 0);
   if (paired < classic && schedule != 1) {
-    const int mm = spgemm(a, rai);
+    // M = A (A + I) straight from A: the product kernels add each A range
+    // itself, so A + I is only planned when a rest factor needs it
+    const int mm = spgemm_plus_i(a, a);
     const int mp = power(mm, m);
     if (!rest) return mp;
-    return spgemm(mp, dlo > b ? power(a, rest) : power(rai, rest));
+    return spgemm(mp, dlo > b ? power(a, rest) : power(rai(), rest));
   }
   const int rad = power(a, dlo);
-  const int raid = power(rai, b);
+  const int raid = power(rai(), b);
   return spgemm(rad, raid);
 }
 
@@ -242,9 +255,9 @@ void Ctx::plan_launch(Plan &p, bool timing) {
       const bool summaries =
           tuning.summary_span && p.slots[o.a].span >= tuning.summary_span;
       enq_spgemm_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status,
-                       timing ? plan_ev[4 * i + 1] : nullptr, c.entries, summaries);
+                       timing ? plan_ev[4 * i + 1] : nullptr, c.entries, summaries, o.plus_i);
       if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
-      enq_spgemm_write(a, b, c.rowmap, c.entries, s, ctr, d_status, true);
+      enq_spgemm_write(a, b, c.rowmap, c.entries, s, ctr, d_status, true, o.plus_i);
     }
     if (timing) (void)hipEventRecord(plan_ev[4 * i + 3], stream);
   }
@@ -260,6 +273,7 @@ static uint64_t plan_key(const Plan &p, const dv_tuning &t, const void *ctr, con
   };
   for (const PlanOp &o : p.ops) {
     mix(o.kind);
+    mix(o.plus_i);
     mix((uint64_t)(int64_t)o.a);
     mix((uint64_t)(int64_t)o.b);
   }

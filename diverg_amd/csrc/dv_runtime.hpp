@@ -80,6 +80,7 @@ struct PlanSlot {
 struct PlanOp {
   PlanKind kind;
   int a, b, c;
+  bool plus_i = false;  // SpGEMM: c = a (b + I)
 };
 
 struct Plan {
@@ -93,6 +94,7 @@ struct Plan {
   int identity(uint64_t n, uint32_t sb);
   int spadd(int a, int b);
   int spgemm(int a, int b);
+  int spgemm_plus_i(int a, int b);  // a (b + I)
   int power(int p, uint32_t k);
   int dindex(int a, uint32_t dlo, uint32_t dup);
   uint32_t schedule = 0;  // 0: fewer SpGEMMs; 1: the reference's order
@@ -201,10 +203,11 @@ struct Ctx {
   void enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                         const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status,
                         hipEvent_t after_lane = nullptr, uint32_t *c_entries = nullptr,
-                        bool summaries = false);
+                        bool summaries = false, bool plus_i = false);
+  // plus_i: the product is a (b + I)
   void enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
                         uint32_t *c_entries, const OpScratch &s, OpCounters *ctr,
-                        const unsigned int *status, bool staged_copied);
+                        const unsigned int *status, bool staged_copied, bool plus_i = false);
   int identity_dev(uint64_t n, uint32_t sb, DevMat &out);
   int spadd_dev(const DevMat &a, const DevMat &b, DevMat &c);
   int spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c);

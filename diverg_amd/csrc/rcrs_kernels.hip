@@ -443,7 +443,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     uint32_t *__restrict__ ovf_list, uint32_t *__restrict__ mid_list,
     uint32_t *__restrict__ big_list, OpCounters *__restrict__ ctr,
     const unsigned int *__restrict__ status, const uint2 *__restrict__ pre,
-    const uint2 *__restrict__ suf, uint32_t summ_min) {
+    const uint2 *__restrict__ suf, uint32_t summ_min, uint32_t plus_i) {
   if (*status) return;  // an earlier step overflowed: operands are not valid
   unsigned long long nnz_acc = 0, slice_acc = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -525,6 +525,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
         len += (be - bs) >> 1;
         if (len > lane_max) ok = false;
         if (!ok) continue;  // only the slice size is still needed
+        if (plus_i) fold((uint32_t)s, (uint32_t)e, cl, ch, r, ok);  // C = A (B + I)
         if (SUMM && (s / SUMM_ROWS) != (e / SUMM_ROWS) && e - s + 1 >= summ_min) {
           fold_summarised(B, pre, suf, s, e, bs, be, cl, ch, r, ok, slice_acc);
         } else {
@@ -572,7 +573,7 @@ __device__ __forceinline__ void spgemm_row(uint64_t row, Mat<S> A, Mat<S> B, uin
                                            uint64_t nstage, uint32_t *cnt, uint32_t *stage,
                                            uint32_t *ovf_list, unsigned int *ovf_count,
                                            uint32_t *c_en, uint64_t c_base,
-                                           unsigned long long &nnz_acc) {
+                                           unsigned long long &nnz_acc, uint32_t plus_i) {
   const uint64_t a0 = A.rm[row], a1 = A.rm[row + 1];
 
   // 1. column extent.  A single-segment slice of at most CACHE * G ranges is
@@ -580,8 +581,10 @@ __device__ __forceinline__ void spgemm_row(uint64_t row, Mat<S> A, Mat<S> B, uin
   uint2 rc[CACHE];
   bool cached = false;
   uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+  uint2 a_only = make_uint2(1u, 0u);  // the single A range, for C = A (B + I)
   if (a1 - a0 == 2) {
     const uint32_t s = A.en[a0], e = A.en[a0 + 1];
+    if (plus_i) a_only = make_uint2(s, e);
     const uint64_t p0 = B.rm[s], p1 = B.rm[(uint64_t)e + 1];
     if (p1 - p0 <= 2ull * G * CACHE) {
       cached = true;
@@ -610,6 +613,11 @@ __device__ __forceinline__ void spgemm_row(uint64_t row, Mat<S> A, Mat<S> B, uin
       }
     }
   }
+  if (plus_i)  // the A ranges themselves are part of the row
+    for (uint64_t ai = a0; ai < a1; ai += 2) {
+      lo = min(lo, (uint32_t)A.en[ai]);
+      hi = max(hi, (uint32_t)A.en[ai + 1]);
+    }
   lo = grp_min<G>(lo);
   hi = grp_max<G>(hi);
   if (lo > hi) {
@@ -635,9 +643,17 @@ __device__ __forceinline__ void spgemm_row(uint64_t row, Mat<S> A, Mat<S> B, uin
         const uint64_t rs = max((uint64_t)rc[k].x, cb), rf = min((uint64_t)rc[k].y, cl);
         if (rs <= rf) window_set(lds, (uint32_t)(rs - cb), (uint32_t)(rf - cb));
       }
+      if (lane == 0 && a_only.x <= a_only.y) {
+        const uint64_t rs = max((uint64_t)a_only.x, cb), rf = min((uint64_t)a_only.y, cl);
+        if (rs <= rf) window_set(lds, (uint32_t)(rs - cb), (uint32_t)(rf - cb));
+      }
     } else {
       for (uint64_t ai = a0; ai < a1; ai += 2) {
         const uint32_t s = A.en[ai], e = A.en[ai + 1];
+        if (plus_i && lane == 0) {
+          const uint64_t rs = max((uint64_t)s, cb), rf = min((uint64_t)e, cl);
+          if (rs <= rf) window_set(lds, (uint32_t)(rs - cb), (uint32_t)(rf - cb));
+        }
         const uint64_t p1 = B.rm[(uint64_t)e + 1];
         for (uint64_t p = (uint64_t)B.rm[s] + 2 * lane; p < p1; p += 2 * G) {
           const uint2 r = ld_range(B.en, p);
@@ -715,7 +731,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_group(
     const uint32_t *__restrict__ rows, const unsigned int *__restrict__ n_dev, uint32_t n_host,
     uint32_t *__restrict__ cnt, uint32_t *__restrict__ stage, uint32_t *__restrict__ ovf_list,
     OpCounters *__restrict__ ctr, const S *__restrict__ c_rm, uint32_t *__restrict__ c_en,
-    const unsigned int *__restrict__ status) {
+    const unsigned int *__restrict__ status, uint32_t plus_i) {
   extern __shared__ uint64_t lds_all[];
   if (*status) return;
   constexpr uint32_t GPB = BLOCK / G;
@@ -731,7 +747,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_group(
       const uint64_t row = uniform_u64<G>(rows[i]);
       spgemm_row<S, G, DIRECT>(row, A, B, lds, W, lane, KS, nstage, cnt, stage, ovf_list,
                                DIRECT ? nullptr : &ctr->overflow, c_en,
-                               DIRECT ? (uint64_t)c_rm[row] : 0, nnz_acc);
+                               DIRECT ? (uint64_t)c_rm[row] : 0, nnz_acc, plus_i);
     }
   }
   if (!DIRECT) block_add_u64(nnz_acc, &ctr->nnz);
@@ -1248,7 +1264,7 @@ void launch_group(hipStream_t s, DView a, DView b, const GroupLaunch &g) {
   hipLaunchKernelGGL((k_spgemm_group<S, G, DIRECT>), dim3(g.grid), dim3(BLOCK), lds, s,
                      mat<S>(a), mat<S>(b), g.lds_words, g.stage, g.rows_per_group, g.nstage,
                      g.rows, g.n_dev, g.n_host, g.cnt, g.stage_buf, g.ovf_list, g.ctr,
-                     static_cast<const S *>(g.c_rowmap), g.c_entries, g.status);
+                     static_cast<const S *>(g.c_rowmap), g.c_entries, g.status, g.plus_i);
 }
 
 template <typename S, bool DIRECT>
@@ -1352,7 +1368,8 @@ void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t max_entries, OpC
 void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, uint32_t lane_max,
                  uint32_t mid_max, uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
-                 const unsigned int *status, const void *summ, uint32_t summ_min) {
+                 const unsigned int *status, const void *summ, uint32_t summ_min,
+                 uint32_t plus_i) {
   if (!a.nrows) return;
   const uint2 *pre = static_cast<const uint2 *>(summ);
   const uint2 *suf = pre ? pre + b.nrows : nullptr;
@@ -1361,7 +1378,7 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
 #define DV_LANE(S_, M_)                                                                        \
   hipLaunchKernelGGL((k_spgemm_lane<S_, M_>), dim3(grid), dim3(BLOCK), 0, s, mat<S_>(a),       \
                      mat<S_>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list, \
-                     mid_list, big_list, ctr, status, pre, suf, summ_min)
+                     mid_list, big_list, ctr, status, pre, suf, summ_min, plus_i)
   if (sb == 4) {
     if (pre) DV_LANE(uint32_t, true);
     else DV_LANE(uint32_t, false);

@@ -46,6 +46,7 @@ struct GroupLaunch {
   const void *c_rowmap;
   uint32_t *c_entries;
   const unsigned int *status;  // kernels exit at once when *status != 0
+  uint32_t plus_i;             // 1: the product is A (B + I)
 };
 
 void fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
@@ -83,7 +84,8 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
                  uint32_t mid_max,
                  uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
-                 const unsigned int *status, const void *summ, uint32_t summ_min);
+                 const unsigned int *status, const void *summ, uint32_t summ_min,
+                 uint32_t plus_i);
 // Interval summaries of B in 32-row blocks: per row, the union of the rows
 // from its block's start to it (prefix) and from it to its block's end
 // (suffix), as one interval when known to be one; summary_words(nrows) uint2
