@@ -158,7 +158,6 @@ __device__ void block_add_u64(unsigned long long v, unsigned long long *dst) {
 // is handed to the group tiers.
 constexpr int LANE_SLOTS = 4;
 constexpr int LANE_BATCH = 8;  // slice loads issued together per lane
-constexpr int A_BATCH = 2;     // A ranges whose B row bounds load together
 
 struct Runs {
   uint32_t lo[LANE_SLOTS], hi[LANE_SLOTS];  // empty slot: lo = 0xFFFFFFFF, hi = 0
@@ -447,33 +446,57 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
   if (*status) return;  // an earlier step overflowed: operands are not valid
   unsigned long long nnz_acc = 0, slice_acc = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  // Software pipeline over this lane's rows: the A row bounds of the row two
-  // visits ahead and the first A range of the next row are loaded while the
-  // current row is folded, so a row's own chain is B row bounds -> B ranges.
+  // Three-stage software pipeline over this lane's rows (one visit apart):
+  // while the current row is folded, the next row's first A range's B row
+  // bounds, the first A range of the row after, and the A row bounds of the
+  // third are loaded.  A row's own chain is then a single round trip to its
+  // B ranges (or summaries).
   uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-  S n_a0 = 0, n_a1 = 0, m_a0 = 0, m_a1 = 0;  // next row, the one after
-  uint2 n_first = make_uint2(1u, 0u);
+  S c_a0 = 0, c_a1 = 0, c_bs = 0, c_be = 0;  // current row
+  S p1_a0 = 0, p1_a1 = 0;                    // next row
+  S p2_a0 = 0, p2_a1 = 0;                    // the one after
+  uint2 c_first = make_uint2(1u, 0u), p1_first = make_uint2(1u, 0u);
   if (row < nrows) {
-    n_a0 = A.rm[row];
-    n_a1 = A.rm[row + 1];
+    c_a0 = A.rm[row];
+    c_a1 = A.rm[row + 1];
   }
   if (row + stride < nrows) {
-    m_a0 = A.rm[row + stride];
-    m_a1 = A.rm[row + stride + 1];
+    p1_a0 = A.rm[row + stride];
+    p1_a1 = A.rm[row + stride + 1];
   }
-  if (n_a0 < n_a1) n_first = ld_range(A.en, n_a0);
+  if (row + 2 * stride < nrows) {
+    p2_a0 = A.rm[row + 2 * stride];
+    p2_a1 = A.rm[row + 2 * stride + 1];
+  }
+  if (c_a0 < c_a1) c_first = ld_range(A.en, c_a0);
+  if (p1_a0 < p1_a1) p1_first = ld_range(A.en, p1_a0);
+  if (c_a0 < c_a1) {
+    c_bs = B.rm[c_first.x];
+    c_be = B.rm[(uint64_t)c_first.y + 1];
+  }
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows; base += stride) {
     row = base + threadIdx.x;
-    const uint64_t a0 = n_a0, a1 = n_a1;  // (S: the pipeline holds row bounds at their width)
-    const uint2 first = n_first;
-    n_a0 = m_a0;
-    n_a1 = m_a1;
-    n_first = make_uint2(1u, 0u);
-    if (n_a0 < n_a1) n_first = ld_range(A.en, n_a0);
-    m_a0 = m_a1 = 0;
-    if (row + 2 * stride < nrows) {
-      m_a0 = A.rm[row + 2 * stride];
-      m_a1 = A.rm[row + 2 * stride + 1];
+    const S a0 = c_a0, a1 = c_a1;
+    const uint2 first = c_first;
+    const S first_bs = c_bs, first_be = c_be;
+    // advance the pipeline: next row's B bounds, the first range of the row
+    // after, the A bounds of the third
+    c_a0 = p1_a0;
+    c_a1 = p1_a1;
+    c_first = p1_first;
+    c_bs = c_be = 0;
+    if (c_a0 < c_a1) {
+      c_bs = B.rm[c_first.x];
+      c_be = B.rm[(uint64_t)c_first.y + 1];
+    }
+    p1_a0 = p2_a0;
+    p1_a1 = p2_a1;
+    p1_first = make_uint2(1u, 0u);
+    if (p1_a0 < p1_a1) p1_first = ld_range(A.en, p1_a0);
+    p2_a0 = p2_a1 = 0;
+    if (row + 3 * stride < nrows) {
+      p2_a0 = A.rm[row + 3 * stride];
+      p2_a1 = A.rm[row + 3 * stride + 1];
     }
     int cls = 0;  // 0 done here (or no row), 1 -> medium groups, 2 -> 64-lane groups
     bool ovf = false;
@@ -490,32 +513,13 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
       // common case in banded slices); any other range moves it into the
       // slots and starts a new running interval.
       uint32_t cl = 0xFFFFFFFFu, ch = 0u;
-      // the first A_BATCH A ranges and their B row bounds are loaded together
-      uint2 arq[A_BATCH];
-      S bsq[A_BATCH], beq[A_BATCH];
-#pragma unroll
-      for (int q = 0; q < A_BATCH; ++q) {
-        const uint64_t ai = a0 + 2 * q;
-        arq[q] = q == 0 ? first : ai < a1 ? ld_range(A.en, ai) : make_uint2(0u, 0u);
-      }
-#pragma unroll
-      for (int q = 0; q < A_BATCH; ++q) {
-        const bool live = a0 + 2 * q < a1;
-        bsq[q] = live ? B.rm[arq[q].x] : (S)0;
-        beq[q] = live ? B.rm[(uint64_t)arq[q].y + 1] : (S)0;
-      }
       for (uint64_t ai = a0; ai < a1; ai += 2) {
-        const uint64_t q = (ai - a0) >> 1;
         uint2 ar;
         uint64_t bs, be;
-        if (q < A_BATCH) {
-#pragma unroll
-          for (int t = 0; t < A_BATCH; ++t)
-            if (t == (int)q) {
-              ar = arq[t];
-              bs = bsq[t];
-              be = beq[t];
-            }
+        if (ai == a0) {  // from the pipeline
+          ar = first;
+          bs = first_bs;
+          be = first_be;
         } else {
           ar = ld_range(A.en, ai);
           bs = B.rm[ar.x];
