@@ -87,10 +87,12 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
   summaries = summaries && b.nrows <= n + 1 && tuning.summary_span >= 2;
   if (n) {
     if (summaries) block_summary(stream, b.sb, view(b), s.summ, status);
-    // persistent lane-tier grid: few blocks keep the per-block counter atomics
-    // (nnz, slice ranges) uncontended
+    // persistent lane-tier grid of exactly the blocks the GPU holds at once:
+    // no block waits for a slot (and runs alone at the end), and few blocks
+    // keep the per-block counter atomics (nnz, slice ranges) uncontended
+    const uint64_t resident = (uint64_t)num_cu * lane_blocks_per_cu(a.sb, summaries);
     const uint32_t lane_grid =
-        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)num_cu * 8));
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, resident));
     spgemm_lane(stream, a.sb, view(a), view(b), s.stage_entries, tuning.lane_slice_max,
                 tuning.small_slice_max, lane_grid, s.cnt, s.stage, s.ovf, s.mid, s.big, ctr,
                 status, summaries ? s.summ : nullptr, tuning.summary_span, plus_i ? 1u : 0u);

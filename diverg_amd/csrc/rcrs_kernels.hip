@@ -1393,6 +1393,28 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
 #undef DV_LANE
 }
 
+uint32_t lane_blocks_per_cu(uint32_t sb, bool summ) {
+  // resident blocks of the lane kernel per CU (occupancy), per variant
+  static int cache[4] = {0, 0, 0, 0};
+  const int k = (sb == 4 ? 0 : 2) + (summ ? 1 : 0);
+  if (!cache[k]) {
+    int nb = 0;
+    hipError_t e;
+    if (sb == 4)
+      e = summ ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spgemm_lane<uint32_t, true>, BLOCK, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spgemm_lane<uint32_t, false>, BLOCK, 0);
+    else
+      e = summ ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spgemm_lane<uint64_t, true>, BLOCK, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spgemm_lane<uint64_t, false>, BLOCK, 0);
+    if (e != hipSuccess || nb < 1) {
+      (void)hipGetLastError();
+      nb = 8;
+    }
+    cache[k] = nb;
+  }
+  return (uint32_t)cache[k];
+}
+
 uint64_t summary_words(uint64_t nrows) { return 2 * nrows; }
 
 void block_summary(hipStream_t s, uint32_t sb, DView b, void *summ, const unsigned int *status) {

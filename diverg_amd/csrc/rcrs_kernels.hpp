@@ -93,6 +93,9 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
 // summ_min rows that crosses a block boundary as a suffix, whole blocks and a
 // prefix: 2 + span / 32 reads instead of the rows' ranges.
 uint64_t summary_words(uint64_t nrows);
+// Blocks of the lane-tier kernel resident per CU (its occupancy), for its
+// persistent grid.
+uint32_t lane_blocks_per_cu(uint32_t sb, bool summaries);
 void block_summary(hipStream_t s, uint32_t sb, DView b, void *summ, const unsigned int *status);
 // Group tiers (counting or direct write), see GroupLaunch.
 void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunch &g);
