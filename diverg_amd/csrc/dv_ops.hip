@@ -30,11 +30,10 @@ int Ctx::identity_dev(uint64_t n, uint32_t sb, DevMat &out) {
 bool Ctx::op_scratch(uint64_t n, OpScratch &s) {
   const uint64_t n1 = n ? n : 1;
   const uint32_t KS = tuning.stage_entries;
-  void *buf[8];
-  const uint64_t sz[8] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * scan_words(n),
-                          8 * summary_words(n1), 8 * scan_words(n)};
-  if (!scratch_carve(sz, buf, 8)) return false;
-  s.tmp2 = static_cast<unsigned long long *>(buf[7]);
+  void *buf[7];
+  const uint64_t sz[7] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * scan_words(n),
+                          8 * summary_words(n1)};
+  if (!scratch_carve(sz, buf, 7)) return false;
   s.summ = buf[6];
   s.cnt = static_cast<uint32_t *>(buf[0]);
   s.stage = static_cast<uint32_t *>(buf[1]);
@@ -61,12 +60,10 @@ GroupLaunch Ctx::group_launch(uint32_t lanes, uint64_t nstage) const {
 // SpAdd halves (range_sparse.hpp:112-357)
 void Ctx::enq_spadd_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                           const OpScratch &s, OpCounters *ctr, uint64_t cap,
-                          unsigned int *status, int parity) {
+                          unsigned int *status) {
   spadd_count(stream, a.sb, view(a), view(b), s.cnt, status);
-  unsigned long long *ts = parity == 1 ? s.tmp2 : s.tmp;
-  unsigned long long *next = parity < 0 ? nullptr : parity == 1 ? s.tmp : s.tmp2;
-  scan_rowmap(stream, c_sb, s.cnt, a.nrows, c_rowmap, ts, ctr, cap, status, nullptr, 0,
-              nullptr, /*sums_ready=*/false, next);
+  scan_rowmap(stream, c_sb, s.cnt, a.nrows, c_rowmap, s.tmp, ctr, cap, status, nullptr, 0,
+                nullptr);
 }
 
 void Ctx::enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
@@ -84,16 +81,12 @@ void Ctx::enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap
 void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                            const OpScratch &s, OpCounters *ctr, uint64_t cap,
                            unsigned int *status, hipEvent_t after_lane, uint32_t *c_entries,
-                           bool summaries, bool plus_i, int parity) {
+                           bool summaries, bool plus_i) {
   const uint64_t n = a.nrows;
-  unsigned long long *ts = parity == 1 ? s.tmp2 : s.tmp;
-  unsigned long long *next = parity < 0 ? nullptr : parity == 1 ? s.tmp : s.tmp2;
   // the summary buffer of the scratch holds summary_words(n + 1) words
   summaries = summaries && b.nrows <= n + 1 && tuning.summary_span >= 2;
   if (n) {
-    // the counting kernels add each tile's sum (a planned build's buffer was
-    // zeroed by the previous scan)
-    if (parity < 0) clear_tile_sums(stream, ts, n);
+    clear_tile_sums(stream, s.tmp, n);  // the counting kernels add each tile's sum
     if (summaries) block_summary(stream, b.sb, view(b), s.summ, status);
     // persistent lane-tier grid of exactly the blocks the GPU holds at once:
     // no block waits for a slot (and runs alone at the end), and few blocks
@@ -103,11 +96,11 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, resident));
     spgemm_lane(stream, a.sb, view(a), view(b), s.stage_entries, tuning.lane_slice_max,
                 tuning.small_slice_max, lane_grid, s.cnt, s.stage, s.ovf, s.mid, s.big, ctr,
-                status, summaries ? s.summ : nullptr, tuning.summary_span, plus_i ? 1u : 0u, ts);
+                status, summaries ? s.summ : nullptr, tuning.summary_span, plus_i ? 1u : 0u, s.tmp);
     if (after_lane) (void)hipEventRecord(after_lane, stream);
     GroupLaunch g = group_launch(tuning.lanes_small, n);
     g.plus_i = plus_i ? 1u : 0u;
-    g.tile_sums = ts;
+    g.tile_sums = s.tmp;
     g.rows = s.mid;
     g.n_dev = &ctr->n_mid;
     g.cnt = s.cnt;
@@ -124,8 +117,8 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
   } else if (after_lane) {
     (void)hipEventRecord(after_lane, stream);
   }
-  scan_rowmap(stream, c_sb, s.cnt, n, c_rowmap, ts, ctr, cap, status, s.stage,
-              s.stage_entries, c_entries, /*sums_ready=*/n != 0, next);
+  scan_rowmap(stream, c_sb, s.cnt, n, c_rowmap, s.tmp, ctr, cap, status, s.stage,
+              s.stage_entries, c_entries, /*sums_ready=*/n != 0);
 }
 
 void Ctx::enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,

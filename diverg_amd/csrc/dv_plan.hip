@@ -234,12 +234,6 @@ void Ctx::plan_launch(Plan &p, bool timing) {
   fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
            (uint32_t)(nops * sizeof(OpCounters) / 4), 0u);
   fill_u32(stream, d_status, 1, 0u);
-  // tile sums alternate between two buffers: each scan zeroes the one the
-  // next operation's counting kernels add to; the first is zeroed here
-  uint64_t nmax = 0;
-  for (const PlanOp &o : p.ops) nmax = std::max(nmax, p.slots[o.c].nrows);
-  clear_tile_sums(stream, s.tmp, nmax);
-  int parity = 0;
   for (int i = 0; i < nops; ++i) {
     const PlanOp &o = p.ops[i];
     PlanSlot &c = p.slots[o.c];
@@ -253,8 +247,7 @@ void Ctx::plan_launch(Plan &p, bool timing) {
     }
     const DevMat a = p.slots[o.a].as_mat(), b = p.slots[o.b].as_mat();
     if (o.kind == P_SPADD) {
-      enq_spadd_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status, parity);
-      parity ^= 1;
+      enq_spadd_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status);
       if (timing) (void)hipEventRecord(plan_ev[4 * i + 1], stream);
       if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
       enq_spadd_write(a, b, c.rowmap, c.entries, c.cap, ctr, d_status);
@@ -262,9 +255,7 @@ void Ctx::plan_launch(Plan &p, bool timing) {
       const bool summaries =
           tuning.summary_span && p.slots[o.a].span >= tuning.summary_span;
       enq_spgemm_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status,
-                       timing ? plan_ev[4 * i + 1] : nullptr, c.entries, summaries, o.plus_i,
-                       parity);
-      parity ^= 1;
+                       timing ? plan_ev[4 * i + 1] : nullptr, c.entries, summaries, o.plus_i);
       if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
       enq_spgemm_write(a, b, c.rowmap, c.entries, s, ctr, d_status, true, o.plus_i);
     }
@@ -299,7 +290,6 @@ static uint64_t plan_key(const Plan &p, const dv_tuning &t, const void *ctr, con
   mix((uint64_t)s.cnt);
   mix((uint64_t)s.stage);
   mix((uint64_t)s.tmp);
-  mix((uint64_t)s.tmp2);
   mix(s.stage_entries);
   const uint32_t *tw = reinterpret_cast<const uint32_t *>(&t);
   for (size_t i = 0; i < sizeof(dv_tuning) / 4; ++i) mix(tw[i]);

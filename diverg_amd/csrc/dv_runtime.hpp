@@ -40,7 +40,6 @@ struct OpScratch {
   uint32_t *mid = nullptr;    // rows for the medium groups
   uint32_t *big = nullptr;    // rows for the 64-lane groups
   unsigned long long *tmp = nullptr;  // scan tile words
-  unsigned long long *tmp2 = nullptr; // the second tile-word buffer (planned builds alternate)
   void *summ = nullptr;               // block summaries of the right operand
   uint32_t stage_entries = 0;
 };
@@ -194,12 +193,8 @@ struct Ctx {
   // dv_ops.hip
   bool op_scratch(uint64_t n, OpScratch &s);
   GroupLaunch group_launch(uint32_t lanes, uint64_t nstage) const;
-  // parity >= 0 (planned builds): the operation's tile sums are buffer
-  // parity of the scratch, which is already zero for a product, and its scan
-  // zeroes the other buffer for the next operation; -1: stand-alone
   void enq_spadd_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
-                       const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status,
-                       int parity = -1);
+                       const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status);
   void enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
                        uint32_t *c_entries, uint64_t cap, OpCounters *ctr,
                        const unsigned int *status);
@@ -208,7 +203,7 @@ struct Ctx {
   void enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                         const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status,
                         hipEvent_t after_lane = nullptr, uint32_t *c_entries = nullptr,
-                        bool summaries = false, bool plus_i = false, int parity = -1);
+                        bool summaries = false, bool plus_i = false);
   // plus_i: the product is a (b + I)
   void enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
                         uint32_t *c_entries, const OpScratch &s, OpCounters *ctr,

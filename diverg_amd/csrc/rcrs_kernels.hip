@@ -905,13 +905,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(
     const uint32_t *__restrict__ cnt, uint64_t n, S *__restrict__ rm,
     const unsigned long long *__restrict__ tsum, const uint32_t *__restrict__ stage, uint32_t KS,
     uint32_t *__restrict__ c_en, OpCounters *__restrict__ ctr, unsigned long long cap,
-    uint32_t sb, unsigned int *__restrict__ status, int prefixed,
-    unsigned long long *__restrict__ clear, uint64_t clear_words) {
+    uint32_t sb, unsigned int *__restrict__ status, int prefixed) {
   __shared__ uint32_t s_cnt[SCAN_PAD];
-  // the other tile-sum buffer, which the next product's counting kernels add to
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < clear_words;
-       i += (uint64_t)gridDim.x * BLOCK)
-    clear[i] = 0;
   __shared__ uint32_t s_rel[SCAN_PAD];  // exclusive offsets within the tile
   const uint64_t tile = blockIdx.x;
   const uint64_t t0 = tile * SCAN_TILE;
@@ -1342,7 +1337,7 @@ void clear_tile_sums(hipStream_t s, unsigned long long *tile_sums, uint64_t n) {
 void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
                  unsigned long long *tile_sums, OpCounters *ctr, uint64_t cap,
                  unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
-                 uint32_t *c_entries, bool sums_ready, unsigned long long *clear_next) {
+                 uint32_t *c_entries, bool sums_ready) {
   if (n == 0) {
     fill_u32(s, static_cast<uint32_t *>(rowmap), sb / 4, 0u);
     return;
@@ -1357,8 +1352,7 @@ void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, vo
 #define DV_SR(S_, E_)                                                                       \
   hipLaunchKernelGGL((k_scan_rows<S_, E_>), dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, \
                      static_cast<S_ *>(rowmap), tile_sums, stage, stage_entries, c_entries,     \
-                     ctr, (unsigned long long)cap, sb, status, prefixed, clear_next,     \
-                     clear_next ? ntiles : 0)
+                     ctr, (unsigned long long)cap, sb, status, prefixed)
   if (sb == 4) {
     if (emit) DV_SR(uint32_t, true);
     else DV_SR(uint32_t, false);

@@ -64,11 +64,7 @@ uint64_t scan_words(uint64_t n);
 void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
                  unsigned long long *tile_sums, OpCounters *ctr, uint64_t cap,
                  unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
-                 uint32_t *c_entries, bool sums_ready = false,
-                 unsigned long long *clear_next = nullptr);
-// (sums_ready: tile_sums already hold every tile's count sum, added by the
-// counting kernels; clear_next, if given: a second tile-sum buffer of the
-// same size that the scan zeroes for the next operation)
+                 uint32_t *c_entries, bool sums_ready = false);
 // Zero the scan tile sums of n rows before producers add to them.
 void clear_tile_sums(hipStream_t s, unsigned long long *tile_sums, uint64_t n);
 
