@@ -86,7 +86,6 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
   // the summary buffer of the scratch holds summary_words(n + 1) words
   summaries = summaries && b.nrows <= n + 1 && tuning.summary_span >= 2;
   if (n) {
-    clear_tile_sums(stream, s.tmp, n);  // the counting kernels add each tile's sum
     if (summaries) block_summary(stream, b.sb, view(b), s.summ, status);
     // persistent lane-tier grid of exactly the blocks the GPU holds at once:
     // no block waits for a slot (and runs alone at the end), and few blocks
@@ -96,11 +95,10 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, resident));
     spgemm_lane(stream, a.sb, view(a), view(b), s.stage_entries, tuning.lane_slice_max,
                 tuning.small_slice_max, lane_grid, s.cnt, s.stage, s.ovf, s.mid, s.big, ctr,
-                status, summaries ? s.summ : nullptr, tuning.summary_span, plus_i ? 1u : 0u, s.tmp);
+                status, summaries ? s.summ : nullptr, tuning.summary_span, plus_i ? 1u : 0u);
     if (after_lane) (void)hipEventRecord(after_lane, stream);
     GroupLaunch g = group_launch(tuning.lanes_small, n);
     g.plus_i = plus_i ? 1u : 0u;
-    g.tile_sums = s.tmp;
     g.rows = s.mid;
     g.n_dev = &ctr->n_mid;
     g.cnt = s.cnt;
@@ -118,7 +116,7 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
     (void)hipEventRecord(after_lane, stream);
   }
   scan_rowmap(stream, c_sb, s.cnt, n, c_rowmap, s.tmp, ctr, cap, status, s.stage,
-              s.stage_entries, c_entries, /*sums_ready=*/n != 0);
+                s.stage_entries, c_entries);
 }
 
 void Ctx::enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,

@@ -45,17 +45,6 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int CACHE = 4;             // group tiers: cached ranges per lane
 
-// Row-pointer scan tiles: SCAN_TILE consecutive rows (the producers of the
-// row counts add each tile's sum as they go).
-constexpr int SCAN_ITEMS = 8;
-constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
-}
-
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -453,8 +442,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     uint32_t *__restrict__ ovf_list, uint32_t *__restrict__ mid_list,
     uint32_t *__restrict__ big_list, OpCounters *__restrict__ ctr,
     const unsigned int *__restrict__ status, const uint2 *__restrict__ pre,
-    const uint2 *__restrict__ suf, uint32_t summ_min, uint32_t plus_i,
-    unsigned long long *__restrict__ tsum) {
+    const uint2 *__restrict__ suf, uint32_t summ_min, uint32_t plus_i) {
   if (*status) return;  // an earlier step overflowed: operands are not valid
   unsigned long long nnz_acc = 0, slice_acc = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -512,7 +500,6 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     }
     int cls = 0;  // 0 done here (or no row), 1 -> medium groups, 2 -> 64-lane groups
     bool ovf = false;
-    uint32_t my_cnt = 0;  // entries this lane wrote a count for
     if (row < nrows) {
       uint64_t len = 0;  // B-slice ranges of the row, the tier choice
       bool ok = true;
@@ -569,17 +556,12 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
           }
         }
         cnt[row] = idx;
-        my_cnt = idx;
         ovf = idx > KS;
       }
     }
     wave_append(ovf, (uint32_t)row, ovf_list, &ctr->overflow);
     wave_append(cls == 1, (uint32_t)row, mid_list, &ctr->n_mid);
     wave_append(cls == 2, (uint32_t)row, big_list, &ctr->n_big);
-    // the wave's 64 consecutive rows lie in one scan tile: add their counts
-    const uint32_t wsum = wave_sum_u32(my_cnt);
-    if ((threadIdx.x & 63) == 0 && wsum)
-      atomicAdd(&tsum[(row - (threadIdx.x & 63)) / SCAN_TILE], (unsigned long long)wsum);
   }
   block_add_u64(nnz_acc, &ctr->nnz);
   block_add_u64(slice_acc, &ctr->slice);
@@ -595,8 +577,7 @@ __device__ __forceinline__ void spgemm_row(uint64_t row, Mat<S> A, Mat<S> B, uin
                                            uint64_t nstage, uint32_t *cnt, uint32_t *stage,
                                            uint32_t *ovf_list, unsigned int *ovf_count,
                                            uint32_t *c_en, uint64_t c_base,
-                                           unsigned long long &nnz_acc, uint32_t plus_i,
-                                           unsigned long long *tsum) {
+                                           unsigned long long &nnz_acc, uint32_t plus_i) {
   const uint64_t a0 = A.rm[row], a1 = A.rm[row + 1];
 
   // 1. column extent.  A single-segment slice of at most CACHE * G ranges is
@@ -728,7 +709,6 @@ __device__ __forceinline__ void spgemm_row(uint64_t row, Mat<S> A, Mat<S> B, uin
   }
   if (!DIRECT && lane == 0) {
     cnt[row] = count;
-    if (count) atomicAdd(&tsum[row / SCAN_TILE], (unsigned long long)count);
     if (count > KS) {
       const unsigned int slot = atomicAdd(ovf_count, 1u);
       ovf_list[slot] = (uint32_t)row;
@@ -755,8 +735,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_group(
     const uint32_t *__restrict__ rows, const unsigned int *__restrict__ n_dev, uint32_t n_host,
     uint32_t *__restrict__ cnt, uint32_t *__restrict__ stage, uint32_t *__restrict__ ovf_list,
     OpCounters *__restrict__ ctr, const S *__restrict__ c_rm, uint32_t *__restrict__ c_en,
-    const unsigned int *__restrict__ status, uint32_t plus_i,
-    unsigned long long *__restrict__ tsum) {
+    const unsigned int *__restrict__ status, uint32_t plus_i) {
   extern __shared__ uint64_t lds_all[];
   if (*status) return;
   constexpr uint32_t GPB = BLOCK / G;
@@ -772,7 +751,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_group(
       const uint64_t row = uniform_u64<G>(rows[i]);
       spgemm_row<S, G, DIRECT>(row, A, B, lds, W, lane, KS, nstage, cnt, stage, ovf_list,
                                DIRECT ? nullptr : &ctr->overflow, c_en,
-                               DIRECT ? (uint64_t)c_rm[row] : 0, nnz_acc, plus_i, tsum);
+                               DIRECT ? (uint64_t)c_rm[row] : 0, nnz_acc, plus_i);
     }
   }
   if (!DIRECT) block_add_u64(nnz_acc, &ctr->nnz);
@@ -797,7 +776,9 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_emit_staged(
 }
 
 // ---------------------------------------------------------------------------
-// Row-pointer scan: tiles of SCAN_TILE counts.
+// Row-pointer scan: tiles of BLOCK * ITEMS counts.
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
 
 __device__ unsigned long long block_scan_incl(unsigned long long v,
                                               unsigned long long &total) {
@@ -869,7 +850,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile_sums(const uint32_t *__restrict_
 // scanned, the chunks rewritten), so that a tile reads its prefix instead of
 // adding up all earlier tiles.
 constexpr int SPINE_THREADS = 1024;
-constexpr uint64_t SPINE_MIN_TILES = 4096;
+constexpr uint64_t SPINE_MIN_TILES = 2048;
 
 __global__ __launch_bounds__(SPINE_THREADS) void k_scan_spine(unsigned long long *__restrict__ tsum,
                                                               uint64_t ntiles) {
@@ -1287,8 +1268,7 @@ void launch_group(hipStream_t s, DView a, DView b, const GroupLaunch &g) {
   hipLaunchKernelGGL((k_spgemm_group<S, G, DIRECT>), dim3(g.grid), dim3(BLOCK), lds, s,
                      mat<S>(a), mat<S>(b), g.lds_words, g.stage, g.rows_per_group, g.nstage,
                      g.rows, g.n_dev, g.n_host, g.cnt, g.stage_buf, g.ovf_list, g.ctr,
-                     static_cast<const S *>(g.c_rowmap), g.c_entries, g.status, g.plus_i,
-                     g.tile_sums);
+                     static_cast<const S *>(g.c_rowmap), g.c_entries, g.status, g.plus_i);
 }
 
 template <typename S, bool DIRECT>
@@ -1329,22 +1309,16 @@ void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *en
 
 uint64_t scan_words(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
-void clear_tile_sums(hipStream_t s, unsigned long long *tile_sums, uint64_t n) {
-  const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-  if (ntiles) fill_u32(s, reinterpret_cast<uint32_t *>(tile_sums), 2 * ntiles, 0u);
-}
-
 void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
                  unsigned long long *tile_sums, OpCounters *ctr, uint64_t cap,
                  unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
-                 uint32_t *c_entries, bool sums_ready) {
+                 uint32_t *c_entries) {
   if (n == 0) {
     fill_u32(s, static_cast<uint32_t *>(rowmap), sb / 4, 0u);
     return;
   }
   const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-  if (!sums_ready)
-    hipLaunchKernelGGL(k_tile_sums, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, tile_sums);
+  hipLaunchKernelGGL(k_tile_sums, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, tile_sums);
   const int prefixed = ntiles >= SPINE_MIN_TILES;
   if (prefixed)
     hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(SPINE_THREADS), 0, s, tile_sums, ntiles);
@@ -1399,7 +1373,7 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
                  uint32_t mid_max, uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
                  const unsigned int *status, const void *summ, uint32_t summ_min,
-                 uint32_t plus_i, unsigned long long *tile_sums) {
+                 uint32_t plus_i) {
   if (!a.nrows) return;
   const uint2 *pre = static_cast<const uint2 *>(summ);
   const uint2 *suf = pre ? pre + b.nrows : nullptr;
@@ -1408,7 +1382,7 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
 #define DV_LANE(S_, M_)                                                                        \
   hipLaunchKernelGGL((k_spgemm_lane<S_, M_>), dim3(grid), dim3(BLOCK), 0, s, mat<S_>(a),       \
                      mat<S_>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list, \
-                     mid_list, big_list, ctr, status, pre, suf, summ_min, plus_i, tile_sums)
+                     mid_list, big_list, ctr, status, pre, suf, summ_min, plus_i)
   if (sb == 4) {
     if (pre) DV_LANE(uint32_t, true);
     else DV_LANE(uint32_t, false);

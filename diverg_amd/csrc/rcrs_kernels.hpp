@@ -47,7 +47,6 @@ struct GroupLaunch {
   uint32_t *c_entries;
   const unsigned int *status;  // kernels exit at once when *status != 0
   uint32_t plus_i;             // 1: the product is A (B + I)
-  unsigned long long *tile_sums;  // counting pass: per scan tile, add the rows' counts
 };
 
 void fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
@@ -64,9 +63,7 @@ uint64_t scan_words(uint64_t n);
 void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
                  unsigned long long *tile_sums, OpCounters *ctr, uint64_t cap,
                  unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
-                 uint32_t *c_entries, bool sums_ready = false);
-// Zero the scan tile sums of n rows before producers add to them.
-void clear_tile_sums(hipStream_t s, unsigned long long *tile_sums, uint64_t n);
+                 uint32_t *c_entries);
 
 // Kernels that read operands take `status` and do nothing once it is set.
 void spadd_count(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt,
@@ -88,7 +85,7 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
                  uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
                  const unsigned int *status, const void *summ, uint32_t summ_min,
-                 uint32_t plus_i, unsigned long long *tile_sums);
+                 uint32_t plus_i);
 // Interval summaries of B in 32-row blocks: per row, the union of the rows
 // from its block's start to it (prefix) and from it to its block's end
 // (suffix), as one interval when known to be one; summary_words(nrows) uint2
