@@ -777,7 +777,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_emit_staged(
 
 // ---------------------------------------------------------------------------
 // Row-pointer scan: tiles of BLOCK * ITEMS counts.
-constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
 
 __device__ unsigned long long block_scan_incl(unsigned long long v,
@@ -850,7 +850,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile_sums(const uint32_t *__restrict_
 // scanned, the chunks rewritten), so that a tile reads its prefix instead of
 // adding up all earlier tiles.
 constexpr int SPINE_THREADS = 1024;
-constexpr uint64_t SPINE_MIN_TILES = 2048;
+constexpr uint64_t SPINE_MIN_TILES = 4096;
 
 __global__ __launch_bounds__(SPINE_THREADS) void k_scan_spine(unsigned long long *__restrict__ tsum,
                                                               uint64_t ntiles) {

@@ -158,9 +158,9 @@ def test_distance_index_block_summaries(ctx, summary_span):
 
 def test_distance_index_many_scan_tiles(ctx):
     """A graph of over 8.4 million loci: the row-pointer scans have more than
-    2048 tiles of 4096 rows and take the spine path."""
+    4096 tiles of 2048 rows and take the spine path."""
     a = dv.generate_graph(2, backbone_bp=9_000_000, seed=12, size_bytes=8)
-    assert a.nrows > 2048 * 4096
+    assert a.nrows > 4096 * 2048
     print(f"dindex many tiles: {a.nrows} loci", flush=True)
     assert_identical(ctx.create_distance_index(a, 5, 24), orc.dindex(a, 5, 24), "D[5,24] 9e6 bp")
 
