@@ -81,7 +81,7 @@ void Ctx::enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap
 void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                            const OpScratch &s, OpCounters *ctr, uint64_t cap,
                            unsigned int *status, hipEvent_t after_lane, uint32_t *c_entries,
-                           bool summaries, bool plus_i) {
+                           bool summaries, bool plus_i, hipEvent_t before_lane) {
   const uint64_t n = a.nrows;
   // the summary buffer of the scratch holds summary_words(n + 1) words
   summaries = summaries && b.nrows <= n + 1 && tuning.summary_span >= 2;
@@ -93,6 +93,7 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
     const uint64_t resident = (uint64_t)num_cu * lane_blocks_per_cu(a.sb, summaries);
     const uint32_t lane_grid =
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, resident));
+    if (before_lane) (void)hipEventRecord(before_lane, stream);
     spgemm_lane(stream, a.sb, view(a), view(b), s.stage_entries, tuning.lane_slice_max,
                 tuning.small_slice_max, lane_grid, s.cnt, s.stage, s.ovf, s.mid, s.big, ctr,
                 status, summaries ? s.summ : nullptr, tuning.summary_span, plus_i ? 1u : 0u);
@@ -112,8 +113,9 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
     g.rows = s.big;
     g.n_dev = &ctr->n_big;
     spgemm_group(stream, a.sb, view(a), view(b), g);
-  } else if (after_lane) {
-    (void)hipEventRecord(after_lane, stream);
+  } else {
+    if (before_lane) (void)hipEventRecord(before_lane, stream);
+    if (after_lane) (void)hipEventRecord(after_lane, stream);
   }
   scan_rowmap(stream, c_sb, s.cnt, n, c_rowmap, s.tmp, ctr, cap, status, s.stage,
                 s.stage_entries, c_entries);
@@ -208,7 +210,8 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   zero_counters(stream, d_ctr);
   fill_u32(stream, d_status, 1, 0u);
   prof_mark(0);
-  enq_spgemm_count(a, b, c.rowmap, c.sb, s, d_ctr, ~0ull, d_status, prof ? pev[1] : nullptr);
+  enq_spgemm_count(a, b, c.rowmap, c.sb, s, d_ctr, ~0ull, d_status, prof ? pev[1] : nullptr,
+                   nullptr, false, false, prof ? pev[5] : nullptr);
   prof_mark(2);
   OpCounters h{};
   uint32_t st = 0;
@@ -236,7 +239,8 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   }
   {
     const uint64_t alg = mat_bytes(a) + mat_bytes(b) + mat_bytes(c);
-    prof_add(0, 0, 1, alg, 8ull * h.slice);
+    prof_add(0, 5, 1, alg, 8ull * h.slice);
+    prof_add(2, 0, 5, 0, 0, false);
     prof_add(2, 1, 2, 0, 0);
     prof_add(1, 3, 4, 0, 0);
   }

@@ -238,28 +238,29 @@ void Ctx::plan_launch(Plan &p, bool timing) {
     const PlanOp &o = p.ops[i];
     PlanSlot &c = p.slots[o.c];
     OpCounters *ctr = d_plan_ctr + i;
-    if (timing) (void)hipEventRecord(plan_ev[4 * i], stream);
+    if (timing) (void)hipEventRecord(plan_ev[5 * i], stream);
     if (o.kind == P_IDENT) {
       identity(stream, c.sb, c.nrows, c.rowmap, c.entries);
       if (timing)
-        for (int e = 1; e < 4; ++e) (void)hipEventRecord(plan_ev[4 * i + e], stream);
+        for (int e = 1; e < 5; ++e) (void)hipEventRecord(plan_ev[5 * i + e], stream);
       continue;
     }
     const DevMat a = p.slots[o.a].as_mat(), b = p.slots[o.b].as_mat();
     if (o.kind == P_SPADD) {
       enq_spadd_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status);
-      if (timing) (void)hipEventRecord(plan_ev[4 * i + 1], stream);
-      if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
+      if (timing)
+        for (int e = 1; e < 4; ++e) (void)hipEventRecord(plan_ev[5 * i + e], stream);
       enq_spadd_write(a, b, c.rowmap, c.entries, c.cap, ctr, d_status);
     } else {
       const bool summaries =
           tuning.summary_span && p.slots[o.a].span >= tuning.summary_span;
       enq_spgemm_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status,
-                       timing ? plan_ev[4 * i + 1] : nullptr, c.entries, summaries, o.plus_i);
-      if (timing) (void)hipEventRecord(plan_ev[4 * i + 2], stream);
+                       timing ? plan_ev[5 * i + 2] : nullptr, c.entries, summaries, o.plus_i,
+                       timing ? plan_ev[5 * i + 1] : nullptr);
+      if (timing) (void)hipEventRecord(plan_ev[5 * i + 3], stream);
       enq_spgemm_write(a, b, c.rowmap, c.entries, s, ctr, d_status, true, o.plus_i);
     }
-    if (timing) (void)hipEventRecord(plan_ev[4 * i + 3], stream);
+    if (timing) (void)hipEventRecord(plan_ev[5 * i + 4], stream);
   }
 }
 
@@ -343,7 +344,7 @@ int Ctx::plan_finish(Plan &p, uint32_t &status_out) {
             hip(hipStreamSynchronize(stream), "plan synchronize");
   if (!ok) return DV_ERR_DEVICE;
   status_out = *h_status;
-  const bool timing = prof && (int)plan_ev.size() >= 4 * nops;
+  const bool timing = prof && (int)plan_ev.size() >= 5 * nops;
   for (int i = 0; i < nops; ++i) {
     const PlanOp &o = p.ops[i];
     PlanSlot &c = p.slots[o.c];
@@ -364,22 +365,23 @@ int Ctx::plan_finish(Plan &p, uint32_t &status_out) {
         stats_n_spadd += 1;
       }
       if (timing && !status_out) {
-        float t0 = 0, t1 = 0, t2 = 0;
-        (void)hipEventElapsedTime(&t0, plan_ev[4 * i], plan_ev[4 * i + 1]);
-        (void)hipEventElapsedTime(&t1, plan_ev[4 * i + 1], plan_ev[4 * i + 2]);
-        (void)hipEventElapsedTime(&t2, plan_ev[4 * i + 2], plan_ev[4 * i + 3]);
+        // events: op start, before the lane kernel, after it, after the
+        // count phase (group tiers, scan), after the write phase
+        float t[4] = {0, 0, 0, 0};
+        for (int e = 0; e < 4; ++e)
+          (void)hipEventElapsedTime(&t[e], plan_ev[5 * i + e], plan_ev[5 * i + e + 1]);
         if (o.kind == P_SPGEMM) {
           prof_t[0].launches += 1;
-          prof_t[0].total_ms += t0;
+          prof_t[0].total_ms += t[1];
           prof_t[0].alg_bytes += alg;
           prof_t[0].slice_bytes += 8ull * h.slice;
           prof_t[1].launches += 1;
-          prof_t[1].total_ms += t2;
+          prof_t[1].total_ms += t[3];
           prof_t[2].launches += 1;
-          prof_t[2].total_ms += t1;
+          prof_t[2].total_ms += t[0] + t[2];
         } else {
           prof_t[3].launches += 1;
-          prof_t[3].total_ms += t0 + t1 + t2;
+          prof_t[3].total_ms += t[0] + t[1] + t[2] + t[3];
           prof_t[3].alg_bytes += alg;
         }
       }
@@ -472,7 +474,7 @@ bool Ctx::plan_ctr_reserve(int nops) {
 }
 
 bool Ctx::plan_events_reserve(int nops) {
-  while ((int)plan_ev.size() < 4 * nops) {
+  while ((int)plan_ev.size() < 5 * nops) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) {
       (void)hipGetLastError();

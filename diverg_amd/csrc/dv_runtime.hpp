@@ -157,12 +157,12 @@ struct Ctx {
   void prof_mark(int i) {
     if (prof) (void)hipEventRecord(pev[i], stream);
   }
-  void prof_add(int kind, int e0, int e1, uint64_t alg, uint64_t slice) {
+  void prof_add(int kind, int e0, int e1, uint64_t alg, uint64_t slice, bool launch = true) {
     if (!prof) return;
     float ms = 0;
     (void)hipEventSynchronize(pev[e1]);
     (void)hipEventElapsedTime(&ms, pev[e0], pev[e1]);
-    prof_t[kind].launches += 1;
+    prof_t[kind].launches += launch ? 1 : 0;
     prof_t[kind].total_ms += ms;
     prof_t[kind].alg_bytes += alg;
     prof_t[kind].slice_bytes += slice;
@@ -198,12 +198,14 @@ struct Ctx {
   void enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
                        uint32_t *c_entries, uint64_t cap, OpCounters *ctr,
                        const unsigned int *status);
-  // after_lane (optional) is recorded right after the lane-tier kernel;
-  // with c_entries the scan also copies the staged rows into C
+  // before_lane / after_lane (optional) are recorded right before and right
+  // after the lane-tier kernel (so they bracket it alone); with c_entries the
+  // scan also copies the staged rows into C
   void enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                         const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status,
                         hipEvent_t after_lane = nullptr, uint32_t *c_entries = nullptr,
-                        bool summaries = false, bool plus_i = false);
+                        bool summaries = false, bool plus_i = false,
+                        hipEvent_t before_lane = nullptr);
   // plus_i: the product is a (b + I)
   void enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,
                         uint32_t *c_entries, const OpScratch &s, OpCounters *ctr,

@@ -160,9 +160,9 @@ int dv_last_tuning(const dv_context *ctx, dv_tuning *t);
 /* Per-kernel timing with HIP events on the context stream (no counterpart in
  * the reference, whose DIVERG_STATS timers print symbolic/numeric times after
  * a fence, range_sparse.hpp:2243-2278).  Kinds: 0 the SpGEMM lane-per-row
- * counting kernel (k_spgemm_lane), 1 SpGEMM write passes (staged copy +
- * direct rows), 2 the SpGEMM group-tier kernels and the row-pointer scan,
- * 3 SpAdd.  alg_bytes sums bytes(X) + bytes(Y) + bytes(C) of the timed
+ * counting kernel (k_spgemm_lane, bracketed alone), 1 SpGEMM write passes
+ * (staged copy + direct rows), 2 the rest of a SpGEMM's counting phase
+ * (interval summaries, group-tier kernels, row-pointer scan), 3 SpAdd.  alg_bytes sums bytes(X) + bytes(Y) + bytes(C) of the timed
  * SpGEMMs; slice_bytes sums 8 bytes per B-slice range the lane and group
  * tiers gathered. */
 #define DV_KERNEL_KINDS 4
