@@ -117,6 +117,16 @@ __device__ __forceinline__ uint2 ld_range(const uint32_t *__restrict__ en, uint6
   return *reinterpret_cast<const uint2 *>(en + p);
 }
 
+// Use a loaded value here, so that the compiler waits for its load at this
+// point.  Without it the wait can land at a join or a loop back edge, where
+// the memory counter (loads and stores complete in order) also makes the wave
+// wait for every load and store issued after it: prefetches meant for a later
+// row, or the current row's stores.
+template <typename T>
+__device__ __forceinline__ void need(T x) {
+  asm volatile("" ::"v"(x));
+}
+
 // Append `value` to list[*counter ...] for the flagged lanes of the wave with
 // one atomic per wave.  Must be reached by every lane of the wave.
 __device__ __forceinline__ void wave_append(bool flag, uint32_t value, uint32_t *list,
@@ -474,6 +484,12 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     c_bs = B.rm[c_first.x];
     c_be = B.rm[(uint64_t)c_first.y + 1];
   }
+  need(c_bs);
+  need(c_be);
+  need(p1_first.x);
+  need(p1_first.y);
+  need(p2_a0);
+  need(p2_a1);
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows; base += stride) {
     row = base + threadIdx.x;
     const S a0 = c_a0, a1 = c_a1;
@@ -500,30 +516,29 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     }
     int cls = 0;  // 0 done here (or no row), 1 -> medium groups, 2 -> 64-lane groups
     bool ovf = false;
-    if (row < nrows) {
-      uint64_t len = 0;  // B-slice ranges of the row, the tier choice
-      bool ok = true;
-      Runs r;
+    const bool live = row < nrows;
+    uint64_t len = 0;  // B-slice ranges of the row, the tier choice
+    bool ok = true;
+    Runs r;
 #pragma unroll
-      for (int t = 0; t < LANE_SLOTS; ++t) {
-        r.lo[t] = 0xFFFFFFFFu;
-        r.hi[t] = 0u;
-      }
-      // The running interval [cl, ch] absorbs ranges that touch it (the
-      // common case in banded slices); any other range moves it into the
-      // slots and starts a new running interval.
-      uint32_t cl = 0xFFFFFFFFu, ch = 0u;
+    for (int t = 0; t < LANE_SLOTS; ++t) {
+      r.lo[t] = 0xFFFFFFFFu;
+      r.hi[t] = 0u;
+    }
+    // The running interval [cl, ch] absorbs ranges that touch it (the
+    // common case in banded slices); any other range moves it into the
+    // slots and starts a new running interval.
+    uint32_t cl = 0xFFFFFFFFu, ch = 0u;
+    if (live) {
       for (uint64_t ai = a0; ai < a1; ai += 2) {
-        uint2 ar;
-        uint64_t bs, be;
-        if (ai == a0) {  // from the pipeline
-          ar = first;
-          bs = first_bs;
-          be = first_be;
-        } else {
+        uint2 ar = first;  // the first range and its B bounds: from the pipeline
+        uint64_t bs = first_bs, be = first_be;
+        if (ai != a0) {
           ar = ld_range(A.en, ai);
           bs = B.rm[ar.x];
           be = B.rm[(uint64_t)ar.y + 1];
+          need(bs);
+          need(be);
         }
         const uint64_t s = ar.x, e = ar.y;
         len += (be - bs) >> 1;
@@ -536,6 +551,16 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
           fold_segment(B.en, bs, be, cl, ch, r, ok, slice_acc);
         }
       }
+    }
+    // the prefetches issued above have landed with the slice loads: take
+    // them now, before this row's stores are issued
+    need(c_bs);
+    need(c_be);
+    need(p1_first.x);
+    need(p1_first.y);
+    need(p2_a0);
+    need(p2_a1);
+    if (live) {
       if (ok && cl <= ch && !insert(r, cl, ch)) {
         compact(r);
         ok = insert(r, cl, ch);
