@@ -249,18 +249,54 @@ __device__ __forceinline__ void fold(uint32_t vs, uint32_t ve, uint32_t &cl, uin
   }
 }
 
+// Fold a batch of ranges (empty ones are (1, 0)) in order.  The common case,
+// every range touching the running interval as it grows (a banded slice), is
+// done without branches; a lane with any other range redoes the batch from
+// the same state through fold(), so the result is fold()'s either way.
+__device__ __forceinline__ void fold_batch(const uint2 (&v)[LANE_BATCH], uint32_t &cl,
+                                           uint32_t &ch, Runs &r, bool &ok) {
+  uint32_t ncl = cl, nch = ch;
+  bool all = true;
+#pragma unroll
+  for (int k = 0; k < LANE_BATCH; ++k) {
+    const bool valid = v[k].x <= v[k].y;
+    // an empty running interval takes the range as it is
+    const bool touch = ncl > nch || (v[k].x <= nch + 1u && v[k].y + 1u >= ncl);
+    all = all && (!valid || touch);
+    ncl = valid && touch ? min(ncl, v[k].x) : ncl;
+    nch = valid && touch ? max(nch, v[k].y) : nch;
+  }
+  if (all) {
+    cl = ncl;
+    ch = nch;
+  } else {
+#pragma unroll
+    for (int k = 0; k < LANE_BATCH; ++k) fold(v[k].x, v[k].y, cl, ch, r, ok);
+  }
+}
+
 // Fold the flat run of ranges en[p0 .. p1) with LANE_BATCH loads in flight.
-__device__ __forceinline__ void fold_segment(const uint32_t *__restrict__ en, uint64_t p0,
-                                             uint64_t p1, uint32_t &cl, uint32_t &ch, Runs &r,
-                                             bool &ok, unsigned long long &gathered) {
+// Offsets are of the rowmap's size type (32-bit arithmetic for 32-bit rowmaps).
+// FAST selects fold_batch's branch-free common case; without it each range
+// goes through fold() (fewer registers).
+template <typename S, bool FAST = true>
+__device__ __forceinline__ void fold_segment(const uint32_t *__restrict__ en, S p0, S p1,
+                                             uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
+                                             unsigned long long &gathered) {
   gathered += (p1 - p0) >> 1;
-  for (uint64_t p = p0; ok && p < p1; p += 2 * LANE_BATCH) {
+  for (S p = p0; ok && p < p1;) {
+    const S rem = p1 - p;  // no offset past p1 is formed (it could wrap)
     uint2 v[LANE_BATCH];
 #pragma unroll
     for (int k = 0; k < LANE_BATCH; ++k)
-      v[k] = p + 2 * k < p1 ? ld_range(en, p + 2 * k) : make_uint2(1u, 0u);
+      v[k] = (S)(2 * k) < rem ? ld_range(en, p + 2 * k) : make_uint2(1u, 0u);
+    if (FAST) {
+      fold_batch(v, cl, ch, r, ok);
+    } else {
 #pragma unroll
-    for (int k = 0; k < LANE_BATCH; ++k) fold(v[k].x, v[k].y, cl, ch, r, ok);
+      for (int k = 0; k < LANE_BATCH; ++k) fold(v[k].x, v[k].y, cl, ch, r, ok);
+    }
+    p = rem > (S)(2 * LANE_BATCH) ? p + 2 * LANE_BATCH : p1;
   }
 }
 
@@ -403,7 +439,7 @@ constexpr int SUMM_BATCH = 8;
 template <typename S>
 __device__ __forceinline__ void fold_summarised(const Mat<S> &B, const uint2 *__restrict__ pre,
                                                 const uint2 *__restrict__ suf, uint64_t s,
-                                                uint64_t e, uint64_t bs, uint64_t be,
+                                                uint64_t e, S bs, S be,
                                                 uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
                                                 unsigned long long &gathered) {
   const uint64_t h1 = (s + SUMM_ROWS - 1) / SUMM_ROWS * SUMM_ROWS;
@@ -439,7 +475,7 @@ __device__ __forceinline__ void fold_summarised(const Mat<S> &B, const uint2 *__
       unknown &= unknown - 1;
       const uint64_t r0 = j < head ? s : (tail && j + 1 == nitems) ? t0 : h1 + (j - head) * SUMM_ROWS;
       const uint64_t r1 = j < head ? h1 : (tail && j + 1 == nitems) ? e + 1 : r0 + SUMM_ROWS;
-      fold_segment(B.en, r0 == s ? bs : (uint64_t)B.rm[r0], r1 == e + 1 ? be : (uint64_t)B.rm[r1],
+      fold_segment<S, sizeof(S) == 4>(B.en, r0 == s ? bs : B.rm[r0], r1 == e + 1 ? be : B.rm[r1],
                    cl, ch, r, ok, gathered);
     }
   }
@@ -532,7 +568,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     if (live) {
       for (uint64_t ai = a0; ai < a1; ai += 2) {
         uint2 ar = first;  // the first range and its B bounds: from the pipeline
-        uint64_t bs = first_bs, be = first_be;
+        S bs = first_bs, be = first_be;
         if (ai != a0) {
           ar = ld_range(A.en, ai);
           bs = B.rm[ar.x];
@@ -548,7 +584,9 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
         if (SUMM && (s / SUMM_ROWS) != (e / SUMM_ROWS) && e - s + 1 >= summ_min) {
           fold_summarised(B, pre, suf, s, e, bs, be, cl, ch, r, ok, slice_acc);
         } else {
-          fold_segment(B.en, bs, be, cl, ch, r, ok, slice_acc);
+          // the 64-bit summarised variant keeps 4 waves per SIMD only
+          // without the fast batch
+          fold_segment<S, !(SUMM && sizeof(S) == 8)>(B.en, bs, be, cl, ch, r, ok, slice_acc);
         }
       }
     }
@@ -560,7 +598,19 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     need(p1_first.y);
     need(p2_a0);
     need(p2_a1);
-    if (live) {
+    // live slots form a prefix (insert fills the first free slot, compact
+    // moves emptied ones last): with slot 0 empty the row is [cl, ch] alone
+    const bool single = ok && r.lo[0] > r.hi[0];
+    if (live && single) {
+      uint32_t idx = 0;
+      if (cl <= ch) {
+        nnz_acc += (unsigned long long)(ch - cl) + 1;
+        if (0 < KS) reinterpret_cast<uint2 *>(stage)[row] = make_uint2(cl, ch);
+        idx = 2;
+      }
+      cnt[row] = idx;
+      ovf = idx > KS;
+    } else if (live) {
       if (ok && cl <= ch && !insert(r, cl, ch)) {
         compact(r);
         ok = insert(r, cl, ch);
