@@ -12,7 +12,7 @@
 #include <string>
 #include <unordered_map>
 
-#include "../../include/diverg_amd_graph.h"
+#include "diverg_amd_graph.h"
 #include "dv_graph.hpp"
 #include "dv_internal.hpp"
 #include "dv_runtime.hpp"

@@ -6,8 +6,8 @@
 #include <cstring>
 #include <string>
 
-#include "../../include/diverg_amd.h"
-#include "../../include/diverg_amd_gen.h"
+#include "diverg_amd.h"
+#include "diverg_amd_gen.h"
 
 namespace dv {
 

@@ -30,11 +30,15 @@ int Ctx::identity_dev(uint64_t n, uint32_t sb, DevMat &out) {
 bool Ctx::op_scratch(uint64_t n, OpScratch &s) {
   const uint64_t n1 = n ? n : 1;
   const uint32_t KS = tuning.stage_entries;
-  void *buf[7];
-  const uint64_t sz[7] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * scan_words(n),
-                          8 * summary_words(n1)};
-  if (!scratch_carve(sz, buf, 7)) return false;
+  // the lane-tier grid is at most 8 blocks of 256 threads per CU
+  const uint32_t parts_cap = 8 * std::max<uint32_t>(1, num_cu);
+  void *buf[8];
+  const uint64_t sz[8] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * scan_words(n),
+                          8 * summary_words(n1), 16ull * parts_cap};
+  if (!scratch_carve(sz, buf, 8)) return false;
   s.summ = buf[6];
+  s.parts = static_cast<unsigned long long *>(buf[7]);
+  s.parts_cap = parts_cap;
   s.cnt = static_cast<uint32_t *>(buf[0]);
   s.stage = static_cast<uint32_t *>(buf[1]);
   s.ovf = static_cast<uint32_t *>(buf[2]);
@@ -83,6 +87,7 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
                            unsigned int *status, hipEvent_t after_lane, uint32_t *c_entries,
                            bool summaries, bool plus_i, hipEvent_t before_lane) {
   const uint64_t n = a.nrows;
+  uint32_t nparts = 0;  // lane-tier blocks whose partial sums the scan adds up
   // the summary buffer of the scratch holds summary_words(n + 1) words
   summaries = summaries && b.nrows <= n + 1 && tuning.summary_span >= 2;
   if (n) {
@@ -91,12 +96,14 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
     // no block waits for a slot (and runs alone at the end), and few blocks
     // keep the per-block counter atomics (nnz, slice ranges) uncontended
     const uint64_t resident = (uint64_t)num_cu * lane_blocks_per_cu(a.sb, summaries);
-    const uint32_t lane_grid =
-        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, resident));
+    const uint32_t lane_grid = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>({(n + 255) / 256, resident, (uint64_t)s.parts_cap}));
     if (before_lane) (void)hipEventRecord(before_lane, stream);
     spgemm_lane(stream, a.sb, view(a), view(b), s.stage_entries, tuning.lane_slice_max,
                 tuning.small_slice_max, lane_grid, s.cnt, s.stage, s.ovf, s.mid, s.big, ctr,
-                status, summaries ? s.summ : nullptr, tuning.summary_span, plus_i ? 1u : 0u);
+                status, summaries ? s.summ : nullptr, tuning.summary_span, plus_i ? 1u : 0u,
+                s.parts);
+    nparts = lane_grid;
     if (after_lane) (void)hipEventRecord(after_lane, stream);
     GroupLaunch g = group_launch(tuning.lanes_small, n);
     g.plus_i = plus_i ? 1u : 0u;
@@ -118,7 +125,7 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
     if (after_lane) (void)hipEventRecord(after_lane, stream);
   }
   scan_rowmap(stream, c_sb, s.cnt, n, c_rowmap, s.tmp, ctr, cap, status, s.stage,
-                s.stage_entries, c_entries);
+              s.stage_entries, c_entries, s.parts, nparts);
 }
 
 void Ctx::enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,

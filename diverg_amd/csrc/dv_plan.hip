@@ -291,6 +291,7 @@ static uint64_t plan_key(const Plan &p, const dv_tuning &t, const void *ctr, con
   mix((uint64_t)s.cnt);
   mix((uint64_t)s.stage);
   mix((uint64_t)s.tmp);
+  mix((uint64_t)s.parts);
   mix(s.stage_entries);
   const uint32_t *tw = reinterpret_cast<const uint32_t *>(&t);
   for (size_t i = 0; i < sizeof(dv_tuning) / 4; ++i) mix(tw[i]);

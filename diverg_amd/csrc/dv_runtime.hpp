@@ -41,6 +41,8 @@ struct OpScratch {
   uint32_t *big = nullptr;    // rows for the 64-lane groups
   unsigned long long *tmp = nullptr;  // scan tile words
   void *summ = nullptr;               // block summaries of the right operand
+  unsigned long long *parts = nullptr;  // lane-tier per-block (nnz, slice) sums
+  uint32_t parts_cap = 0;               // blocks that parts has room for
   uint32_t stage_entries = 0;
 };
 

@@ -141,6 +141,26 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t value, uint32_t 
   if (flag) list[base + __popcll(m & ((1ull << lane) - 1))] = value;
 }
 
+// Sums over a wave / a block (every thread of the block must call it; the
+// trailing barrier protects the reduction buffer for the next call).
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v) {
+  __shared__ unsigned long long red[BLOCK / 64];
+  v = wave_sum_u64(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned long long t = 0;
+#pragma unroll
+  for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
 // Block sum of v added to *dst by one lane.  Callable repeatedly (the
 // trailing barrier protects the reduction buffer).
 __device__ void block_add_u64(unsigned long long v, unsigned long long *dst) {
@@ -488,7 +508,8 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     uint32_t *__restrict__ ovf_list, uint32_t *__restrict__ mid_list,
     uint32_t *__restrict__ big_list, OpCounters *__restrict__ ctr,
     const unsigned int *__restrict__ status, const uint2 *__restrict__ pre,
-    const uint2 *__restrict__ suf, uint32_t summ_min, uint32_t plus_i) {
+    const uint2 *__restrict__ suf, uint32_t summ_min, uint32_t plus_i,
+    unsigned long long *__restrict__ parts) {
   if (*status) return;  // an earlier step overflowed: operands are not valid
   unsigned long long nnz_acc = 0, slice_acc = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -638,8 +659,15 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     wave_append(cls == 1, (uint32_t)row, mid_list, &ctr->n_mid);
     wave_append(cls == 2, (uint32_t)row, big_list, &ctr->n_big);
   }
-  block_add_u64(nnz_acc, &ctr->nnz);
-  block_add_u64(slice_acc, &ctr->slice);
+  // per-block partial sums, added to the counters by the row-pointer scan:
+  // one atomic per block on the same two words would serialise at the end
+  // of the kernel, when every block finishes at once
+  nnz_acc = block_sum_u64(nnz_acc);
+  slice_acc = block_sum_u64(slice_acc);
+  if (threadIdx.x == 0) {
+    parts[2 * blockIdx.x] = nnz_acc;
+    parts[2 * blockIdx.x + 1] = slice_acc;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -889,26 +917,12 @@ __device__ unsigned long long block_scan_incl(unsigned long long v,
 // per range, rows across lanes); a tile whose entries
 // would pass `cap` writes none of them (the total then flags bit 0).
 
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
-}
-
-__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v) {
-  __shared__ unsigned long long red[BLOCK / 64];
-  v = wave_sum_u64(v);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  unsigned long long t = 0;
-#pragma unroll
-  for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
-  __syncthreads();
-  return t;
-}
-
+// Per-tile sums of the counts.  The last block also adds the counting
+// kernel's per-block (nnz, slice) partial sums, if any, to the counters.
 __global__ __launch_bounds__(BLOCK) void k_tile_sums(const uint32_t *__restrict__ cnt, uint64_t n,
-                                                     unsigned long long *__restrict__ tsum) {
+                                                     unsigned long long *__restrict__ tsum,
+                                                     const unsigned long long *__restrict__ parts,
+                                                     uint32_t nparts, OpCounters *__restrict__ ctr) {
   const uint64_t t0 = (uint64_t)blockIdx.x * SCAN_TILE;
   unsigned long long v = 0;
 #pragma unroll
@@ -918,6 +932,19 @@ __global__ __launch_bounds__(BLOCK) void k_tile_sums(const uint32_t *__restrict_
   }
   v = block_sum_u64(v);
   if (threadIdx.x == 0) tsum[blockIdx.x] = v;
+  if (nparts && blockIdx.x + 1 == gridDim.x) {
+    unsigned long long a = 0, b = 0;
+    for (uint32_t j = threadIdx.x; j < nparts; j += BLOCK) {
+      a += parts[2 * j];
+      b += parts[2 * j + 1];
+    }
+    a = block_sum_u64(a);
+    b = block_sum_u64(b);
+    if (threadIdx.x == 0) {
+      atomicAdd(&ctr->nnz, a);
+      atomicAdd(&ctr->slice, b);
+    }
+  }
 }
 
 // With many tiles, one block turns the tile sums into exclusive prefixes in
@@ -1387,13 +1414,14 @@ uint64_t scan_words(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
                  unsigned long long *tile_sums, OpCounters *ctr, uint64_t cap,
                  unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
-                 uint32_t *c_entries) {
+                 uint32_t *c_entries, const unsigned long long *parts, uint32_t nparts) {
   if (n == 0) {
     fill_u32(s, static_cast<uint32_t *>(rowmap), sb / 4, 0u);
     return;
   }
   const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-  hipLaunchKernelGGL(k_tile_sums, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, tile_sums);
+  hipLaunchKernelGGL(k_tile_sums, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, tile_sums,
+                     parts, parts ? nparts : 0u, ctr);
   const int prefixed = ntiles >= SPINE_MIN_TILES;
   if (prefixed)
     hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(SPINE_THREADS), 0, s, tile_sums, ntiles);
@@ -1448,7 +1476,7 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
                  uint32_t mid_max, uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
                  const unsigned int *status, const void *summ, uint32_t summ_min,
-                 uint32_t plus_i) {
+                 uint32_t plus_i, unsigned long long *parts) {
   if (!a.nrows) return;
   const uint2 *pre = static_cast<const uint2 *>(summ);
   const uint2 *suf = pre ? pre + b.nrows : nullptr;
@@ -1457,7 +1485,7 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
 #define DV_LANE(S_, M_)                                                                        \
   hipLaunchKernelGGL((k_spgemm_lane<S_, M_>), dim3(grid), dim3(BLOCK), 0, s, mat<S_>(a),       \
                      mat<S_>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list, \
-                     mid_list, big_list, ctr, status, pre, suf, summ_min, plus_i)
+                     mid_list, big_list, ctr, status, pre, suf, summ_min, plus_i, parts)
   if (sb == 4) {
     if (pre) DV_LANE(uint32_t, true);
     else DV_LANE(uint32_t, false);

@@ -58,12 +58,15 @@ void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *en
 // total -> ctr->total; a total above `cap` sets bit 0 of *status, above a
 // 32-bit rowmap bit 1.  Two launches (tile sums, then rows) and, when
 // c_entries is given, the copy of staged rows into C (stage of
-// stage_entries per row).  tile_sums needs scan_words(n) words.
+// stage_entries per row).  tile_sums needs scan_words(n) words.  `parts`
+// holds nparts (nnz, slice) pairs of the lane-tier kernel's blocks, added to
+// ctr->nnz and ctr->slice.
 uint64_t scan_words(uint64_t n);
 void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
                  unsigned long long *tile_sums, OpCounters *ctr, uint64_t cap,
                  unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
-                 uint32_t *c_entries);
+                 uint32_t *c_entries, const unsigned long long *parts = nullptr,
+                 uint32_t nparts = 0);
 
 // Kernels that read operands take `status` and do nothing once it is set.
 void spadd_count(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt,
@@ -79,13 +82,14 @@ void range_nnz(hipStream_t s, const uint32_t *entries, uint64_t max_entries,
 // are appended to mid_list (slices of at most mid_max ranges) or big_list.
 // Writes per-row entry counts, the first `stage` entries of each row into
 // stage_buf (range k of row r at uint2 k * nrows + r), rows with more output to ovf_list,
-// and adds set cells to ctr->nnz.
+// and per block the set cells and gathered slice ranges to parts[2 b], parts[2 b + 1]
+// (grid pairs; scan_rowmap adds them to the counters).
 void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, uint32_t lane_max,
                  uint32_t mid_max,
                  uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
                  const unsigned int *status, const void *summ, uint32_t summ_min,
-                 uint32_t plus_i);
+                 uint32_t plus_i, unsigned long long *parts);
 // Interval summaries of B in 32-row blocks: per row, the union of the rows
 // from its block's start to it (prefix) and from it to its block's end
 // (suffix), as one interval when known to be one; summary_words(nrows) uint2

@@ -11,7 +11,7 @@
 #include <unordered_map>
 #include <vector>
 
-#include "../../include/diverg_amd_graph.h"
+#include "diverg_amd_graph.h"
 #include "dv_graph.hpp"
 
 namespace dv {
