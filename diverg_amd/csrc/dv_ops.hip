@@ -85,13 +85,13 @@ void Ctx::enq_spadd_write(const DevMat &a, const DevMat &b, const void *c_rowmap
 void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                            const OpScratch &s, OpCounters *ctr, uint64_t cap,
                            unsigned int *status, hipEvent_t after_lane, uint32_t *c_entries,
-                           bool summaries, bool plus_i, hipEvent_t before_lane) {
+                           uint32_t summ_rows, bool plus_i, hipEvent_t before_lane) {
   const uint64_t n = a.nrows;
   uint32_t nparts = 0;  // lane-tier blocks whose partial sums the scan adds up
   // the summary buffer of the scratch holds summary_words(n + 1) words
-  summaries = summaries && b.nrows <= n + 1 && tuning.summary_span >= 2;
+  const bool summaries = summ_rows && b.nrows <= n + 1 && tuning.summary_span >= 2;
   if (n) {
-    if (summaries) block_summary(stream, b.sb, view(b), s.summ, status);
+    if (summaries) block_summary(stream, b.sb, view(b), s.summ, status, summ_rows);
     // persistent lane-tier grid of exactly the blocks the GPU holds at once:
     // no block waits for a slot (and runs alone at the end), and few blocks
     // keep the per-block counter atomics (nnz, slice ranges) uncontended
@@ -101,8 +101,8 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
     if (before_lane) (void)hipEventRecord(before_lane, stream);
     spgemm_lane(stream, a.sb, view(a), view(b), s.stage_entries, tuning.lane_slice_max,
                 tuning.small_slice_max, lane_grid, s.cnt, s.stage, s.ovf, s.mid, s.big, ctr,
-                status, summaries ? s.summ : nullptr, tuning.summary_span, plus_i ? 1u : 0u,
-                s.parts);
+                status, summaries ? s.summ : nullptr, tuning.summary_span, summ_rows,
+                plus_i ? 1u : 0u, s.parts);
     nparts = lane_grid;
     if (after_lane) (void)hipEventRecord(after_lane, stream);
     GroupLaunch g = group_launch(tuning.lanes_small, n);
@@ -218,7 +218,7 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   fill_u32(stream, d_status, 1, 0u);
   prof_mark(0);
   enq_spgemm_count(a, b, c.rowmap, c.sb, s, d_ctr, ~0ull, d_status, prof ? pev[1] : nullptr,
-                   nullptr, false, false, prof ? pev[5] : nullptr);
+                   nullptr, 0, false, prof ? pev[5] : nullptr);
   prof_mark(2);
   OpCounters h{};
   uint32_t st = 0;

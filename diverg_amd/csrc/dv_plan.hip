@@ -252,10 +252,11 @@ void Ctx::plan_launch(Plan &p, bool timing) {
         for (int e = 1; e < 4; ++e) (void)hipEventRecord(plan_ev[5 * i + e], stream);
       enq_spadd_write(a, b, c.rowmap, c.entries, c.cap, ctr, d_status);
     } else {
-      const bool summaries =
-          tuning.summary_span && p.slots[o.a].span >= tuning.summary_span;
+      const uint32_t summ_rows = tuning.summary_span && p.slots[o.a].span >= tuning.summary_span
+                                     ? summary_block_rows(p.slots[o.a].span)
+                                     : 0u;
       enq_spgemm_count(a, b, c.rowmap, c.sb, s, ctr, c.cap, d_status,
-                       timing ? plan_ev[5 * i + 2] : nullptr, c.entries, summaries, o.plus_i,
+                       timing ? plan_ev[5 * i + 2] : nullptr, c.entries, summ_rows, o.plus_i,
                        timing ? plan_ev[5 * i + 1] : nullptr);
       if (timing) (void)hipEventRecord(plan_ev[5 * i + 3], stream);
       enq_spgemm_write(a, b, c.rowmap, c.entries, s, ctr, d_status, true, o.plus_i);

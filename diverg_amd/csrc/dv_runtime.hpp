@@ -202,11 +202,12 @@ struct Ctx {
                        const unsigned int *status);
   // before_lane / after_lane (optional) are recorded right before and right
   // after the lane-tier kernel (so they bracket it alone); with c_entries the
-  // scan also copies the staged rows into C
+  // scan also copies the staged rows into C; summ_rows > 0: interval
+  // summaries of b in blocks of that many rows
   void enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                         const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status,
                         hipEvent_t after_lane = nullptr, uint32_t *c_entries = nullptr,
-                        bool summaries = false, bool plus_i = false,
+                        uint32_t summ_rows = 0, bool plus_i = false,
                         hipEvent_t before_lane = nullptr);
   // plus_i: the product is a (b + I)
   void enq_spgemm_write(const DevMat &a, const DevMat &b, const void *c_rowmap,

@@ -320,18 +320,19 @@ __device__ __forceinline__ void fold_segment(const uint32_t *__restrict__ en, S 
   }
 }
 
-// Interval summaries of the right operand, per 32-row block k (rows
-// 32k .. 32k+31): pre[r] is the union of rows 32k .. r and suf[r] the union
-// of rows r .. 32k+31, each (lo, hi) when that union is known to be one
-// interval, (0xFFFFFFFF, 0) when the rows are empty, (1, 0) otherwise.  The
-// whole block's union is pre[32k + 31].
+// Interval summaries of the right operand, per block k of W rows (rows
+// Wk .. Wk+W-1, W a power of two from 32 to 256, chosen per product from how
+// many rows its ranges span): pre[r] is the union of rows Wk .. r and suf[r]
+// the union of rows r .. Wk+W-1, each (lo, hi) when that union is known to be
+// one interval, (0xFFFFFFFF, 0) when the rows are empty, (1, 0) otherwise.
+// The whole block's union is pre[Wk + W - 1].
 // Partial unions in the scan are kept exactly as up to two disjoint
 // intervals, so a row with a gap (a bubble's alternate locus reaches past the
 // reference locus) does not spoil the block when neighbouring rows fill the
 // gap; a partial union of three or more pieces becomes "unknown" (the lane
 // tier then reads those rows' ranges).  A summary is never claimed for a
 // union that is not one interval.
-constexpr int SUMM_ROWS = 32;
+constexpr uint32_t SUMM_ROWS_MIN = 32, SUMM_ROWS_MAX = 256;
 constexpr uint32_t SUMM_EMPTY = 0xFFFFFFFFu;
 
 // A partial union: up to two pieces in ascending order, an unused piece is
@@ -405,45 +406,77 @@ __device__ __forceinline__ uint2 ivl_enc(const Ivl2 &v) {
   return ivl_empty(v) ? make_uint2(SUMM_EMPTY, 0u) : make_uint2(1u, 0u);
 }
 
-__device__ __forceinline__ Ivl2 ivl_shfl_up(const Ivl2 &v, int d) {
-  return Ivl2{__shfl_up(v.lo0, d, SUMM_ROWS), __shfl_up(v.hi0, d, SUMM_ROWS),
-              __shfl_up(v.lo1, d, SUMM_ROWS), __shfl_up(v.hi1, d, SUMM_ROWS)};
+// Shuffles within aligned groups of w lanes (w a power of two, at most 64).
+constexpr int SUMM_ROWS_PER_LANE = 4;
+
+__device__ __forceinline__ Ivl2 ivl_shfl_up(const Ivl2 &v, int d, int w) {
+  return Ivl2{__shfl_up(v.lo0, d, w), __shfl_up(v.hi0, d, w), __shfl_up(v.lo1, d, w),
+              __shfl_up(v.hi1, d, w)};
 }
 
-__device__ __forceinline__ Ivl2 ivl_shfl_down(const Ivl2 &v, int d) {
-  return Ivl2{__shfl_down(v.lo0, d, SUMM_ROWS), __shfl_down(v.hi0, d, SUMM_ROWS),
-              __shfl_down(v.lo1, d, SUMM_ROWS), __shfl_down(v.hi1, d, SUMM_ROWS)};
+__device__ __forceinline__ Ivl2 ivl_shfl_down(const Ivl2 &v, int d, int w) {
+  return Ivl2{__shfl_down(v.lo0, d, w), __shfl_down(v.hi0, d, w), __shfl_down(v.lo1, d, w),
+              __shfl_down(v.hi1, d, w)};
 }
 
+// Each lane takes SUMM_ROWS_PER_LANE consecutive rows: their unions are
+// joined serially into the lane's own prefixes and suffixes, the lane totals
+// are scanned across the block's `lanes` lanes (both directions), and each
+// row's summary is the join of the lanes before (after) it with its own
+// prefix (suffix).  Joins of summaries in a different order may give
+// "unknown" where another order would not, never a different interval.
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t nrows,
                                                             uint2 *__restrict__ pre,
                                                             uint2 *__restrict__ suf,
-                                                            const unsigned int *__restrict__ status) {
+                                                            const unsigned int *__restrict__ status,
+                                                            int lanes) {
   if (*status) return;
-  const int l32 = threadIdx.x & 31;
-  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows;
-       base += (uint64_t)gridDim.x * BLOCK) {
-    const uint64_t r = base + threadIdx.x;
-    Ivl2 v{SUMM_EMPTY, 0u, SUMM_EMPTY, 0u};
-    if (r < nrows) {
-      const uint64_t p1 = B.rm[r + 1];
-      for (uint64_t p = B.rm[r]; p < p1 && !ivl_unknown(v); p += 2) {
-        const uint2 x = ld_range(B.en, p);
-        v = ivl_join(v, Ivl2{x.x, x.y, SUMM_EMPTY, 0u});
+  constexpr int K = SUMM_ROWS_PER_LANE;
+  const int pos = threadIdx.x & (lanes - 1);  // the lane's place in its block
+  const Ivl2 none{SUMM_EMPTY, 0u, SUMM_EMPTY, 0u};
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK * K; base < nrows;
+       base += (uint64_t)gridDim.x * BLOCK * K) {
+    const uint64_t r0 = base + (uint64_t)threadIdx.x * K;
+    Ivl2 v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      v[k] = none;
+      const uint64_t r = r0 + k;
+      if (r < nrows) {
+        const uint64_t p1 = B.rm[r + 1];
+        for (uint64_t p = B.rm[r]; p < p1 && !ivl_unknown(v[k]); p += 2) {
+          const uint2 x = ld_range(B.en, p);
+          v[k] = ivl_join(v[k], Ivl2{x.x, x.y, SUMM_EMPTY, 0u});
+        }
       }
     }
-    Ivl2 f = v, g = v;
+    Ivl2 lp[K], ls[K];
+    lp[0] = v[0];
 #pragma unroll
-    for (int d = 1; d < SUMM_ROWS; d <<= 1) {
-      const Ivl2 o = ivl_shfl_up(f, d);
-      if (l32 >= d) f = ivl_join(o, f);
-      const Ivl2 u = ivl_shfl_down(g, d);
-      if (l32 + d < SUMM_ROWS) g = ivl_join(g, u);
+    for (int k = 1; k < K; ++k) lp[k] = ivl_join(lp[k - 1], v[k]);
+    ls[K - 1] = v[K - 1];
+#pragma unroll
+    for (int k = K - 2; k >= 0; --k) ls[k] = ivl_join(v[k], ls[k + 1]);
+    // inclusive scans of the lane totals, then shifted by one lane
+    Ivl2 f = lp[K - 1], g = lp[K - 1];
+#pragma unroll
+    for (int d = 1; d < lanes; d <<= 1) {
+      const Ivl2 o = ivl_shfl_up(f, d, lanes);
+      if (pos >= d) f = ivl_join(o, f);
+      const Ivl2 u = ivl_shfl_down(g, d, lanes);
+      if (pos + d < lanes) g = ivl_join(g, u);
     }
-    if (r < nrows) {
-      pre[r] = ivl_enc(f);
-      suf[r] = ivl_enc(g);
+    Ivl2 before = ivl_shfl_up(f, 1, lanes), after = ivl_shfl_down(g, 1, lanes);
+    if (pos == 0) before = none;
+    if (pos == lanes - 1) after = none;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint64_t r = r0 + k;
+      if (r < nrows) {
+        pre[r] = ivl_enc(ivl_join(before, lp[k]));
+        suf[r] = ivl_enc(ivl_join(ls[k], after));
+      }
     }
   }
 }
@@ -459,13 +492,14 @@ constexpr int SUMM_BATCH = 8;
 template <typename S>
 __device__ __forceinline__ void fold_summarised(const Mat<S> &B, const uint2 *__restrict__ pre,
                                                 const uint2 *__restrict__ suf, uint64_t s,
-                                                uint64_t e, S bs, S be,
+                                                uint64_t e, S bs, S be, uint32_t sh,
                                                 uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
                                                 unsigned long long &gathered) {
-  const uint64_t h1 = (s + SUMM_ROWS - 1) / SUMM_ROWS * SUMM_ROWS;
-  const uint64_t t0 = (e + 1) / SUMM_ROWS * SUMM_ROWS;
+  const uint64_t W = 1ull << sh;  // rows per summary block
+  const uint64_t h1 = (s + W - 1) >> sh << sh;
+  const uint64_t t0 = (e + 1) >> sh << sh;
   const uint64_t head = s < h1 ? 1 : 0, tail = t0 <= e ? 1 : 0;
-  const uint64_t nitems = head + (t0 - h1) / SUMM_ROWS + tail;
+  const uint64_t nitems = head + ((t0 - h1) >> sh) + tail;
   for (uint64_t j0 = 0; ok && j0 < nitems; j0 += SUMM_BATCH) {
     uint2 sm[SUMM_BATCH];
 #pragma unroll
@@ -475,7 +509,7 @@ __device__ __forceinline__ void fold_summarised(const Mat<S> &B, const uint2 *__
       if (j < nitems) {
         if (j < head) sm[q] = suf[s];
         else if (tail && j + 1 == nitems) sm[q] = pre[e];
-        else sm[q] = pre[h1 + (j - head) * SUMM_ROWS + SUMM_ROWS - 1];
+        else sm[q] = pre[h1 + ((j - head) << sh) + W - 1];
       }
     }
     uint32_t unknown = 0;
@@ -493,8 +527,8 @@ __device__ __forceinline__ void fold_summarised(const Mat<S> &B, const uint2 *__
     while (ok && unknown) {
       const uint64_t j = j0 + (uint64_t)__builtin_ctz(unknown);
       unknown &= unknown - 1;
-      const uint64_t r0 = j < head ? s : (tail && j + 1 == nitems) ? t0 : h1 + (j - head) * SUMM_ROWS;
-      const uint64_t r1 = j < head ? h1 : (tail && j + 1 == nitems) ? e + 1 : r0 + SUMM_ROWS;
+      const uint64_t r0 = j < head ? s : (tail && j + 1 == nitems) ? t0 : h1 + ((j - head) << sh);
+      const uint64_t r1 = j < head ? h1 : (tail && j + 1 == nitems) ? e + 1 : r0 + W;
       fold_segment<S, sizeof(S) == 4>(B.en, r0 == s ? bs : B.rm[r0], r1 == e + 1 ? be : B.rm[r1],
                    cl, ch, r, ok, gathered);
     }
@@ -508,7 +542,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     uint32_t *__restrict__ ovf_list, uint32_t *__restrict__ mid_list,
     uint32_t *__restrict__ big_list, OpCounters *__restrict__ ctr,
     const unsigned int *__restrict__ status, const uint2 *__restrict__ pre,
-    const uint2 *__restrict__ suf, uint32_t summ_min, uint32_t plus_i,
+    const uint2 *__restrict__ suf, uint32_t summ_min, uint32_t summ_shift, uint32_t plus_i,
     unsigned long long *__restrict__ parts) {
   if (*status) return;  // an earlier step overflowed: operands are not valid
   unsigned long long nnz_acc = 0, slice_acc = 0;
@@ -602,8 +636,8 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
         if (len > lane_max) ok = false;
         if (!ok) continue;  // only the slice size is still needed
         if (plus_i) fold((uint32_t)s, (uint32_t)e, cl, ch, r, ok);  // C = A (B + I)
-        if (SUMM && (s / SUMM_ROWS) != (e / SUMM_ROWS) && e - s + 1 >= summ_min) {
-          fold_summarised(B, pre, suf, s, e, bs, be, cl, ch, r, ok, slice_acc);
+        if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min) {
+          fold_summarised(B, pre, suf, s, e, bs, be, summ_shift, cl, ch, r, ok, slice_acc);
         } else {
           // the 64-bit summarised variant keeps 4 waves per SIMD only
           // without the fast batch
@@ -1476,7 +1510,7 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
                  uint32_t mid_max, uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
                  const unsigned int *status, const void *summ, uint32_t summ_min,
-                 uint32_t plus_i, unsigned long long *parts) {
+                 uint32_t summ_rows, uint32_t plus_i, unsigned long long *parts) {
   if (!a.nrows) return;
   const uint2 *pre = static_cast<const uint2 *>(summ);
   const uint2 *suf = pre ? pre + b.nrows : nullptr;
@@ -1485,7 +1519,8 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
 #define DV_LANE(S_, M_)                                                                        \
   hipLaunchKernelGGL((k_spgemm_lane<S_, M_>), dim3(grid), dim3(BLOCK), 0, s, mat<S_>(a),       \
                      mat<S_>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list, \
-                     mid_list, big_list, ctr, status, pre, suf, summ_min, plus_i, parts)
+                     mid_list, big_list, ctr, status, pre, suf, summ_min,                      \
+                     (uint32_t)__builtin_ctz(summ_rows ? summ_rows : 1), plus_i, parts)
   if (sb == 4) {
     if (pre) DV_LANE(uint32_t, true);
     else DV_LANE(uint32_t, false);
@@ -1520,15 +1555,23 @@ uint32_t lane_blocks_per_cu(uint32_t sb, bool summ) {
 
 uint64_t summary_words(uint64_t nrows) { return 2 * nrows; }
 
-void block_summary(hipStream_t s, uint32_t sb, DView b, void *summ, const unsigned int *status) {
+uint32_t summary_block_rows(uint64_t span) {
+  uint32_t w = SUMM_ROWS_MIN;
+  while (w < SUMM_ROWS_MAX && 8ull * w <= span) w <<= 1;
+  return w;
+}
+
+void block_summary(hipStream_t s, uint32_t sb, DView b, void *summ, const unsigned int *status,
+                   uint32_t block_rows) {
   if (!b.nrows) return;
+  const int lanes = (int)(block_rows / SUMM_ROWS_PER_LANE);
   uint2 *pre = static_cast<uint2 *>(summ), *suf = pre + b.nrows;
   if (sb == 4)
-    hipLaunchKernelGGL(k_interval_summary<uint32_t>, dim3(grid_for(b.nrows)), dim3(BLOCK), 0, s,
-                       mat<uint32_t>(b), b.nrows, pre, suf, status);
+    hipLaunchKernelGGL(k_interval_summary<uint32_t>, dim3(grid_for(b.nrows, BLOCK * 4)), dim3(BLOCK), 0, s,
+                       mat<uint32_t>(b), b.nrows, pre, suf, status, lanes);
   else
-    hipLaunchKernelGGL(k_interval_summary<uint64_t>, dim3(grid_for(b.nrows)), dim3(BLOCK), 0, s,
-                       mat<uint64_t>(b), b.nrows, pre, suf, status);
+    hipLaunchKernelGGL(k_interval_summary<uint64_t>, dim3(grid_for(b.nrows, BLOCK * 4)), dim3(BLOCK), 0, s,
+                       mat<uint64_t>(b), b.nrows, pre, suf, status, lanes);
 }
 
 void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunch &g) {

@@ -89,18 +89,24 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
                  uint32_t grid, uint32_t *cnt, uint32_t *stage_buf, uint32_t *ovf_list,
                  uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
                  const unsigned int *status, const void *summ, uint32_t summ_min,
-                 uint32_t plus_i, unsigned long long *parts);
-// Interval summaries of B in 32-row blocks: per row, the union of the rows
-// from its block's start to it (prefix) and from it to its block's end
-// (suffix), as one interval when known to be one; summary_words(nrows) uint2
-// words (prefixes, then suffixes).  The lane tier folds an A range of at least
-// summ_min rows that crosses a block boundary as a suffix, whole blocks and a
-// prefix: 2 + span / 32 reads instead of the rows' ranges.
+                 uint32_t summ_rows, uint32_t plus_i, unsigned long long *parts);
+// Interval summaries of B in blocks of block_rows rows (a power of two, 32 to
+// 256): per row, the union of the rows from its block's start to it (prefix)
+// and from it to its block's end (suffix), as one interval when known to be
+// one; summary_words(nrows) uint2 words (prefixes, then suffixes).  The lane
+// tier (summ, summ_rows = block_rows) folds an A range of at least summ_min
+// rows that crosses a block boundary as a suffix, whole blocks and a prefix:
+// 2 + span / block_rows reads instead of the rows' ranges.
 uint64_t summary_words(uint64_t nrows);
 // Blocks of the lane-tier kernel resident per CU (its occupancy), for its
 // persistent grid.
 uint32_t lane_blocks_per_cu(uint32_t sb, bool summaries);
-void block_summary(hipStream_t s, uint32_t sb, DView b, void *summ, const unsigned int *status);
+void block_summary(hipStream_t s, uint32_t sb, DView b, void *summ, const unsigned int *status,
+                   uint32_t block_rows);
+// Summary block rows for products whose left operand's ranges cover up to
+// `span` rows of the right one: the power of two between an eighth and a
+// quarter of the span (so a range reads 4 to 8 block summaries), 32 to 256.
+uint32_t summary_block_rows(uint64_t span);
 // Group tiers (counting or direct write), see GroupLaunch.
 void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunch &g);
 // Copy staged rows (count <= stage) to their place in C.

@@ -156,6 +156,17 @@ def test_distance_index_block_summaries(ctx, summary_span):
                          f"D[{dlo},{dup}] summary_span={summary_span}")
 
 
+@pytest.mark.parametrize("dlo,dup", [(0, 2100), (900, 2100)])
+def test_distance_index_wide_summary_blocks(ctx, dlo, dup):
+    """Windows wide enough that the last products summarise in blocks of 64,
+    128 and 256 rows (a quarter to an eighth of the ranges' span)."""
+    a = dv.generate_graph(3, backbone_bp=12000, seed=9, snp_rate=0.03, indel_rate=0.005,
+                          sv_rate=1e-3, sv_min=10, sv_max=200)
+    print(f"dindex wide summary blocks [{dlo},{dup}] {a.nrows} loci", flush=True)
+    assert_identical(ctx.create_distance_index(a, dlo, dup), orc.dindex(a, dlo, dup),
+                     f"D[{dlo},{dup}]")
+
+
 def test_distance_index_many_scan_tiles(ctx):
     """A graph of over 8.4 million loci: the row-pointer scans have more than
     4096 tiles of 2048 rows and take the spine path."""
