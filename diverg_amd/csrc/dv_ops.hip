@@ -39,6 +39,7 @@ bool Ctx::op_scratch(uint64_t n, OpScratch &s) {
   s.summ = buf[6];
   s.parts = static_cast<unsigned long long *>(buf[7]);
   s.parts_cap = parts_cap;
+  s.rows = n1;
   s.cnt = static_cast<uint32_t *>(buf[0]);
   s.stage = static_cast<uint32_t *>(buf[1]);
   s.ovf = static_cast<uint32_t *>(buf[2]);
@@ -88,8 +89,8 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
                            uint32_t summ_rows, bool plus_i, hipEvent_t before_lane) {
   const uint64_t n = a.nrows;
   uint32_t nparts = 0;  // lane-tier blocks whose partial sums the scan adds up
-  // the summary buffer of the scratch holds summary_words(n + 1) words
-  const bool summaries = summ_rows && b.nrows <= n + 1 && tuning.summary_span >= 2;
+  // the summary buffer of the scratch holds summary_words(s.rows) words
+  const bool summaries = summ_rows && b.nrows <= s.rows && tuning.summary_span >= 2;
   if (n) {
     if (summaries) block_summary(stream, b.sb, view(b), s.summ, status, summ_rows);
     // persistent lane-tier grid of exactly the blocks the GPU holds at once:

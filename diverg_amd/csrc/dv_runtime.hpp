@@ -43,6 +43,7 @@ struct OpScratch {
   void *summ = nullptr;               // block summaries of the right operand
   unsigned long long *parts = nullptr;  // lane-tier per-block (nnz, slice) sums
   uint32_t parts_cap = 0;               // blocks that parts has room for
+  uint64_t rows = 0;                    // rows the per-row buffers (and summ) hold
   uint32_t stage_entries = 0;
 };
 

@@ -323,9 +323,9 @@ __device__ __forceinline__ void fold_segment(const uint32_t *__restrict__ en, S 
 // Interval summaries of the right operand, per block k of W rows (rows
 // Wk .. Wk+W-1, W a power of two from 32 to 256, chosen per product from how
 // many rows its ranges span): pre[r] is the union of rows Wk .. r and suf[r]
-// the union of rows r .. Wk+W-1, each (lo, hi) when that union is known to be
-// one interval, (0xFFFFFFFF, 0) when the rows are empty, (1, 0) otherwise.
-// The whole block's union is pre[Wk + W - 1].
+// the union of rows r .. Wk+W-1, each as its one or two pieces when it has at
+// most two (ivl_store), else "unknown".  The whole block's union is
+// pre[Wk + W - 1].
 // Partial unions in the scan are kept exactly as up to two disjoint
 // intervals, so a row with a gap (a bubble's alternate locus reaches past the
 // reference locus) does not spoil the block when neighbouring rows fill the
@@ -401,54 +401,180 @@ __device__ __forceinline__ Ivl2 ivl_join(const Ivl2 &a, const Ivl2 &b) {
   return Ivl2{l0, h0, l1, h1};
 }
 
-__device__ __forceinline__ uint2 ivl_enc(const Ivl2 &v) {
-  if (!ivl_empty(v) && !ivl_unknown(v) && v.lo1 == SUMM_EMPTY) return make_uint2(v.lo0, v.hi0);
-  return ivl_empty(v) ? make_uint2(SUMM_EMPTY, 0u) : make_uint2(1u, 0u);
+// A stored summary is 8 bytes: (lo, hi) for one interval, (EMPTY, 0) for
+// the empty union, (1, 0) for an unknown one and (2, 0) for a union of two
+// pieces, which are then stored in a 16-byte side entry of the same row
+// (written only for such rows).
+constexpr uint32_t SUMM_UNKNOWN = 1u, SUMM_TWO = 2u;
+
+__device__ __forceinline__ void ivl_store(const Ivl2 &v, uint2 *__restrict__ one,
+                                          uint4 *__restrict__ two) {
+  if (ivl_empty(v)) {
+    *one = make_uint2(SUMM_EMPTY, 0u);
+  } else if (ivl_unknown(v)) {
+    *one = make_uint2(SUMM_UNKNOWN, 0u);
+  } else if (v.lo1 == SUMM_EMPTY) {
+    *one = make_uint2(v.lo0, v.hi0);
+  } else {
+    *one = make_uint2(SUMM_TWO, 0u);
+    *two = make_uint4(v.lo0, v.hi0, v.lo1, v.hi1);
+  }
 }
 
 // Shuffles within aligned groups of w lanes (w a power of two, at most 64).
 constexpr int SUMM_ROWS_PER_LANE = 4;
 
-__device__ __forceinline__ Ivl2 ivl_shfl_up(const Ivl2 &v, int d, int w) {
-  return Ivl2{__shfl_up(v.lo0, d, w), __shfl_up(v.hi0, d, w), __shfl_up(v.lo1, d, w),
-              __shfl_up(v.hi1, d, w)};
+// The value of lane L - D (UP) or L + D in a group of W lanes.  Groups of at
+// most 16 lanes stay within a 16-lane DPP row, so a row shift does it (one
+// VALU move; a lane whose source lies outside the row keeps its own value);
+// wider groups go through the LDS crossbar.  Callers use the value only where
+// the source lane is in the lane's own group.
+template <int W, int D, bool UP>
+__device__ __forceinline__ uint32_t lane_shift(uint32_t x) {
+  if constexpr (W <= 16)
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, (UP ? 0x110 : 0x100) + D, 0xF,
+                                                 0xF, false);
+  else
+    return UP ? (uint32_t)__shfl_up((int)x, D, W) : (uint32_t)__shfl_down((int)x, D, W);
 }
 
-__device__ __forceinline__ Ivl2 ivl_shfl_down(const Ivl2 &v, int d, int w) {
-  return Ivl2{__shfl_down(v.lo0, d, w), __shfl_down(v.hi0, d, w), __shfl_down(v.lo1, d, w),
-              __shfl_down(v.hi1, d, w)};
+template <int W, int D, bool UP>
+__device__ __forceinline__ Ivl2 ivl_shift(const Ivl2 &v) {
+  return Ivl2{lane_shift<W, D, UP>(v.lo0), lane_shift<W, D, UP>(v.hi0),
+              lane_shift<W, D, UP>(v.lo1), lane_shift<W, D, UP>(v.hi1)};
+}
+
+// Single intervals [lo, hi]: one step of the scans of the fast path, which
+// holds while every join is of two touching intervals (ok).
+__device__ __forceinline__ bool iv_touch(uint32_t alo, uint32_t ahi, uint32_t blo, uint32_t bhi) {
+  return blo <= ahi + 1u && bhi + 1u >= alo;
+}
+
+template <int W, int D>
+__device__ __forceinline__ void iv_scan_step(uint32_t &flo, uint32_t &fhi, uint32_t &glo,
+                                             uint32_t &ghi, bool &ok, int pos) {
+  if constexpr (D < W) {
+    const uint32_t olo = lane_shift<W, D, true>(flo), ohi = lane_shift<W, D, true>(fhi);
+    const uint32_t ulo = lane_shift<W, D, false>(glo), uhi = lane_shift<W, D, false>(ghi);
+    if (pos >= D) {
+      ok = ok && iv_touch(olo, ohi, flo, fhi);
+      flo = min(flo, olo);
+      fhi = max(fhi, ohi);
+    }
+    if (pos + D < W) {
+      ok = ok && iv_touch(glo, ghi, ulo, uhi);
+      glo = min(glo, ulo);
+      ghi = max(ghi, uhi);
+    }
+  }
+}
+
+// One step of the inclusive scans of the lane totals: prefix f from the
+// lane D before, suffix g from the lane D after.
+template <int W, int D>
+__device__ __forceinline__ void ivl_scan_step(Ivl2 &f, Ivl2 &g, int pos) {
+  if constexpr (D < W) {
+    const Ivl2 o = ivl_shift<W, D, true>(f);
+    if (pos >= D) f = ivl_join(o, f);
+    const Ivl2 u = ivl_shift<W, D, false>(g);
+    if (pos + D < W) g = ivl_join(g, u);
+  }
 }
 
 // Each lane takes SUMM_ROWS_PER_LANE consecutive rows: their unions are
 // joined serially into the lane's own prefixes and suffixes, the lane totals
-// are scanned across the block's `lanes` lanes (both directions), and each
+// are scanned across the block's LANES lanes (both directions), and each
 // row's summary is the join of the lanes before (after) it with its own
 // prefix (suffix).  Joins of summaries in a different order may give
 // "unknown" where another order would not, never a different interval.
-template <typename S>
+template <typename S, int LANES>
 __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t nrows,
                                                             uint2 *__restrict__ pre,
                                                             uint2 *__restrict__ suf,
-                                                            const unsigned int *__restrict__ status,
-                                                            int lanes) {
+                                                            uint4 *__restrict__ pre2,
+                                                            uint4 *__restrict__ suf2,
+                                                            const unsigned int *__restrict__ status) {
   if (*status) return;
   constexpr int K = SUMM_ROWS_PER_LANE;
-  const int pos = threadIdx.x & (lanes - 1);  // the lane's place in its block
+  const int pos = threadIdx.x & (LANES - 1);  // the lane's place in its block
   const Ivl2 none{SUMM_EMPTY, 0u, SUMM_EMPTY, 0u};
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK * K; base < nrows;
        base += (uint64_t)gridDim.x * BLOCK * K) {
     const uint64_t r0 = base + (uint64_t)threadIdx.x * K;
+    // the rows' bounds, then every row's first range, each set loaded at
+    // once (rows past the end are empty); further ranges of a row (rare)
+    // are read one by one
+    S rb[K + 1];
+#pragma unroll
+    for (int k = 0; k <= K; ++k)
+      rb[k] = r0 + k <= nrows ? B.rm[r0 + k] : (k ? rb[k - 1] : (S)0);
+    uint2 x[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      x[k] = rb[k] < rb[k + 1] ? ld_range(B.en, rb[k]) : make_uint2(SUMM_EMPTY, 0u);
+    // Fast path: every row one range and every join below one of two
+    // touching intervals, so each partial union is a single interval and
+    // the two-piece joins would give the same.  A wave with any other case
+    // takes the general path.
+    {
+      bool ok = true;
+#pragma unroll
+      for (int k = 0; k < K; ++k) ok = ok && rb[k + 1] - rb[k] == 2;
+      uint32_t plo[K], phi[K], slo[K], shi[K];
+      plo[0] = x[0].x;
+      phi[0] = x[0].y;
+#pragma unroll
+      for (int k = 1; k < K; ++k) {
+        ok = ok && iv_touch(plo[k - 1], phi[k - 1], x[k].x, x[k].y);
+        plo[k] = min(plo[k - 1], x[k].x);
+        phi[k] = max(phi[k - 1], x[k].y);
+      }
+      slo[K - 1] = x[K - 1].x;
+      shi[K - 1] = x[K - 1].y;
+#pragma unroll
+      for (int k = K - 2; k >= 0; --k) {
+        ok = ok && iv_touch(x[k].x, x[k].y, slo[k + 1], shi[k + 1]);
+        slo[k] = min(slo[k + 1], x[k].x);
+        shi[k] = max(shi[k + 1], x[k].y);
+      }
+      uint32_t flo = plo[K - 1], fhi = phi[K - 1], glo = flo, ghi = fhi;
+      iv_scan_step<LANES, 1>(flo, fhi, glo, ghi, ok, pos);
+      iv_scan_step<LANES, 2>(flo, fhi, glo, ghi, ok, pos);
+      iv_scan_step<LANES, 4>(flo, fhi, glo, ghi, ok, pos);
+      iv_scan_step<LANES, 8>(flo, fhi, glo, ghi, ok, pos);
+      iv_scan_step<LANES, 16>(flo, fhi, glo, ghi, ok, pos);
+      iv_scan_step<LANES, 32>(flo, fhi, glo, ghi, ok, pos);
+      const uint32_t blo = lane_shift<LANES, 1, true>(flo), bhi = lane_shift<LANES, 1, true>(fhi);
+      const uint32_t alo = lane_shift<LANES, 1, false>(glo), ahi = lane_shift<LANES, 1, false>(ghi);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (pos > 0) {
+          ok = ok && iv_touch(blo, bhi, plo[k], phi[k]);
+          plo[k] = min(plo[k], blo);
+          phi[k] = max(phi[k], bhi);
+        }
+        if (pos < LANES - 1) {
+          ok = ok && iv_touch(slo[k], shi[k], alo, ahi);
+          slo[k] = min(slo[k], alo);
+          shi[k] = max(shi[k], ahi);
+        }
+      }
+      if (__all(ok)) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          pre[r0 + k] = make_uint2(plo[k], phi[k]);
+          suf[r0 + k] = make_uint2(slo[k], shi[k]);
+        }
+        continue;
+      }
+    }
     Ivl2 v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      v[k] = none;
-      const uint64_t r = r0 + k;
-      if (r < nrows) {
-        const uint64_t p1 = B.rm[r + 1];
-        for (uint64_t p = B.rm[r]; p < p1 && !ivl_unknown(v[k]); p += 2) {
-          const uint2 x = ld_range(B.en, p);
-          v[k] = ivl_join(v[k], Ivl2{x.x, x.y, SUMM_EMPTY, 0u});
-        }
+      v[k] = Ivl2{x[k].x, x[k].y, SUMM_EMPTY, 0u};
+      for (S p = rb[k] + 2; p < rb[k + 1] && !ivl_unknown(v[k]); p += 2) {
+        const uint2 y = ld_range(B.en, p);
+        v[k] = ivl_join(v[k], Ivl2{y.x, y.y, SUMM_EMPTY, 0u});
       }
     }
     Ivl2 lp[K], ls[K];
@@ -460,22 +586,21 @@ __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t n
     for (int k = K - 2; k >= 0; --k) ls[k] = ivl_join(v[k], ls[k + 1]);
     // inclusive scans of the lane totals, then shifted by one lane
     Ivl2 f = lp[K - 1], g = lp[K - 1];
-#pragma unroll
-    for (int d = 1; d < lanes; d <<= 1) {
-      const Ivl2 o = ivl_shfl_up(f, d, lanes);
-      if (pos >= d) f = ivl_join(o, f);
-      const Ivl2 u = ivl_shfl_down(g, d, lanes);
-      if (pos + d < lanes) g = ivl_join(g, u);
-    }
-    Ivl2 before = ivl_shfl_up(f, 1, lanes), after = ivl_shfl_down(g, 1, lanes);
+    ivl_scan_step<LANES, 1>(f, g, pos);
+    ivl_scan_step<LANES, 2>(f, g, pos);
+    ivl_scan_step<LANES, 4>(f, g, pos);
+    ivl_scan_step<LANES, 8>(f, g, pos);
+    ivl_scan_step<LANES, 16>(f, g, pos);
+    ivl_scan_step<LANES, 32>(f, g, pos);
+    Ivl2 before = ivl_shift<LANES, 1, true>(f), after = ivl_shift<LANES, 1, false>(g);
     if (pos == 0) before = none;
-    if (pos == lanes - 1) after = none;
+    if (pos == LANES - 1) after = none;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const uint64_t r = r0 + k;
       if (r < nrows) {
-        pre[r] = ivl_enc(ivl_join(before, lp[k]));
-        suf[r] = ivl_enc(ivl_join(ls[k], after));
+        ivl_store(ivl_join(before, lp[k]), pre + r, pre2 + r);
+        ivl_store(ivl_join(ls[k], after), suf + r, suf2 + r);
       }
     }
   }
@@ -491,20 +616,25 @@ constexpr int SUMM_BATCH = 8;
 
 template <typename S>
 __device__ __forceinline__ void fold_summarised(const Mat<S> &B, const uint2 *__restrict__ pre,
-                                                const uint2 *__restrict__ suf, uint64_t s,
-                                                uint64_t e, S bs, S be, uint32_t sh,
+                                                const uint2 *__restrict__ suf,
+                                                const uint4 *__restrict__ pre2,
+                                                const uint4 *__restrict__ suf2, uint32_t s,
+                                                uint32_t e, S bs, S be, uint32_t sh,
                                                 uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
                                                 unsigned long long &gathered) {
-  const uint64_t W = 1ull << sh;  // rows per summary block
-  const uint64_t h1 = (s + W - 1) >> sh << sh;
-  const uint64_t t0 = (e + 1) >> sh << sh;
-  const uint64_t head = s < h1 ? 1 : 0, tail = t0 <= e ? 1 : 0;
-  const uint64_t nitems = head + ((t0 - h1) >> sh) + tail;
-  for (uint64_t j0 = 0; ok && j0 < nitems; j0 += SUMM_BATCH) {
+  // row indices are column indices of A, at most 0xFFFFFFFD: 32-bit math
+  const uint32_t W = 1u << sh;  // rows per summary block
+  const uint32_t h1 = (s + W - 1) >> sh << sh;
+  const uint32_t t0 = (e + 1) >> sh << sh;
+  const uint32_t head = s < h1 ? 1 : 0, tail = t0 <= e ? 1 : 0;
+  const uint32_t nitems = head + ((t0 - h1) >> sh) + tail;  // at most 64 (the caller's check)
+  uint64_t unknown = 0;  // pieces whose summary is unknown, read after the summaries
+  uint64_t two = 0;      // pieces whose summary has two pieces (in the side entries)
+  for (uint32_t j0 = 0; ok && j0 < nitems; j0 += SUMM_BATCH) {
     uint2 sm[SUMM_BATCH];
 #pragma unroll
     for (int q = 0; q < SUMM_BATCH; ++q) {
-      const uint64_t j = j0 + q;
+      const uint32_t j = j0 + q;
       sm[q] = make_uint2(SUMM_EMPTY, 0u);
       if (j < nitems) {
         if (j < head) sm[q] = suf[s];
@@ -512,26 +642,37 @@ __device__ __forceinline__ void fold_summarised(const Mat<S> &B, const uint2 *__
         else sm[q] = pre[h1 + ((j - head) << sh) + W - 1];
       }
     }
-    uint32_t unknown = 0;
 #pragma unroll
     for (int q = 0; q < SUMM_BATCH; ++q) {
       if (j0 + q >= nitems) continue;
       if (sm[q].x <= sm[q].y) {
         fold(sm[q].x, sm[q].y, cl, ch, r, ok);
         gathered += 1;
+      } else if (sm[q].x == SUMM_TWO) {
+        two |= 1ull << (j0 + q);
       } else if (sm[q].x != SUMM_EMPTY) {
-        unknown |= 1u << q;
+        unknown |= 1ull << (j0 + q);
       }
     }
-    // unknown pieces: their own ranges
-    while (ok && unknown) {
-      const uint64_t j = j0 + (uint64_t)__builtin_ctz(unknown);
-      unknown &= unknown - 1;
-      const uint64_t r0 = j < head ? s : (tail && j + 1 == nitems) ? t0 : h1 + ((j - head) << sh);
-      const uint64_t r1 = j < head ? h1 : (tail && j + 1 == nitems) ? e + 1 : r0 + W;
-      fold_segment<S, sizeof(S) == 4>(B.en, r0 == s ? bs : B.rm[r0], r1 == e + 1 ? be : B.rm[r1],
-                   cl, ch, r, ok, gathered);
-    }
+  }
+  // two-piece summaries: one more read each
+  while (ok && two) {
+    const uint32_t j = (uint32_t)__builtin_ctzll(two);
+    two &= two - 1;
+    const uint4 t = j < head ? suf2[s] : (tail && j + 1 == nitems) ? pre2[e]
+                                                                  : pre2[h1 + ((j - head) << sh) + W - 1];
+    fold(t.x, t.y, cl, ch, r, ok);
+    fold(t.z, t.w, cl, ch, r, ok);
+    gathered += 1;
+  }
+  // unknown pieces: their own ranges (the union does not depend on the order)
+  while (ok && unknown) {
+    const uint32_t j = (uint32_t)__builtin_ctzll(unknown);
+    unknown &= unknown - 1;
+    const uint32_t r0 = j < head ? s : (tail && j + 1 == nitems) ? t0 : h1 + ((j - head) << sh);
+    const uint32_t r1 = j < head ? h1 : (tail && j + 1 == nitems) ? e + 1 : r0 + W;
+    fold_segment<S, false>(B.en, r0 == s ? bs : B.rm[r0], r1 == e + 1 ? be : B.rm[r1],
+                                    cl, ch, r, ok, gathered);
   }
 }
 
@@ -542,7 +683,8 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     uint32_t *__restrict__ ovf_list, uint32_t *__restrict__ mid_list,
     uint32_t *__restrict__ big_list, OpCounters *__restrict__ ctr,
     const unsigned int *__restrict__ status, const uint2 *__restrict__ pre,
-    const uint2 *__restrict__ suf, uint32_t summ_min, uint32_t summ_shift, uint32_t plus_i,
+    const uint2 *__restrict__ suf, const uint4 *__restrict__ pre2,
+    const uint4 *__restrict__ suf2, uint32_t summ_min, uint32_t summ_shift, uint32_t plus_i,
     unsigned long long *__restrict__ parts) {
   if (*status) return;  // an earlier step overflowed: operands are not valid
   unsigned long long nnz_acc = 0, slice_acc = 0;
@@ -636,8 +778,12 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
         if (len > lane_max) ok = false;
         if (!ok) continue;  // only the slice size is still needed
         if (plus_i) fold((uint32_t)s, (uint32_t)e, cl, ch, r, ok);  // C = A (B + I)
-        if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min) {
-          fold_summarised(B, pre, suf, s, e, bs, be, summ_shift, cl, ch, r, ok, slice_acc);
+        // summaries for a range that crosses a block boundary and touches at
+        // most 64 blocks
+        if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min &&
+            (e >> summ_shift) - (s >> summ_shift) < 64) {
+          fold_summarised(B, pre, suf, pre2, suf2, ar.x, ar.y, bs, be, summ_shift, cl, ch, r, ok,
+                          slice_acc);
         } else {
           // the 64-bit summarised variant keeps 4 waves per SIMD only
           // without the fast batch
@@ -1512,14 +1658,17 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
                  const unsigned int *status, const void *summ, uint32_t summ_min,
                  uint32_t summ_rows, uint32_t plus_i, unsigned long long *parts) {
   if (!a.nrows) return;
+  // summary layout: pre, suf (uint2 per row), then pre2, suf2 (uint4 per row)
   const uint2 *pre = static_cast<const uint2 *>(summ);
   const uint2 *suf = pre ? pre + b.nrows : nullptr;
+  const uint4 *pre2 = pre ? reinterpret_cast<const uint4 *>(pre + 2 * b.nrows) : nullptr;
+  const uint4 *suf2 = pre ? pre2 + b.nrows : nullptr;
   // without summaries the kernel is built without the summary walk (fewer
   // registers, more waves per SIMD)
 #define DV_LANE(S_, M_)                                                                        \
   hipLaunchKernelGGL((k_spgemm_lane<S_, M_>), dim3(grid), dim3(BLOCK), 0, s, mat<S_>(a),       \
                      mat<S_>(b), a.nrows, stage, lane_max, mid_max, cnt, stage_buf, ovf_list, \
-                     mid_list, big_list, ctr, status, pre, suf, summ_min,                      \
+                     mid_list, big_list, ctr, status, pre, suf, pre2, suf2, summ_min,          \
                      (uint32_t)__builtin_ctz(summ_rows ? summ_rows : 1), plus_i, parts)
   if (sb == 4) {
     if (pre) DV_LANE(uint32_t, true);
@@ -1553,7 +1702,7 @@ uint32_t lane_blocks_per_cu(uint32_t sb, bool summ) {
   return (uint32_t)cache[k];
 }
 
-uint64_t summary_words(uint64_t nrows) { return 2 * nrows; }
+uint64_t summary_words(uint64_t nrows) { return 6 * nrows; }
 
 uint32_t summary_block_rows(uint64_t span) {
   uint32_t w = SUMM_ROWS_MIN;
@@ -1564,14 +1713,26 @@ uint32_t summary_block_rows(uint64_t span) {
 void block_summary(hipStream_t s, uint32_t sb, DView b, void *summ, const unsigned int *status,
                    uint32_t block_rows) {
   if (!b.nrows) return;
-  const int lanes = (int)(block_rows / SUMM_ROWS_PER_LANE);
   uint2 *pre = static_cast<uint2 *>(summ), *suf = pre + b.nrows;
-  if (sb == 4)
-    hipLaunchKernelGGL(k_interval_summary<uint32_t>, dim3(grid_for(b.nrows, BLOCK * 4)), dim3(BLOCK), 0, s,
-                       mat<uint32_t>(b), b.nrows, pre, suf, status, lanes);
-  else
-    hipLaunchKernelGGL(k_interval_summary<uint64_t>, dim3(grid_for(b.nrows, BLOCK * 4)), dim3(BLOCK), 0, s,
-                       mat<uint64_t>(b), b.nrows, pre, suf, status, lanes);
+  uint4 *pre2 = reinterpret_cast<uint4 *>(pre + 2 * b.nrows), *suf2 = pre2 + b.nrows;
+  const uint32_t grid = grid_for(b.nrows, BLOCK * SUMM_ROWS_PER_LANE);
+#define DV_SUMM(S_, L_)                                                                     \
+  hipLaunchKernelGGL((k_interval_summary<S_, L_>), dim3(grid), dim3(BLOCK), 0, s, mat<S_>(b), \
+                     b.nrows, pre, suf, pre2, suf2, status)
+#define DV_SUMM_W(S_)                        \
+  switch (block_rows / SUMM_ROWS_PER_LANE) { \
+    case 8: DV_SUMM(S_, 8); break;           \
+    case 16: DV_SUMM(S_, 16); break;         \
+    case 32: DV_SUMM(S_, 32); break;         \
+    default: DV_SUMM(S_, 64); break;         \
+  }
+  if (sb == 4) {
+    DV_SUMM_W(uint32_t);
+  } else {
+    DV_SUMM_W(uint64_t);
+  }
+#undef DV_SUMM_W
+#undef DV_SUMM
 }
 
 void spgemm_group(hipStream_t s, uint32_t sb, DView a, DView b, const GroupLaunch &g) {

@@ -92,11 +92,11 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
                  uint32_t summ_rows, uint32_t plus_i, unsigned long long *parts);
 // Interval summaries of B in blocks of block_rows rows (a power of two, 32 to
 // 256): per row, the union of the rows from its block's start to it (prefix)
-// and from it to its block's end (suffix), as one interval when known to be
-// one; summary_words(nrows) uint2 words (prefixes, then suffixes).  The lane
-// tier (summ, summ_rows = block_rows) folds an A range of at least summ_min
-// rows that crosses a block boundary as a suffix, whole blocks and a prefix:
-// 2 + span / block_rows reads instead of the rows' ranges.
+// and from it to its block's end (suffix), as its one or two pieces (else
+// "unknown"); summary_words(nrows) uint2 words (8-byte prefixes, suffixes,
+// then 16-byte two-piece side entries of each).  The lane tier (summ, summ_rows = block_rows) folds an A range of
+// at least summ_min rows that crosses a block boundary as a suffix, whole
+// blocks and a prefix: 2 + span / block_rows reads instead of the rows' ranges.
 uint64_t summary_words(uint64_t nrows);
 // Blocks of the lane-tier kernel resident per CU (its occupancy), for its
 // persistent grid.
