@@ -56,7 +56,10 @@ GroupLaunch Ctx::group_launch(uint32_t lanes, uint64_t nstage) const {
   g.lds_words = std::max<uint32_t>(1, tuning.lds_words * lanes / 64);
   g.stage = tuning.stage_entries;
   g.rows_per_group = std::max<uint32_t>(1, tuning.rows_per_group);
-  g.grid = std::max<uint32_t>(8, (num_cu * 8 + 7) / 8 * 8);
+  // one block per CU: the lists are usually short, and a larger grid costs
+  // more to launch and retire than its blocks save (rows are taken in a
+  // grid-stride loop)
+  g.grid = std::max<uint32_t>(8, (num_cu + 7) / 8 * 8);
   g.nstage = nstage;
   return g;
 }
