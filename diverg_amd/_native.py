@@ -71,6 +71,7 @@ class dv_tuning(C.Structure):
         ("graphs", C.c_uint32),
         ("summary_span", C.c_uint32),
         ("schedule", C.c_uint32),
+        ("layout", C.c_uint32),
     ]
 
 

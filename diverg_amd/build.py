@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdiverg_amd.so")
 
-SOURCES = ["rcrs_kernels.hip", "dv_api.hip", "dv_ops.hip", "dv_plan.hip", "dv_batch.hip",
+SOURCES = ["rcrs_kernels.hip", "dv_api.hip", "dv_ops.hip", "dv_plan.hip", "dv_slots.hip", "dv_batch.hip",
            "dv_server.hip", "dv_graph.hip", "vgraph_gen.cpp", "vgraph_gfa.cpp", "dv_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -401,6 +401,7 @@ int dv_finalize(dv_context *c) {
   if (c->d_status) (void)hipFree(c->d_status);
   if (c->h_plan_ctr) (void)hipHostFree(c->h_plan_ctr);
   if (c->d_plan_ctr) (void)hipFree(c->d_plan_ctr);
+  if (c->slot_parts) (void)hipFree(c->slot_parts);
   if (c->h_flag) (void)hipHostFree(c->h_flag);
   if (c->d_ctr) (void)hipFree(c->d_ctr);
   if (c->d_flag) (void)hipFree(c->d_flag);
@@ -442,6 +443,7 @@ int dv_set_tuning(dv_context *c, const dv_tuning *t) {
   if (n.graphs > 1) return C(c)->fail("graphs must be 0 or 1");
   if (n.autotune > 1) return C(c)->fail("autotune must be 0 or 1");
   if (n.schedule > 1) return C(c)->fail("schedule must be 0 or 1");
+  if (n.layout > 1) return C(c)->fail("layout must be 0 or 1");
   if (n.summary_span == 1) return C(c)->fail("summary_span must be 0 or >= 2");
   c->tuning = n;
   return DV_OK;

@@ -117,7 +117,7 @@ struct Ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t num_cu = 256;
   uint32_t lds_per_block = 65536;
-  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0, 0, 32, 0};
+  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0, 0, 32, 0, 0};
   std::string last_error;
 
   // pool accounting
@@ -142,6 +142,14 @@ struct Ctx {
   uint64_t plan_retries = 0;
   OpCounters *d_plan_ctr = nullptr, *h_plan_ctr = nullptr;
   int plan_ctr_cap = 0;
+  // slot-format builds (dv_slots.hip): lane-tier partial sums of every
+  // operation, and the predicted side-array entries per row
+  unsigned long long *slot_parts = nullptr;
+  uint64_t slot_parts_words = 0;
+  double slot_side_factor = 1.0;
+  // build shapes (rows, entries, nnz, size type, d1, d2) whose slot-form
+  // products had many rows of several ranges: built in range CRS from then on
+  std::map<std::array<uint64_t, 6>, bool> rcrs_shapes;
   std::vector<hipEvent_t> plan_ev;
   // hipGraph of the last planned build (replayed while its fingerprint holds)
   hipGraphExec_t graph_exec = nullptr;
@@ -234,6 +242,9 @@ struct Ctx {
   int power_dev(const DevMat &a, uint32_t k, DevMat &out);
   int dindex_dev(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
   int dindex_planned(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
+  // dv_slots.hip
+  int dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
+  bool slot_parts_reserve(uint64_t words);
   int autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning &best);
 
   // dv_graph.hip

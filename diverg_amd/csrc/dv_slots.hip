@@ -1,0 +1,278 @@
+// Planned index builds over slot-format intermediates.  The build's
+// operations are those of Plan::dindex (the same products in the same
+// order); the input is converted to slots once, every product reads and
+// writes slots (one uint2 per row, rows of several ranges in a side array),
+// and only the index itself is converted back to range CRS (per-row counts,
+// row-pointer scan, copy).  A product thus needs no count pass, row-pointer
+// scan or staged copy of its own.  Side arrays get a predicted capacity; a
+// product that runs out flags it on the device, and after the single
+// readback the build is repeated in exact form with a larger prediction.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "dv_runtime.hpp"
+
+namespace dv {
+
+namespace {
+SlotView view_of(const PlanSlot &s) { return SlotView{s.rowmap, s.entries}; }
+}  // namespace
+
+bool Ctx::slot_parts_reserve(uint64_t words) {
+  if (words <= slot_parts_words) return true;
+  if (slot_parts) (void)hipFree(slot_parts);
+  slot_parts = nullptr;
+  slot_parts_words = 0;
+  if (hipMalloc((void **)&slot_parts, 8 * words) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  slot_parts_words = words;
+  return true;
+}
+
+int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
+  reset_stats();
+  Plan p;
+  p.schedule = tuning.schedule;
+  const int x = p.external(a);
+  p.out = p.dindex(x, dlo, dup);
+  auto exact = [&](DevMat &o) { return dindex_exact(a, dlo, dup, o); };
+  const int nops = (int)p.ops.size();
+  // every sum must be I + X (Plan::dindex plans no other)
+  for (const PlanOp &o : p.ops)
+    if (o.kind == P_SPADD && p.slots[o.a].kind != P_IDENT && p.slots[o.b].kind != P_IDENT)
+      return plan_run(p, d, exact);
+  if (p.out == x || nops == 0) return plan_run(p, d, exact);
+
+  // records: one per operation, then the input conversion and the output scan
+  if (!plan_ctr_reserve(nops + 2)) return fail("device allocation failed (plan counters)", DV_ERR_NOMEM);
+  OpCounters *c_in = d_plan_ctr + nops, *c_out = d_plan_ctr + nops + 1;
+  uint64_t nmax = a.nrows;
+  for (const PlanOp &o : p.ops) nmax = std::max(nmax, p.slots[o.c].nrows);
+  OpScratch s;
+  if (!op_scratch(nmax, s)) return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
+  const uint32_t per_op = s.parts_cap;  // lane-tier blocks per operation at most
+  if (!slot_parts_reserve(3ull * per_op * nops))
+    return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
+
+  // buffers: slot array (8 bytes per row) in `rowmap`, side array in `entries`
+  auto release_all = [&]() {
+    for (PlanSlot &sl : p.slots)
+      if (sl.buf >= 0 && !sl.released) {
+        bufs[sl.buf].in_use = false;
+        sl.released = true;
+      }
+  };
+  auto acquire = [&](PlanSlot &sl, uint64_t side_cap) -> bool {
+    side_cap = std::min<uint64_t>(side_cap, 0xFFFFFFF0ull);  // 32-bit offsets in the slots
+    sl.cap = side_cap;
+    sl.buf = buf_acquire(8 * std::max<uint64_t>(1, sl.nrows), 2 * (side_cap + SLOT_SIDE_PAD));
+    if (sl.buf < 0) return false;
+    sl.rowmap = bufs[sl.buf].rowmap;
+    sl.entries = bufs[sl.buf].entries;
+    return true;
+  };
+  // the input: every row of several ranges takes its ranges plus a header
+  PlanSlot &in = p.slots[x];
+  const uint64_t in_ranges = a.n_entries / 2;
+  const PlanSlot in_ext = in;  // the caller's matrix (rowmap / entries)
+  in.external = false;
+  if (!acquire(in, in_ranges + in_ranges / 2 + 16)) {
+    release_all();
+    return fail("device allocation failed (plan buffers)", DV_ERR_NOMEM);
+  }
+  const double f = std::max(0.001, slot_side_factor);
+  for (int i = 0; i < nops; ++i) {
+    PlanSlot &c = p.slots[p.ops[i].c];
+    const uint64_t cap = p.ops[i].kind == P_IDENT
+                             ? 0
+                             : std::max<uint64_t>(4096, (uint64_t)(f * (double)c.nrows));
+    if (!acquire(c, cap)) {
+      release_all();
+      return fail("device allocation failed (plan buffers)", DV_ERR_NOMEM);
+    }
+    // buffers whose last reader is this operation go back to the cache
+    for (int y = 0; y < (int)p.slots.size(); ++y) {
+      PlanSlot &sl = p.slots[y];
+      if (sl.buf >= 0 && sl.last_use == i && y != p.out && !sl.released) {
+        bufs[sl.buf].in_use = false;
+        sl.released = true;
+      }
+    }
+  }
+  d = DevMat{};
+  d.nrows = a.nrows;
+  d.ncols = a.ncols;
+  d.sb = a.sb;
+  d.owned = true;
+  d.rowmap = dalloc(a.sb * (a.nrows + 1));
+  if (!d.rowmap) {
+    release_all();
+    return fail("device allocation failed (output)", DV_ERR_NOMEM);
+  }
+
+  // ---- enqueue
+  const bool timing = prof && plan_events_reserve(nops);
+  fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
+           (uint32_t)((nops + 2) * sizeof(OpCounters) / 4), 0u);
+  fill_u32(stream, d_status, 1, 0u);
+  fill_u32(stream, reinterpret_cast<uint32_t *>(slot_parts), 6ull * per_op * nops, 0u);
+  slots_from_rcrs(stream, a.sb, view(a), in.rowmap, in.entries, in.cap, c_in, d_status);
+  for (int i = 0; i < nops; ++i) {
+    const PlanOp &o = p.ops[i];
+    PlanSlot &c = p.slots[o.c];
+    OpCounters *ctr = d_plan_ctr + i;
+    if (timing) (void)hipEventRecord(plan_ev[5 * i], stream);
+    if (o.kind == P_IDENT) {
+      slots_identity(stream, c.nrows, c.rowmap);
+      if (timing)
+        for (int e = 1; e < 5; ++e) (void)hipEventRecord(plan_ev[5 * i + e], stream);
+      continue;
+    }
+    if (o.kind == P_SPADD) {
+      const PlanSlot &xs = p.slots[p.slots[o.a].kind == P_IDENT ? o.b : o.a];
+      slots_plus_identity(stream, view_of(xs), c.nrows, c.rowmap, c.entries, c.cap, ctr, d_status,
+                          slot_parts + 3ull * per_op * i, per_op);
+      if (timing)
+        for (int e = 1; e < 5; ++e) (void)hipEventRecord(plan_ev[5 * i + e], stream);
+      continue;
+    }
+    const PlanSlot &A = p.slots[o.a], &B = p.slots[o.b];
+    uint32_t summ_rows = tuning.summary_span >= 2 && A.span >= tuning.summary_span
+                             ? summary_block_rows(A.span)
+                             : 0u;
+    if (B.nrows > s.rows) summ_rows = 0;
+    if (summ_rows) slots_summary(stream, view_of(B), B.nrows, s.summ, d_status, summ_rows);
+    if (timing) (void)hipEventRecord(plan_ev[5 * i + 1], stream);
+    const uint64_t n = c.nrows;
+    const uint64_t resident = (uint64_t)num_cu * slot_lane_blocks_per_cu(summ_rows != 0);
+    const uint32_t grid =
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(n + 255) / 256, resident, per_op}));
+    slots_lane(stream, view_of(A), view_of(B), n, B.nrows, tuning.lane_slice_max,
+               tuning.small_slice_max, grid, c.rowmap, c.entries, c.cap, s.mid, s.big, ctr,
+               d_status, summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows,
+               o.plus_i ? 1u : 0u, slot_parts + 3ull * per_op * i);
+    if (timing) (void)hipEventRecord(plan_ev[5 * i + 2], stream);
+    const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);
+    slots_group(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.rows_per_group, gm.grid,
+                s.mid, &ctr->n_mid, c.rowmap, c.entries, c.cap, ctr, d_status, o.plus_i ? 1u : 0u);
+    slots_group(stream, view_of(A), view_of(B), 64, gb.lds_words, gb.rows_per_group, gb.grid, s.big,
+                &ctr->n_big, c.rowmap, c.entries, c.cap, ctr, d_status, o.plus_i ? 1u : 0u);
+    if (timing) {
+      (void)hipEventRecord(plan_ev[5 * i + 3], stream);
+      (void)hipEventRecord(plan_ev[5 * i + 4], stream);
+    }
+  }
+  slots_reduce_parts(stream, slot_parts, per_op, (uint32_t)nops, d_plan_ctr);
+  const PlanSlot &O = p.slots[p.out];
+  slots_counts(stream, view_of(O), O.nrows, s.cnt, d_status);
+  scan_rowmap(stream, d.sb, s.cnt, O.nrows, d.rowmap, s.tmp, c_out, ~0ull, d_status, nullptr, 0,
+              nullptr);
+
+  // ---- one readback
+  bool ok = hip(hipGetLastError(), "slot plan launch") &&
+            hip(hipMemcpyAsync(h_plan_ctr, d_plan_ctr, (nops + 2) * sizeof(OpCounters),
+                               hipMemcpyDeviceToHost, stream),
+                "plan readback") &&
+            hip(hipMemcpyAsync(h_status, d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
+                "plan status") &&
+            hip(hipStreamSynchronize(stream), "plan synchronize");
+  if (!ok) {
+    release_all();
+    release(d);
+    return DV_ERR_DEVICE;
+  }
+  // plan_mode 2 always takes the fallback (tests)
+  const uint32_t st = *h_status | (tuning.plan_mode == 2 ? 1u : 0u);
+  // side use per row, for the next prediction
+  double use = 0;
+  for (int i = 0; i < nops; ++i)
+    if (p.slots[p.ops[i].c].nrows)
+      use = std::max(use, (double)h_plan_ctr[i].total / (double)p.slots[p.ops[i].c].nrows);
+  if (st & 2) {
+    release_all();
+    release(d);
+    return fail("entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
+  }
+  if (st & 1) {
+    release_all();
+    release(d);
+    plan_retries += 1;
+    const int rc = exact(d);
+    stats_retried = 1;
+    if (!rc && tuning.plan_mode != 2) slot_side_factor = std::max(2.0 * f, 1.5 * use);
+    return rc;
+  }
+  // ---- the index's entries
+  const uint64_t total = h_plan_ctr[nops + 1].total;
+  d.n_entries = total;
+  d.entries = static_cast<uint32_t *>(dalloc(4 * std::max<uint64_t>(1, total)));
+  if (!d.entries) {
+    release_all();
+    release(d);
+    return fail("device allocation failed (output)", DV_ERR_NOMEM);
+  }
+  slots_emit(stream, view_of(O), O.nrows, d.sb, d.rowmap, d.entries);
+  const PlanOp &last = p.ops[O.producer];
+  d.nnz = last.kind == P_IDENT ? O.nrows : h_plan_ctr[O.producer].nnz;
+  if (!hip(hipGetLastError(), "slot plan output")) {
+    release_all();
+    release(d);
+    return DV_ERR_DEVICE;
+  }
+
+  // ---- statistics (bytes(M) = sizeof(S) (n + 1) + 8 ranges(M))
+  std::vector<uint64_t> ranges(p.slots.size(), 0);
+  ranges[x] = in_ranges;
+  for (int i = 0; i < nops; ++i)
+    ranges[p.ops[i].c] = p.ops[i].kind == P_IDENT ? p.slots[p.ops[i].c].nrows : h_plan_ctr[i].ranges;
+  auto bytes = [&](int sl) { return (uint64_t)p.slots[sl].sb * (p.slots[sl].nrows + 1) + 8 * ranges[sl]; };
+  for (int i = 0; i < nops; ++i) {
+    const PlanOp &o = p.ops[i];
+    if (o.kind == P_IDENT) continue;
+    const OpCounters &h = h_plan_ctr[i];
+    const uint64_t alg = bytes(o.a) + bytes(o.b) + bytes(o.c);
+    stats_alg_bytes += alg;
+    if (o.kind == P_SPGEMM) {
+      stats_n_spgemm += 1;
+      stats_mid_rows += h.n_mid;
+      stats_big_rows += h.n_big;
+      if (p.slots[o.c].nrows)
+        stats_max_ratio = std::max(stats_max_ratio, (double)ranges[o.c] / (double)p.slots[o.c].nrows);
+    } else {
+      stats_n_spadd += 1;
+    }
+    if (timing) {
+      float t[4] = {0, 0, 0, 0};
+      for (int e = 0; e < 4; ++e)
+        (void)hipEventElapsedTime(&t[e], plan_ev[5 * i + e], plan_ev[5 * i + e + 1]);
+      if (o.kind == P_SPGEMM) {
+        prof_t[0].launches += 1;
+        prof_t[0].total_ms += t[1];
+        prof_t[0].alg_bytes += alg;
+        prof_t[0].slice_bytes += 8ull * h.slice;
+        prof_t[2].launches += 1;
+        prof_t[2].total_ms += t[0] + t[2];
+      } else {
+        prof_t[3].launches += 1;
+        prof_t[3].total_ms += t[0] + t[1] + t[2] + t[3];
+        prof_t[3].alg_bytes += alg;
+      }
+    }
+  }
+  stats_planned = 1;
+  release_all();
+  // Rows of several ranges cost the slot form a dependent read each (the
+  // side array), where range CRS reads them with the rest of a slice: when
+  // they become common (a product with more side entries than SIDE_SHARE of
+  // its rows), later builds of this shape use range CRS.
+  constexpr double SIDE_SHARE = 0.05;
+  if (use > SIDE_SHARE) rcrs_shapes[{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup}] = true;
+  (void)in_ext;
+  return DV_OK;
+}
+
+}  // namespace dv

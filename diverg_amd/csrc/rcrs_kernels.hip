@@ -487,6 +487,105 @@ __device__ __forceinline__ void ivl_scan_step(Ivl2 &f, Ivl2 &g, int pos) {
 // row's summary is the join of the lanes before (after) it with its own
 // prefix (suffix).  Joins of summaries in a different order may give
 // "unknown" where another order would not, never a different interval.
+// Summaries of one lane's SUMM_ROWS_PER_LANE rows r0 .. r0+K-1: x[k] is row
+// k's first range ((EMPTY, 0) for an empty row), `single` says whether every
+// row is exactly one range, and more(k, v) joins row k's further ranges
+// into v (only called on the general path).
+template <int LANES, typename More>
+__device__ __forceinline__ void summary_lane(const uint2 (&x)[SUMM_ROWS_PER_LANE], bool single,
+                                             More more, uint64_t r0, uint64_t nrows, int pos,
+                                             uint2 *__restrict__ pre, uint2 *__restrict__ suf,
+                                             uint4 *__restrict__ pre2, uint4 *__restrict__ suf2) {
+  constexpr int K = SUMM_ROWS_PER_LANE;
+  const Ivl2 none{SUMM_EMPTY, 0u, SUMM_EMPTY, 0u};
+  // Fast path: every row one range and every join below one of two
+  // touching intervals, so each partial union is a single interval and
+  // the two-piece joins would give the same.  A wave with any other case
+  // takes the general path.
+  {
+    bool ok = single;
+    uint32_t plo[K], phi[K], slo[K], shi[K];
+    plo[0] = x[0].x;
+    phi[0] = x[0].y;
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+      ok = ok && iv_touch(plo[k - 1], phi[k - 1], x[k].x, x[k].y);
+      plo[k] = min(plo[k - 1], x[k].x);
+      phi[k] = max(phi[k - 1], x[k].y);
+    }
+    slo[K - 1] = x[K - 1].x;
+    shi[K - 1] = x[K - 1].y;
+#pragma unroll
+    for (int k = K - 2; k >= 0; --k) {
+      ok = ok && iv_touch(x[k].x, x[k].y, slo[k + 1], shi[k + 1]);
+      slo[k] = min(slo[k + 1], x[k].x);
+      shi[k] = max(shi[k + 1], x[k].y);
+    }
+    uint32_t flo = plo[K - 1], fhi = phi[K - 1], glo = flo, ghi = fhi;
+    iv_scan_step<LANES, 1>(flo, fhi, glo, ghi, ok, pos);
+    iv_scan_step<LANES, 2>(flo, fhi, glo, ghi, ok, pos);
+    iv_scan_step<LANES, 4>(flo, fhi, glo, ghi, ok, pos);
+    iv_scan_step<LANES, 8>(flo, fhi, glo, ghi, ok, pos);
+    iv_scan_step<LANES, 16>(flo, fhi, glo, ghi, ok, pos);
+    iv_scan_step<LANES, 32>(flo, fhi, glo, ghi, ok, pos);
+    const uint32_t blo = lane_shift<LANES, 1, true>(flo), bhi = lane_shift<LANES, 1, true>(fhi);
+    const uint32_t alo = lane_shift<LANES, 1, false>(glo), ahi = lane_shift<LANES, 1, false>(ghi);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (pos > 0) {
+        ok = ok && iv_touch(blo, bhi, plo[k], phi[k]);
+        plo[k] = min(plo[k], blo);
+        phi[k] = max(phi[k], bhi);
+      }
+      if (pos < LANES - 1) {
+        ok = ok && iv_touch(slo[k], shi[k], alo, ahi);
+        slo[k] = min(slo[k], alo);
+        shi[k] = max(shi[k], ahi);
+      }
+    }
+    if (__all(ok)) {  // (every row is then below nrows: rows past it are empty)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        pre[r0 + k] = make_uint2(plo[k], phi[k]);
+        suf[r0 + k] = make_uint2(slo[k], shi[k]);
+      }
+      return;
+    }
+  }
+  Ivl2 v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    v[k] = Ivl2{x[k].x, x[k].y, SUMM_EMPTY, 0u};
+    more(k, v[k]);
+  }
+  Ivl2 lp[K], ls[K];
+  lp[0] = v[0];
+#pragma unroll
+  for (int k = 1; k < K; ++k) lp[k] = ivl_join(lp[k - 1], v[k]);
+  ls[K - 1] = v[K - 1];
+#pragma unroll
+  for (int k = K - 2; k >= 0; --k) ls[k] = ivl_join(v[k], ls[k + 1]);
+  // inclusive scans of the lane totals, then shifted by one lane
+  Ivl2 f = lp[K - 1], g = lp[K - 1];
+  ivl_scan_step<LANES, 1>(f, g, pos);
+  ivl_scan_step<LANES, 2>(f, g, pos);
+  ivl_scan_step<LANES, 4>(f, g, pos);
+  ivl_scan_step<LANES, 8>(f, g, pos);
+  ivl_scan_step<LANES, 16>(f, g, pos);
+  ivl_scan_step<LANES, 32>(f, g, pos);
+  Ivl2 before = ivl_shift<LANES, 1, true>(f), after = ivl_shift<LANES, 1, false>(g);
+  if (pos == 0) before = none;
+  if (pos == LANES - 1) after = none;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint64_t r = r0 + k;
+    if (r < nrows) {
+      ivl_store(ivl_join(before, lp[k]), pre + r, pre2 + r);
+      ivl_store(ivl_join(ls[k], after), suf + r, suf2 + r);
+    }
+  }
+}
+
 template <typename S, int LANES>
 __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t nrows,
                                                             uint2 *__restrict__ pre,
@@ -497,7 +596,6 @@ __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t n
   if (*status) return;
   constexpr int K = SUMM_ROWS_PER_LANE;
   const int pos = threadIdx.x & (LANES - 1);  // the lane's place in its block
-  const Ivl2 none{SUMM_EMPTY, 0u, SUMM_EMPTY, 0u};
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK * K; base < nrows;
        base += (uint64_t)gridDim.x * BLOCK * K) {
     const uint64_t r0 = base + (uint64_t)threadIdx.x * K;
@@ -509,100 +607,21 @@ __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t n
     for (int k = 0; k <= K; ++k)
       rb[k] = r0 + k <= nrows ? B.rm[r0 + k] : (k ? rb[k - 1] : (S)0);
     uint2 x[K];
+    bool single = true;
 #pragma unroll
-    for (int k = 0; k < K; ++k)
+    for (int k = 0; k < K; ++k) {
       x[k] = rb[k] < rb[k + 1] ? ld_range(B.en, rb[k]) : make_uint2(SUMM_EMPTY, 0u);
-    // Fast path: every row one range and every join below one of two
-    // touching intervals, so each partial union is a single interval and
-    // the two-piece joins would give the same.  A wave with any other case
-    // takes the general path.
-    {
-      bool ok = true;
-#pragma unroll
-      for (int k = 0; k < K; ++k) ok = ok && rb[k + 1] - rb[k] == 2;
-      uint32_t plo[K], phi[K], slo[K], shi[K];
-      plo[0] = x[0].x;
-      phi[0] = x[0].y;
-#pragma unroll
-      for (int k = 1; k < K; ++k) {
-        ok = ok && iv_touch(plo[k - 1], phi[k - 1], x[k].x, x[k].y);
-        plo[k] = min(plo[k - 1], x[k].x);
-        phi[k] = max(phi[k - 1], x[k].y);
-      }
-      slo[K - 1] = x[K - 1].x;
-      shi[K - 1] = x[K - 1].y;
-#pragma unroll
-      for (int k = K - 2; k >= 0; --k) {
-        ok = ok && iv_touch(x[k].x, x[k].y, slo[k + 1], shi[k + 1]);
-        slo[k] = min(slo[k + 1], x[k].x);
-        shi[k] = max(shi[k + 1], x[k].y);
-      }
-      uint32_t flo = plo[K - 1], fhi = phi[K - 1], glo = flo, ghi = fhi;
-      iv_scan_step<LANES, 1>(flo, fhi, glo, ghi, ok, pos);
-      iv_scan_step<LANES, 2>(flo, fhi, glo, ghi, ok, pos);
-      iv_scan_step<LANES, 4>(flo, fhi, glo, ghi, ok, pos);
-      iv_scan_step<LANES, 8>(flo, fhi, glo, ghi, ok, pos);
-      iv_scan_step<LANES, 16>(flo, fhi, glo, ghi, ok, pos);
-      iv_scan_step<LANES, 32>(flo, fhi, glo, ghi, ok, pos);
-      const uint32_t blo = lane_shift<LANES, 1, true>(flo), bhi = lane_shift<LANES, 1, true>(fhi);
-      const uint32_t alo = lane_shift<LANES, 1, false>(glo), ahi = lane_shift<LANES, 1, false>(ghi);
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        if (pos > 0) {
-          ok = ok && iv_touch(blo, bhi, plo[k], phi[k]);
-          plo[k] = min(plo[k], blo);
-          phi[k] = max(phi[k], bhi);
-        }
-        if (pos < LANES - 1) {
-          ok = ok && iv_touch(slo[k], shi[k], alo, ahi);
-          slo[k] = min(slo[k], alo);
-          shi[k] = max(shi[k], ahi);
-        }
-      }
-      if (__all(ok)) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          pre[r0 + k] = make_uint2(plo[k], phi[k]);
-          suf[r0 + k] = make_uint2(slo[k], shi[k]);
-        }
-        continue;
-      }
+      single = single && rb[k + 1] - rb[k] == 2;
     }
-    Ivl2 v[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      v[k] = Ivl2{x[k].x, x[k].y, SUMM_EMPTY, 0u};
-      for (S p = rb[k] + 2; p < rb[k + 1] && !ivl_unknown(v[k]); p += 2) {
-        const uint2 y = ld_range(B.en, p);
-        v[k] = ivl_join(v[k], Ivl2{y.x, y.y, SUMM_EMPTY, 0u});
-      }
-    }
-    Ivl2 lp[K], ls[K];
-    lp[0] = v[0];
-#pragma unroll
-    for (int k = 1; k < K; ++k) lp[k] = ivl_join(lp[k - 1], v[k]);
-    ls[K - 1] = v[K - 1];
-#pragma unroll
-    for (int k = K - 2; k >= 0; --k) ls[k] = ivl_join(v[k], ls[k + 1]);
-    // inclusive scans of the lane totals, then shifted by one lane
-    Ivl2 f = lp[K - 1], g = lp[K - 1];
-    ivl_scan_step<LANES, 1>(f, g, pos);
-    ivl_scan_step<LANES, 2>(f, g, pos);
-    ivl_scan_step<LANES, 4>(f, g, pos);
-    ivl_scan_step<LANES, 8>(f, g, pos);
-    ivl_scan_step<LANES, 16>(f, g, pos);
-    ivl_scan_step<LANES, 32>(f, g, pos);
-    Ivl2 before = ivl_shift<LANES, 1, true>(f), after = ivl_shift<LANES, 1, false>(g);
-    if (pos == 0) before = none;
-    if (pos == LANES - 1) after = none;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint64_t r = r0 + k;
-      if (r < nrows) {
-        ivl_store(ivl_join(before, lp[k]), pre + r, pre2 + r);
-        ivl_store(ivl_join(ls[k], after), suf + r, suf2 + r);
-      }
-    }
+    summary_lane<LANES>(
+        x, single,
+        [&](int k, Ivl2 &v) {
+          for (S p = rb[k] + 2; p < rb[k + 1] && !ivl_unknown(v); p += 2) {
+            const uint2 y = ld_range(B.en, p);
+            v = ivl_join(v, Ivl2{y.x, y.y, SUMM_EMPTY, 0u});
+          }
+        },
+        r0, nrows, pos, pre, suf, pre2, suf2);
   }
 }
 
@@ -614,12 +633,13 @@ __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t n
 // back to that piece's ranges.
 constexpr int SUMM_BATCH = 8;
 
-template <typename S>
-__device__ __forceinline__ void fold_summarised(const Mat<S> &B, const uint2 *__restrict__ pre,
+// rows(r0, r1) folds B rows [r0, r1) from their ranges (the fallback).
+template <typename Rows>
+__device__ __forceinline__ void fold_summarised(const uint2 *__restrict__ pre,
                                                 const uint2 *__restrict__ suf,
                                                 const uint4 *__restrict__ pre2,
                                                 const uint4 *__restrict__ suf2, uint32_t s,
-                                                uint32_t e, S bs, S be, uint32_t sh,
+                                                uint32_t e, uint32_t sh, Rows rows,
                                                 uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
                                                 unsigned long long &gathered) {
   // row indices are column indices of A, at most 0xFFFFFFFD: 32-bit math
@@ -671,8 +691,7 @@ __device__ __forceinline__ void fold_summarised(const Mat<S> &B, const uint2 *__
     unknown &= unknown - 1;
     const uint32_t r0 = j < head ? s : (tail && j + 1 == nitems) ? t0 : h1 + ((j - head) << sh);
     const uint32_t r1 = j < head ? h1 : (tail && j + 1 == nitems) ? e + 1 : r0 + W;
-    fold_segment<S, false>(B.en, r0 == s ? bs : B.rm[r0], r1 == e + 1 ? be : B.rm[r1],
-                                    cl, ch, r, ok, gathered);
+    rows(r0, r1);
   }
 }
 
@@ -782,8 +801,13 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
         // most 64 blocks
         if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min &&
             (e >> summ_shift) - (s >> summ_shift) < 64) {
-          fold_summarised(B, pre, suf, pre2, suf2, ar.x, ar.y, bs, be, summ_shift, cl, ch, r, ok,
-                          slice_acc);
+          fold_summarised(
+              pre, suf, pre2, suf2, ar.x, ar.y, summ_shift,
+              [&](uint32_t r0, uint32_t r1) {
+                fold_segment<S, false>(B.en, r0 == ar.x ? bs : B.rm[r0],
+                                       r1 == ar.y + 1 ? be : B.rm[r1], cl, ch, r, ok, slice_acc);
+              },
+              cl, ch, r, ok, slice_acc);
         } else {
           // the 64-bit summarised variant keeps 4 waves per SIMD only
           // without the fast batch
@@ -1564,6 +1588,619 @@ void group_by_lanes(hipStream_t s, DView a, DView b, const GroupLaunch &g) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Slot format: the intermediate matrices of planned index builds.  One uint2
+// per row: (lo, hi) with lo <= hi for a row of that one range, (SLOT_EMPTY, 0)
+// for an empty row, (SLOT_MARK, off) for a row of two or more ranges kept in
+// the side array: side[off] = (count, 0), then its `count` ranges.  Nearly
+// every row of a power of a banded matrix is one range, so a product reads a
+// B slice as consecutive slots (no row pointers) and writes each row in place
+// (no count pass, row-pointer scan or copy).  Side arrays are allocated with
+// SLOT_SIDE_PAD spare entries, so a reader may load a marked row's header and
+// a few entries after it at once.
+constexpr uint32_t SLOT_EMPTY = 0xFFFFFFFFu, SLOT_MARK = 0xFFFFFFFEu;
+
+
+struct Slots {
+  const uint2 *__restrict__ slot;
+  const uint2 *__restrict__ side;
+};
+
+// Reserve `need` entries of a side array for each lane (0: none) with one
+// atomic per wave; returns the lane's first entry.  Every lane of the wave
+// must call it.
+__device__ __forceinline__ unsigned long long wave_alloc(uint32_t need,
+                                                         unsigned long long *counter) {
+  const unsigned long long m = __ballot(need != 0);
+  if (m == 0) return 0;
+  const uint32_t lane = __lane_id();
+  uint32_t incl = need;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)incl, d, 64);
+    if (lane >= (uint32_t)d) incl += t;
+  }
+  const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+  const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)total);
+  const uint32_t blo = (uint32_t)__shfl((int)(uint32_t)base, (int)leader, 64);
+  const uint32_t bhi = (uint32_t)__shfl((int)(uint32_t)(base >> 32), (int)leader, 64);
+  return (((unsigned long long)bhi << 32) | blo) + incl - need;
+}
+
+// Fold the ranges of a marked row (side entry `off`): the header and the
+// first SIDE_BATCH ranges in one batch, any further ones after.
+constexpr int SIDE_BATCH = 4;
+static_assert(SLOT_SIDE_PAD >= SIDE_BATCH, "side padding covers a batch");
+
+__device__ __forceinline__ void fold_side(const uint2 *__restrict__ side, uint32_t off,
+                                          uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
+                                          unsigned long long &gathered) {
+  uint2 v[SIDE_BATCH];
+  const uint2 h = side[off];
+#pragma unroll
+  for (int k = 0; k < SIDE_BATCH; ++k) v[k] = side[off + 1 + k];
+  const uint32_t cnt = h.x;
+  gathered += cnt;
+#pragma unroll
+  for (int k = 0; k < SIDE_BATCH; ++k)
+    if ((uint32_t)k < cnt) fold(v[k].x, v[k].y, cl, ch, r, ok);
+  for (uint32_t q0 = SIDE_BATCH; ok && q0 < cnt; q0 += SIDE_BATCH) {
+#pragma unroll
+    for (int k = 0; k < SIDE_BATCH; ++k)
+      v[k] = q0 + k < cnt ? side[off + 1 + q0 + k] : make_uint2(1u, 0u);
+#pragma unroll
+    for (int k = 0; k < SIDE_BATCH; ++k) fold(v[k].x, v[k].y, cl, ch, r, ok);
+  }
+}
+
+// Fold B rows [s, e] (slots) with LANE_BATCH loads in flight; marked rows'
+// ranges are read after their batch.
+__device__ __forceinline__ void fold_slots(const Slots &B, uint32_t s, uint32_t e, uint32_t &cl,
+                                           uint32_t &ch, Runs &r, bool &ok,
+                                           unsigned long long &gathered) {
+  gathered += (unsigned long long)(e - s) + 1;
+  for (uint32_t j = s; ok && j <= e;) {
+    const uint32_t rem = e - j;  // rows after j in the slice (no index past e is formed)
+    uint2 v[LANE_BATCH];
+#pragma unroll
+    for (int k = 0; k < LANE_BATCH; ++k)
+      v[k] = (uint32_t)k <= rem ? B.slot[j + k] : make_uint2(SLOT_EMPTY, 0u);
+    fold_batch(v, cl, ch, r, ok);  // markers and empty rows are not ranges: skipped
+    uint32_t mk = 0, offs[LANE_BATCH];
+#pragma unroll
+    for (int k = 0; k < LANE_BATCH; ++k) {
+      mk |= v[k].x == SLOT_MARK ? 1u << k : 0u;
+      offs[k] = v[k].y;
+    }
+    while (ok && mk) {
+      const uint32_t k = (uint32_t)__builtin_ctz(mk);
+      mk &= mk - 1;
+      uint32_t off = offs[0];
+#pragma unroll
+      for (int q = 1; q < LANE_BATCH; ++q) off = k == (uint32_t)q ? offs[q] : off;
+      fold_side(B.side, off, cl, ch, r, ok, gathered);
+    }
+    j = rem >= (uint32_t)LANE_BATCH ? j + LANE_BATCH : e + 1;
+    if (j == 0) break;  // (e + 1 wrapped: e was the last ordinal)
+  }
+}
+
+// Write a lane-tier row (slot) from its compacted runs: one range in place,
+// several in the side array (reserved by the caller: off, `need` entries).
+__device__ __forceinline__ void slot_store_runs(const Runs &r, uint32_t nruns, uint64_t row,
+                                                uint2 *__restrict__ c_slot,
+                                                uint2 *__restrict__ c_side,
+                                                unsigned long long off) {
+  if (nruns == 0) {
+    c_slot[row] = make_uint2(SLOT_EMPTY, 0u);
+  } else if (nruns == 1) {
+    c_slot[row] = make_uint2(r.lo[0], r.hi[0]);
+  } else {
+    c_side[off] = make_uint2(nruns, 0u);
+#pragma unroll
+    for (int t = 0; t < LANE_SLOTS; ++t)
+      if ((uint32_t)t < nruns) c_side[off + 1 + t] = make_uint2(r.lo[t], r.hi[t]);
+    c_slot[row] = make_uint2(SLOT_MARK, (uint32_t)off);
+  }
+}
+
+// Lane tier on slots: C = A.B (or A (B + I)), one lane per row of A, rows
+// whose union has more than LANE_SLOTS runs or whose slices are long go to
+// the group lists as in k_spgemm_lane.  A two-stage pipeline loads the A
+// slot of the row after next while this row folds.
+template <bool SUMM>
+__global__ __launch_bounds__(BLOCK) void k_slot_lane(
+    Slots A, Slots B, uint64_t nrows, uint32_t lane_max, uint32_t mid_max,
+    uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side, unsigned long long side_cap,
+    uint32_t *__restrict__ mid_list, uint32_t *__restrict__ big_list,
+    OpCounters *__restrict__ ctr, unsigned int *__restrict__ status,
+    const uint2 *__restrict__ pre, const uint2 *__restrict__ suf,
+    const uint4 *__restrict__ pre2, const uint4 *__restrict__ suf2, uint32_t summ_min,
+    uint32_t summ_shift, uint32_t plus_i, unsigned long long *__restrict__ parts) {
+  if (*status) return;
+  unsigned long long nnz_acc = 0, slice_acc = 0, ranges_acc = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint2 c_a = make_uint2(SLOT_EMPTY, 0u), n_a = make_uint2(SLOT_EMPTY, 0u);
+  if (row < nrows) c_a = A.slot[row];
+  if (row + stride < nrows) n_a = A.slot[row + stride];
+  need(c_a.x);
+  need(c_a.y);
+  need(n_a.x);
+  need(n_a.y);
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows; base += stride) {
+    row = base + threadIdx.x;
+    const uint2 a = c_a;
+    c_a = n_a;
+    n_a = make_uint2(SLOT_EMPTY, 0u);
+    if (row + 2 * stride < nrows) n_a = A.slot[row + 2 * stride];
+    int cls = 0;
+    const bool live = row < nrows;
+    uint64_t len = 0;
+    bool ok = true;
+    Runs r;
+#pragma unroll
+    for (int t = 0; t < LANE_SLOTS; ++t) {
+      r.lo[t] = 0xFFFFFFFFu;
+      r.hi[t] = 0u;
+    }
+    uint32_t cl = 0xFFFFFFFFu, ch = 0u;
+    // A ranges: the slot itself, or the side entries of a marked row
+    uint32_t na = 0;
+    if (live) {
+      if (a.x <= a.y) na = 1;
+      else if (a.x == SLOT_MARK) na = A.side[a.y].x;
+    }
+    for (uint32_t q = 0; q < na; ++q) {
+      const uint2 ar = a.x <= a.y ? a : A.side[a.y + 1 + q];
+      const uint32_t s = ar.x, e = ar.y;
+      len += (uint64_t)(e - s) + 1;
+      if (len > lane_max) ok = false;
+      if (!ok) continue;
+      if (plus_i) fold(s, e, cl, ch, r, ok);  // C = A (B + I)
+      if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min &&
+          (e >> summ_shift) - (s >> summ_shift) < 64) {
+        fold_summarised(
+            pre, suf, pre2, suf2, s, e, summ_shift,
+            [&](uint32_t r0, uint32_t r1) { fold_slots(B, r0, r1 - 1, cl, ch, r, ok, slice_acc); },
+            cl, ch, r, ok, slice_acc);
+      } else {
+        fold_slots(B, s, e, cl, ch, r, ok, slice_acc);
+      }
+    }
+    need(c_a.x);
+    need(c_a.y);
+    need(n_a.x);
+    need(n_a.y);
+    uint32_t nruns = 0;
+    if (live) {
+      if (ok && cl <= ch && !insert(r, cl, ch)) {
+        compact(r);
+        ok = insert(r, cl, ch);
+      }
+      if (!ok) {
+        cls = len <= mid_max ? 1 : 2;
+      } else {
+        compact(r);
+#pragma unroll
+        for (int t = 0; t < LANE_SLOTS; ++t)
+          if (r.lo[t] <= r.hi[t]) {
+            nnz_acc += (unsigned long long)(r.hi[t] - r.lo[t]) + 1;
+            ++nruns;
+          }
+        ranges_acc += nruns;
+      }
+    }
+    const uint32_t need_side = live && ok && nruns > 1 ? nruns + 1 : 0u;
+    const unsigned long long off = wave_alloc(need_side, &ctr->total);
+    if (live && ok) {
+      if (need_side && off + need_side > side_cap) {
+        atomicOr(status, 1u);  // side array full: the build falls back
+      } else {
+        slot_store_runs(r, nruns, row, c_slot, c_side, off);
+      }
+    }
+    wave_append(cls == 1, (uint32_t)row, mid_list, &ctr->n_mid);
+    wave_append(cls == 2, (uint32_t)row, big_list, &ctr->n_big);
+  }
+  nnz_acc = block_sum_u64(nnz_acc);
+  slice_acc = block_sum_u64(slice_acc);
+  ranges_acc = block_sum_u64(ranges_acc);
+  if (threadIdx.x == 0) {
+    parts[3 * blockIdx.x] = nnz_acc;
+    parts[3 * blockIdx.x + 1] = slice_acc;
+    parts[3 * blockIdx.x + 2] = ranges_acc;
+  }
+}
+
+// Group tiers on slots: one row of C by G lanes with an LDS bit window, as
+// spgemm_row; the row is counted first, then (several runs) its side entries
+// are reserved and written in a second pass over the window.
+template <int G>
+__device__ __forceinline__ void slot_row(uint64_t row, const Slots &A, const Slots &B,
+                                         uint64_t *lds, uint32_t W, uint32_t lane,
+                                         uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side,
+                                         unsigned long long side_cap, OpCounters *ctr,
+                                         unsigned int *status, unsigned long long &nnz_acc,
+                                         unsigned long long &ranges_acc, uint32_t plus_i) {
+  const uint2 a = A.slot[row];
+  uint32_t na = 0, aoff = 0;  // A ranges: the slot itself, or side[aoff + 1 ..]
+  if (a.x <= a.y) {
+    na = 1;
+  } else if (a.x == SLOT_MARK) {
+    aoff = a.y;
+    na = A.side[aoff].x;
+  }
+  auto arange = [&](uint32_t q) { return na == 1 && a.x <= a.y ? a : A.side[aoff + 1 + q]; };
+  // visit(x) for every range of B rows [s, e], rows strided over the group
+  auto each_b = [&](uint32_t s, uint32_t e, auto &&visit) {
+    for (uint64_t j = (uint64_t)s + lane; j <= e; j += G) {
+      const uint2 b = B.slot[j];
+      if (b.x <= b.y) {
+        visit(b);
+      } else if (b.x == SLOT_MARK) {
+        const uint32_t nb = B.side[b.y].x;
+        for (uint32_t q = 0; q < nb; ++q) visit(B.side[b.y + 1 + q]);
+      }
+    }
+  };
+  // 1. extent
+  uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+  for (uint32_t q = 0; q < na; ++q) {
+    const uint2 ar = arange(q);
+    each_b(ar.x, ar.y, [&](uint2 x) {
+      lo = min(lo, x.x);
+      hi = max(hi, x.y);
+    });
+    if (plus_i) {
+      lo = min(lo, ar.x);
+      hi = max(hi, ar.y);
+    }
+  }
+  lo = grp_min<G>(lo);
+  hi = grp_max<G>(hi);
+  if (lo > hi) {
+    if (lane == 0) c_slot[row] = make_uint2(SLOT_EMPTY, 0u);
+    return;
+  }
+  const uint64_t w0 = lo >> 6;
+  const uint64_t nwords = (((uint64_t)hi + 1) >> 6) - w0 + 1;
+  const uint64_t nchunks = (nwords + W - 1) / W;
+  // one pass over the chunks: scatter, gather; WRITE stores entries at
+  // word `base` of c_side onwards
+  auto pass = [&](bool write, uint64_t base_word, bool scatter) -> uint32_t {
+    uint32_t count = 0;
+    uint64_t carry = 0;
+    for (uint64_t c0 = 0; c0 < nwords; c0 += W) {
+      const uint32_t cw = (uint32_t)min((uint64_t)W, nwords - c0);
+      const uint64_t cb = (w0 + c0) << 6;
+      const uint64_t clast = cb + ((uint64_t)cw << 6) - 1;
+      if (scatter) {
+        for (uint32_t j = lane; j < cw; j += G) lds[j] = 0;
+        wave_sync();
+        for (uint32_t q = 0; q < na; ++q) {
+          const uint2 ar = arange(q);
+          if (plus_i && lane == 0) {
+            const uint64_t rs = max((uint64_t)ar.x, cb), rf = min((uint64_t)ar.y, clast);
+            if (rs <= rf) window_set(lds, (uint32_t)(rs - cb), (uint32_t)(rf - cb));
+          }
+          each_b(ar.x, ar.y, [&](uint2 x) {
+            const uint64_t rs = max((uint64_t)x.x, cb), rf = min((uint64_t)x.y, clast);
+            if (rs <= rf) window_set(lds, (uint32_t)(rs - cb), (uint32_t)(rf - cb));
+          });
+        }
+        wave_sync();
+      }
+      const uint32_t qn = (cw + G - 1) / G;
+      const uint32_t j0 = min(lane * qn, cw), j1 = min(j0 + qn, cw);
+      const uint64_t cy0 = (j0 == 0) ? carry : (lds[j0 - 1] >> 63);
+      uint32_t nb = 0;
+      {
+        uint64_t cy = cy0;
+        for (uint32_t j = j0; j < j1; ++j) {
+          const uint64_t x = lds[j];
+          nb += __popcll(run_bounds(x, cy));
+          if (!write) nnz_acc += __popcll(x);
+          cy = x >> 63;
+        }
+      }
+      const uint32_t incl = grp_scan_incl<G>(nb, lane);
+      const uint32_t tot = (uint32_t)__shfl((int)incl, G - 1, G);
+      if (write) {
+        uint32_t idx = count + incl - nb;
+        uint32_t *words = reinterpret_cast<uint32_t *>(c_side);
+        uint64_t cy = cy0;
+        for (uint32_t j = j0; j < j1; ++j) {
+          const uint64_t x = lds[j];
+          uint64_t bnd = run_bounds(x, cy);
+          cy = x >> 63;
+          while (bnd) {
+            const uint32_t p = (uint32_t)__builtin_ctzll(bnd);
+            bnd &= bnd - 1;
+            words[base_word + idx] = (uint32_t)(cb + ((uint64_t)j << 6) + p - (idx & 1u));
+            ++idx;
+          }
+        }
+      }
+      count += tot;
+      carry = lds[cw - 1] >> 63;
+      wave_sync();
+    }
+    return count;
+  };
+  const uint32_t count = pass(false, 0, true);  // entries (2 per run)
+  const uint32_t runs = count / 2;
+  if (lane == 0) ranges_acc += runs;
+  if (count == 2) {  // one run: the extent itself
+    if (lane == 0) c_slot[row] = make_uint2(lo, hi);
+    return;
+  }
+  unsigned long long off = 0;
+  if (lane == 0) off = atomicAdd(&ctr->total, (unsigned long long)runs + 1);
+  off = ((unsigned long long)(uint32_t)__shfl((int)(uint32_t)(off >> 32), 0, G) << 32) |
+        (uint32_t)__shfl((int)(uint32_t)off, 0, G);
+  if (off + runs + 1 > side_cap) {
+    if (lane == 0) atomicOr(status, 1u);
+    return;
+  }
+  // a one-chunk window still holds the row's bits
+  pass(true, 2 * (off + 1), nchunks > 1);
+  if (lane == 0) {
+    c_side[off] = make_uint2(runs, 0u);
+    c_slot[row] = make_uint2(SLOT_MARK, (uint32_t)off);
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(BLOCK) void k_slot_group(
+    Slots A, Slots B, uint32_t W, uint32_t RPG, const uint32_t *__restrict__ rows,
+    const unsigned int *__restrict__ n_dev, uint2 *__restrict__ c_slot,
+    uint2 *__restrict__ c_side, unsigned long long side_cap, OpCounters *__restrict__ ctr,
+    unsigned int *__restrict__ status, uint32_t plus_i) {
+  extern __shared__ uint64_t lds_all[];
+  if (*status) return;
+  constexpr uint32_t GPB = BLOCK / G;
+  const uint32_t grp = threadIdx.x / G, lane = threadIdx.x % G;
+  uint64_t *lds = lds_all + (size_t)grp * W;
+  const uint64_t nl = *n_dev;
+  unsigned long long nnz_acc = 0, ranges_acc = 0;
+  const uint64_t groups = (uint64_t)gridDim.x * GPB;
+  const uint64_t gid = (uint64_t)xcd_tile(blockIdx.x, gridDim.x) * GPB + grp;
+  for (uint64_t i0 = gid * RPG; i0 < nl; i0 += groups * RPG) {
+    const uint64_t i1 = min(i0 + RPG, nl);
+    for (uint64_t i = uniform_u64<G>(i0); i < i1; ++i) {
+      const uint64_t row = uniform_u64<G>(rows[i]);
+      slot_row<G>(row, A, B, lds, W, lane, c_slot, c_side, side_cap, ctr, status, nnz_acc,
+                  ranges_acc, plus_i);
+    }
+  }
+  block_add_u64(nnz_acc, &ctr->nnz);
+  block_add_u64(ranges_acc, &ctr->ranges);
+}
+
+// Interval summaries of a slot matrix (as k_interval_summary).
+template <int LANES>
+__global__ __launch_bounds__(BLOCK) void k_slot_summary(Slots B, uint64_t nrows,
+                                                        uint2 *__restrict__ pre,
+                                                        uint2 *__restrict__ suf,
+                                                        uint4 *__restrict__ pre2,
+                                                        uint4 *__restrict__ suf2,
+                                                        const unsigned int *__restrict__ status) {
+  if (*status) return;
+  constexpr int K = SUMM_ROWS_PER_LANE;
+  const int pos = threadIdx.x & (LANES - 1);
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK * K; base < nrows;
+       base += (uint64_t)gridDim.x * BLOCK * K) {
+    const uint64_t r0 = base + (uint64_t)threadIdx.x * K;
+    uint2 sl[K], x[K];
+    bool single = true;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      sl[k] = r0 + k < nrows ? B.slot[r0 + k] : make_uint2(SLOT_EMPTY, 0u);
+      const bool one = sl[k].x <= sl[k].y;
+      x[k] = one ? sl[k] : make_uint2(SUMM_EMPTY, 0u);
+      single = single && one;
+    }
+    summary_lane<LANES>(
+        x, single,
+        [&](int k, Ivl2 &v) {
+          if (sl[k].x != SLOT_MARK) return;
+          const uint32_t nb = B.side[sl[k].y].x;
+          for (uint32_t q = 0; q < nb && !ivl_unknown(v); ++q) {
+            const uint2 y = B.side[sl[k].y + 1 + q];
+            v = ivl_join(v, Ivl2{y.x, y.y, SUMM_EMPTY, 0u});
+          }
+        },
+        r0, nrows, pos, pre, suf, pre2, suf2);
+  }
+}
+
+// rCRS -> slots (a row of several ranges into the side array).
+template <typename S>
+__global__ __launch_bounds__(BLOCK) void k_rcrs_to_slots(Mat<S> A, uint64_t nrows,
+                                                         uint2 *__restrict__ slot,
+                                                         uint2 *__restrict__ side,
+                                                         unsigned long long side_cap,
+                                                         OpCounters *__restrict__ ctr,
+                                                         unsigned int *__restrict__ status) {
+  for (uint64_t b0 = (uint64_t)blockIdx.x * BLOCK; b0 < nrows; b0 += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t row = b0 + threadIdx.x;
+    const bool live = row < nrows;
+    uint64_t p0 = 0, p1 = 0;
+    if (live) {
+      p0 = A.rm[row];
+      p1 = A.rm[row + 1];
+    }
+    const uint64_t nr = (p1 - p0) / 2;
+    const uint32_t need_side = nr > 1 ? (uint32_t)nr + 1 : 0u;
+    const unsigned long long off = wave_alloc(need_side, &ctr->total);
+    if (!live) continue;
+    if (nr == 0) {
+      slot[row] = make_uint2(SLOT_EMPTY, 0u);
+    } else if (nr == 1) {
+      slot[row] = ld_range(A.en, p0);
+    } else if (off + need_side > side_cap) {
+      atomicOr(status, 1u);
+    } else {
+      side[off] = make_uint2((uint32_t)nr, 0u);
+      for (uint64_t q = 0; q < nr; ++q) side[off + 1 + q] = ld_range(A.en, p0 + 2 * q);
+      slot[row] = make_uint2(SLOT_MARK, (uint32_t)off);
+    }
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_slot_identity(uint64_t n, uint2 *__restrict__ slot) {
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * BLOCK)
+    slot[i] = make_uint2((uint32_t)i, (uint32_t)i);
+}
+
+// C = I + A on slots (range_sparse.hpp:112-357 for canonical rows: the
+// union of A's row and {i}).
+__global__ __launch_bounds__(BLOCK) void k_slot_plus_identity(
+    Slots A, uint64_t nrows, uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side,
+    unsigned long long side_cap, OpCounters *__restrict__ ctr, unsigned int *__restrict__ status,
+    unsigned long long *__restrict__ parts) {
+  if (*status) return;
+  unsigned long long nnz_acc = 0, ranges_acc = 0;
+  for (uint64_t b0 = (uint64_t)blockIdx.x * BLOCK; b0 < nrows; b0 += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t row = b0 + threadIdx.x;
+    const bool live = row < nrows;
+    const uint32_t i = (uint32_t)row;
+    uint2 a = make_uint2(SLOT_EMPTY, 0u);
+    if (live) a = A.slot[row];
+    uint32_t na = 0;  // A's ranges (a marked row: from the side array)
+    if (a.x <= a.y) na = 1;
+    else if (a.x == SLOT_MARK) na = A.side[a.y].x;
+    auto arange = [&](uint32_t q) { return na == 1 && a.x <= a.y ? a : A.side[a.y + 1 + q]; };
+    // one range that touches i stays one range; otherwise the merge of
+    // A's ranges with [i, i] goes to the side array (at most na + 1 runs)
+    bool one = false;
+    uint2 u = make_uint2(i, i);
+    if (na == 0) {
+      one = true;
+    } else if (na == 1 && iv_touch(a.x, a.y, i, i)) {
+      one = true;
+      u = make_uint2(min(a.x, i), max(a.y, i));
+    }
+    const uint32_t need_side = live && !one ? na + 2 : 0u;
+    const unsigned long long off = wave_alloc(need_side, &ctr->total);
+    if (!live) continue;
+    if (one) {
+      c_slot[row] = u;
+      nnz_acc += (unsigned long long)(u.y - u.x) + 1;
+      ranges_acc += 1;
+      continue;
+    }
+    if (off + need_side > side_cap) {
+      atomicOr(status, 1u);
+      continue;
+    }
+    // merge: i goes in order; touching neighbours fold together
+    uint32_t k = 0;
+    bool placed = false;
+    uint32_t plo = 0, phi = 0;
+    bool have = false;
+    auto put = [&](uint32_t lo, uint32_t hi) {
+      if (have && lo <= phi + 1u) {
+        phi = max(phi, hi);
+      } else {
+        if (have) {
+          c_side[off + 1 + k] = make_uint2(plo, phi);
+          nnz_acc += (unsigned long long)(phi - plo) + 1;
+          ++k;
+        }
+        plo = lo;
+        phi = hi;
+        have = true;
+      }
+    };
+    for (uint32_t q = 0; q < na; ++q) {
+      const uint2 x = arange(q);
+      if (!placed && i < x.x) {
+        put(i, i);
+        placed = true;
+      }
+      put(x.x, x.y);
+    }
+    if (!placed) put(i, i);
+    c_side[off + 1 + k] = make_uint2(plo, phi);
+    nnz_acc += (unsigned long long)(phi - plo) + 1;
+    ++k;
+    ranges_acc += k;
+    if (k == 1) {
+      c_slot[row] = make_uint2(plo, phi);
+    } else {
+      c_side[off] = make_uint2(k, 0u);
+      c_slot[row] = make_uint2(SLOT_MARK, (uint32_t)off);
+    }
+  }
+  // per-block partial sums (as the lane tier)
+  nnz_acc = block_sum_u64(nnz_acc);
+  ranges_acc = block_sum_u64(ranges_acc);
+  if (threadIdx.x == 0) {
+    parts[3 * blockIdx.x] = nnz_acc;
+    parts[3 * blockIdx.x + 1] = 0;
+    parts[3 * blockIdx.x + 2] = ranges_acc;
+  }
+}
+
+// Per-block (nnz, slice, ranges) sums of the lane-tier launches, per_op
+// triples for each of nops operations (zero where a launch had fewer
+// blocks): one block per operation adds them to its counters.
+__global__ __launch_bounds__(BLOCK) void k_reduce_parts(const unsigned long long *__restrict__ parts,
+                                                        uint32_t per_op,
+                                                        OpCounters *__restrict__ ctr) {
+  const unsigned long long *p = parts + 3ull * per_op * blockIdx.x;
+  unsigned long long a = 0, b = 0, c = 0;
+  for (uint32_t j = threadIdx.x; j < per_op; j += BLOCK) {
+    a += p[3 * j];
+    b += p[3 * j + 1];
+    c += p[3 * j + 2];
+  }
+  a = block_sum_u64(a);
+  b = block_sum_u64(b);
+  c = block_sum_u64(c);
+  if (threadIdx.x == 0) {
+    ctr[blockIdx.x].nnz += a;
+    ctr[blockIdx.x].slice += b;
+    ctr[blockIdx.x].ranges += c;
+  }
+}
+
+// Slots -> rCRS: entry counts per row, then (after the row-pointer scan) the
+// entries at their row pointers.
+__global__ __launch_bounds__(BLOCK) void k_slot_counts(Slots A, uint64_t nrows,
+                                                       uint32_t *__restrict__ cnt,
+                                                       const unsigned int *__restrict__ status) {
+  if (*status) return;  // a product ran out of side entries: slots not valid
+  for (uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; row < nrows;
+       row += (uint64_t)gridDim.x * BLOCK) {
+    const uint2 a = A.slot[row];
+    cnt[row] = a.x <= a.y ? 2u : a.x == SLOT_MARK ? 2u * A.side[a.y].x : 0u;
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(BLOCK) void k_slot_emit(Slots A, uint64_t nrows,
+                                                     const S *__restrict__ rm,
+                                                     uint32_t *__restrict__ en) {
+  for (uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; row < nrows;
+       row += (uint64_t)gridDim.x * BLOCK) {
+    const uint2 a = A.slot[row];
+    const uint64_t p = rm[row];
+    if (a.x <= a.y) {
+      *reinterpret_cast<uint2 *>(en + p) = a;
+    } else if (a.x == SLOT_MARK) {
+      const uint32_t nr = A.side[a.y].x;
+      for (uint32_t q = 0; q < nr; ++q)
+        *reinterpret_cast<uint2 *>(en + p + 2 * q) = A.side[a.y + 1 + q];
+    }
+  }
+}
+
 }  // namespace
 
 void fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v) {
@@ -1835,6 +2472,140 @@ void slice_lengths(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *len) 
   else
     hipLaunchKernelGGL(k_slice_lengths<uint64_t>, dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
                        mat<uint64_t>(a), mat<uint64_t>(b), a.nrows, len);
+}
+
+
+// ---------------------------------------------------------------------------
+// Slot-format launchers.
+namespace {
+Slots slots_of(SlotView v) {
+  return Slots{static_cast<const uint2 *>(v.slot), static_cast<const uint2 *>(v.side)};
+}
+}  // namespace
+
+void slots_from_rcrs(hipStream_t s, uint32_t sb, DView a, void *slot, void *side,
+                     uint64_t side_cap, OpCounters *ctr, unsigned int *status) {
+  if (!a.nrows) return;
+  uint2 *sl = static_cast<uint2 *>(slot), *sd = static_cast<uint2 *>(side);
+  if (sb == 4)
+    hipLaunchKernelGGL(k_rcrs_to_slots<uint32_t>, dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
+                       mat<uint32_t>(a), a.nrows, sl, sd, (unsigned long long)side_cap, ctr, status);
+  else
+    hipLaunchKernelGGL(k_rcrs_to_slots<uint64_t>, dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
+                       mat<uint64_t>(a), a.nrows, sl, sd, (unsigned long long)side_cap, ctr, status);
+}
+
+void slots_identity(hipStream_t s, uint64_t n, void *slot) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_slot_identity, dim3(grid_for(n)), dim3(BLOCK), 0, s, n,
+                     static_cast<uint2 *>(slot));
+}
+
+void slots_plus_identity(hipStream_t s, SlotView a, uint64_t n, void *c_slot, void *c_side,
+                         uint64_t side_cap, OpCounters *ctr, unsigned int *status,
+                         unsigned long long *parts, uint32_t max_blocks) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_slot_plus_identity, dim3(grid_for(n, BLOCK, max_blocks)), dim3(BLOCK), 0, s,
+                     slots_of(a), n, static_cast<uint2 *>(c_slot), static_cast<uint2 *>(c_side),
+                     (unsigned long long)side_cap, ctr, status, parts);
+}
+
+void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
+                   const unsigned int *status, uint32_t block_rows) {
+  if (!nrows) return;
+  uint2 *pre = static_cast<uint2 *>(summ), *suf = pre + nrows;
+  uint4 *pre2 = reinterpret_cast<uint4 *>(pre + 2 * nrows), *suf2 = pre2 + nrows;
+  const uint32_t grid = grid_for(nrows, BLOCK * SUMM_ROWS_PER_LANE);
+#define DV_SSUMM(L_)                                                                         \
+  hipLaunchKernelGGL((k_slot_summary<L_>), dim3(grid), dim3(BLOCK), 0, s, slots_of(b), nrows, \
+                     pre, suf, pre2, suf2, status)
+  switch (block_rows / SUMM_ROWS_PER_LANE) {
+    case 8: DV_SSUMM(8); break;
+    case 16: DV_SSUMM(16); break;
+    case 32: DV_SSUMM(32); break;
+    default: DV_SSUMM(64); break;
+  }
+#undef DV_SSUMM
+}
+
+void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t b_nrows,
+                uint32_t lane_max, uint32_t mid_max, uint32_t grid, void *c_slot, void *c_side,
+                uint64_t side_cap, uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
+                unsigned int *status, const void *summ, uint32_t summ_min, uint32_t summ_rows,
+                uint32_t plus_i, unsigned long long *parts) {
+  if (!nrows) return;
+  const uint2 *pre = static_cast<const uint2 *>(summ);
+  const uint2 *suf = pre ? pre + b_nrows : nullptr;
+  const uint4 *pre2 = pre ? reinterpret_cast<const uint4 *>(pre + 2 * b_nrows) : nullptr;
+  const uint4 *suf2 = pre ? pre2 + b_nrows : nullptr;
+  const uint32_t sh = (uint32_t)__builtin_ctz(summ_rows ? summ_rows : 1);
+#define DV_SLANE(M_)                                                                          \
+  hipLaunchKernelGGL((k_slot_lane<M_>), dim3(grid), dim3(BLOCK), 0, s, slots_of(a), slots_of(b), \
+                     nrows, lane_max, mid_max, static_cast<uint2 *>(c_slot),                  \
+                     static_cast<uint2 *>(c_side), (unsigned long long)side_cap, mid_list,    \
+                     big_list, ctr, status, pre, suf, pre2, suf2, summ_min, sh, plus_i, parts)
+  if (pre) DV_SLANE(true);
+  else DV_SLANE(false);
+#undef DV_SLANE
+}
+
+void slots_group(hipStream_t s, SlotView a, SlotView b, uint32_t lanes, uint32_t lds_words,
+                 uint32_t rows_per_group, uint32_t grid, const uint32_t *rows,
+                 const unsigned int *n_dev, void *c_slot, void *c_side, uint64_t side_cap,
+                 OpCounters *ctr, unsigned int *status, uint32_t plus_i) {
+  uint2 *cs = static_cast<uint2 *>(c_slot), *cd = static_cast<uint2 *>(c_side);
+#define DV_SGRP(G_)                                                                           \
+  hipLaunchKernelGGL((k_slot_group<G_>), dim3(grid), dim3(BLOCK),                             \
+                     (size_t)(BLOCK / G_) * lds_words * 8, s, slots_of(a), slots_of(b), lds_words, \
+                     rows_per_group, rows, n_dev, cs, cd, (unsigned long long)side_cap, ctr,   \
+                     status, plus_i)
+  switch (lanes) {
+    case 8: DV_SGRP(8); break;
+    case 16: DV_SGRP(16); break;
+    case 32: DV_SGRP(32); break;
+    default: DV_SGRP(64); break;
+  }
+#undef DV_SGRP
+}
+
+void slots_reduce_parts(hipStream_t s, const unsigned long long *parts, uint32_t per_op,
+                        uint32_t nops, OpCounters *ctr) {
+  if (!nops) return;
+  hipLaunchKernelGGL(k_reduce_parts, dim3(nops), dim3(BLOCK), 0, s, parts, per_op, ctr);
+}
+
+void slots_counts(hipStream_t s, SlotView a, uint64_t n, uint32_t *cnt,
+                  const unsigned int *status) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_slot_counts, dim3(grid_for(n)), dim3(BLOCK), 0, s, slots_of(a), n, cnt,
+                     status);
+}
+
+void slots_emit(hipStream_t s, SlotView a, uint64_t n, uint32_t sb, const void *rowmap,
+                uint32_t *entries) {
+  if (!n) return;
+  if (sb == 4)
+    hipLaunchKernelGGL(k_slot_emit<uint32_t>, dim3(grid_for(n)), dim3(BLOCK), 0, s, slots_of(a), n,
+                       static_cast<const uint32_t *>(rowmap), entries);
+  else
+    hipLaunchKernelGGL(k_slot_emit<uint64_t>, dim3(grid_for(n)), dim3(BLOCK), 0, s, slots_of(a), n,
+                       static_cast<const uint64_t *>(rowmap), entries);
+}
+
+uint32_t slot_lane_blocks_per_cu(bool summ) {
+  static int cache[2] = {0, 0};
+  const int k = summ ? 1 : 0;
+  if (!cache[k]) {
+    int nb = 0;
+    const hipError_t e = summ ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<true>, BLOCK, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<false>, BLOCK, 0);
+    if (e != hipSuccess || nb < 1) {
+      (void)hipGetLastError();
+      nb = 4;
+    }
+    cache[k] = nb;
+  }
+  return (uint32_t)cache[k];
 }
 
 }  // namespace dv

@@ -24,6 +24,7 @@ struct OpCounters {
   unsigned int n_mid;           // rows handed to the 16-lane groups
   unsigned int n_big;           // rows handed to the 64-lane groups
   unsigned int pad;
+  unsigned long long ranges;    // ranges of C (slot-format products)
 };
 
 // One launch of the group tiers over a row list.  Counting mode when
@@ -157,5 +158,51 @@ void validate_matrix(hipStream_t s, uint32_t sb, DView m, uint64_t n_entries, ui
 // Number of B ranges each row of A gathers (sum over the row's ranges of the
 // B slice length) -- the work measure behind the size classes.
 void slice_lengths(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *len);
+
+
+// --- Slot format (intermediate matrices of planned index builds) -----------
+// One uint2 per row: (lo, hi) for a row of one range, (0xFFFFFFFF, 0) for an
+// empty row, (0xFFFFFFFE, off) for a row of several ranges in the side array
+// (side[off] = (count, 0), then the ranges).  Side arrays hold side_cap
+// entries plus SLOT_SIDE_PAD spare ones (side_cap below 2^32 - 16: offsets
+// are 32-bit); ctr->total counts the entries used (a product that would pass
+// side_cap sets bit 0 of *status, and every later kernel of the build then
+// does nothing), ctr->ranges the ranges written.
+constexpr uint64_t SLOT_SIDE_PAD = 8;
+struct SlotView {
+  const void *slot;
+  const void *side;
+};
+void slots_from_rcrs(hipStream_t s, uint32_t sb, DView a, void *slot, void *side,
+                     uint64_t side_cap, OpCounters *ctr, unsigned int *status);
+void slots_identity(hipStream_t s, uint64_t n, void *slot);
+// C = I + A; parts as slots_lane (at most max_blocks blocks)
+void slots_plus_identity(hipStream_t s, SlotView a, uint64_t n, void *c_slot, void *c_side,
+                         uint64_t side_cap, OpCounters *ctr, unsigned int *status,
+                         unsigned long long *parts, uint32_t max_blocks);
+// Interval summaries of B (layout and block_rows as block_summary).
+void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
+                   const unsigned int *status, uint32_t block_rows);
+// Lane tier of C = A.B (A (B + I) with plus_i): rows it finishes are
+// written; others go to mid_list / big_list (counts in ctr->n_mid, n_big)
+// for slots_group.  parts receives 3 words per block (nnz, slice, ranges).
+void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t b_nrows,
+                uint32_t lane_max, uint32_t mid_max, uint32_t grid, void *c_slot, void *c_side,
+                uint64_t side_cap, uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
+                unsigned int *status, const void *summ, uint32_t summ_min, uint32_t summ_rows,
+                uint32_t plus_i, unsigned long long *parts);
+void slots_group(hipStream_t s, SlotView a, SlotView b, uint32_t lanes, uint32_t lds_words,
+                 uint32_t rows_per_group, uint32_t grid, const uint32_t *rows,
+                 const unsigned int *n_dev, void *c_slot, void *c_side, uint64_t side_cap,
+                 OpCounters *ctr, unsigned int *status, uint32_t plus_i);
+// ctr[i] += the sums of parts[3 per_op i ..] (one block per operation).
+void slots_reduce_parts(hipStream_t s, const unsigned long long *parts, uint32_t per_op,
+                        uint32_t nops, OpCounters *ctr);
+// Slots -> rCRS: per-row entry counts, and the entries at their row pointers.
+void slots_counts(hipStream_t s, SlotView a, uint64_t n, uint32_t *cnt,
+                  const unsigned int *status);
+void slots_emit(hipStream_t s, SlotView a, uint64_t n, uint32_t sb, const void *rowmap,
+                uint32_t *entries);
+uint32_t slot_lane_blocks_per_cu(bool summaries);
 
 }  // namespace dv

@@ -128,16 +128,22 @@ void *dv_stream(dv_context *ctx);
  *                   buffers and tuning stay the same (not while profiling)
  *   summary_span    planned SpGEMMs whose left operand's ranges may span at
  *                   least this many rows (>= 2; 0: never) first summarise the
- *                   right operand in 32-row blocks (per row, the union of the
- *                   rows from its block's start / to its block's end when it
- *                   is one interval); a left range of at least this many rows
- *                   that crosses a block boundary is then read as one suffix,
- *                   whole blocks and one prefix instead of row by row
+ *                   right operand in blocks of 32 to 256 rows (per row, the
+ *                   union of the rows from its block's start / to its block's
+ *                   end when it has at most two pieces); a left range of at
+ *                   least this many rows that crosses a block boundary is
+ *                   then read as one suffix, whole blocks and one prefix
+ *                   instead of row by row
  *   schedule        0: a planned index build takes whichever of
  *                   A^d1 (A + I)^(d2 - d1) and (A (A + I))^m . rest
  *                   (m = min(d1, d2 - d1)) needs fewer SpGEMMs -- the same
  *                   index, as the factors commute; 1: always the reference's
- *                   order (dindex.hpp:131-177) */
+ *                   order (dindex.hpp:131-177)
+ *   layout          0: the products of a planned index build keep their
+ *                   results in slot form (one range per row in place, rows of
+ *                   several ranges in a side array), so no product needs a
+ *                   row-pointer scan; only the index is converted to range
+ *                   CRS; 1: every product writes range CRS */
 typedef struct dv_tuning {
   uint32_t lds_words;
   uint32_t stage_entries;
@@ -150,6 +156,7 @@ typedef struct dv_tuning {
   uint32_t graphs;
   uint32_t summary_span;
   uint32_t schedule;
+  uint32_t layout;
 } dv_tuning;
 int dv_get_tuning(const dv_context *ctx, dv_tuning *t);
 int dv_set_tuning(dv_context *ctx, const dv_tuning *t);
