@@ -1748,13 +1748,24 @@ __global__ __launch_bounds__(BLOCK) void k_slot_lane(
     }
     uint32_t cl = 0xFFFFFFFFu, ch = 0u;
     // A ranges: the slot itself, or the side entries of a marked row
+    // (the reads of a marked row's ranges are waited on where they are
+    // made, so the common row does not wait for the prefetch above)
     uint32_t na = 0;
     if (live) {
-      if (a.x <= a.y) na = 1;
-      else if (a.x == SLOT_MARK) na = A.side[a.y].x;
+      if (a.x <= a.y) {
+        na = 1;
+      } else if (a.x == SLOT_MARK) {
+        na = A.side[a.y].x;
+        need(na);
+      }
     }
     for (uint32_t q = 0; q < na; ++q) {
-      const uint2 ar = a.x <= a.y ? a : A.side[a.y + 1 + q];
+      uint2 ar = a;
+      if (a.x > a.y) {
+        ar = A.side[a.y + 1 + q];
+        need(ar.x);
+        need(ar.y);
+      }
       const uint32_t s = ar.x, e = ar.y;
       len += (uint64_t)(e - s) + 1;
       if (len > lane_max) ok = false;
