@@ -29,8 +29,9 @@ Further legs (fields of the same JSON line; --legs selects them):
                 plain builds; bit-exact per component
 
 The JSON line also carries:
-  roofline      the SpGEMM lane-per-row kernel k_spgemm_lane (the dominant
-                kernel): algorithmic bytes bytes(X) + bytes(Y) + bytes(C) of its
+  roofline      the SpGEMM lane-per-row kernel (the dominant kernel:
+                k_slot_lane for slot-form products, k_spgemm_lane for range-CRS
+                ones): algorithmic bytes bytes(X) + bytes(Y) + bytes(C) of its
                 product per launch over its mean launch time (HIP events on the
                 library stream), against 8 TB/s
   cpu_baseline  the CPU oracle (the reference's OpenMP algorithm restated in
@@ -61,10 +62,10 @@ def hip_sync():
         raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
 
 
-def measured_traffic(args):
-    """HBM bytes per k_spgemm_lane launch from the committed rocprofv3 counter
-    runs (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy's readings), for the
-    workload they were taken on; None otherwise."""
+def measured_traffic(args, kernel):
+    """HBM bytes per lane-tier kernel launch from the committed rocprofv3
+    counter runs (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy's readings),
+    for the workload and kernel they were taken on; None otherwise."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -73,6 +74,8 @@ def measured_traffic(args):
         return None
     w = t.get("workload", {})
     if (w.get("preset"), w.get("d1"), w.get("d2")) != (args.preset, args.dlo, args.dup):
+        return None
+    if t.get("kernel") != kernel:
         return None
     return t.get("traffic_bytes_per_launch")
 
@@ -155,6 +158,7 @@ def main():
     build_ms = float(np.median([s["build_ms"] for s in stats]))
     value = a.nrows * args.steps / elapsed
 
+    lane_kernel = "k_slot_lane" if stats[-1]["layout"] == 0 else "k_spgemm_lane"
     cnt = prof["spgemm_lane"]
     mean_ms = cnt["total_ms"] / max(1, cnt["launches"])
     alg_per_launch = cnt["alg_bytes"] / max(1, cnt["launches"])
@@ -163,12 +167,12 @@ def main():
         if mean_ms > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": "k_spgemm_lane",
+        "kernel": lane_kernel,
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": measured_traffic(args),
+        "traffic": measured_traffic(args, lane_kernel),
         "launches": cnt["launches"],
         "mean_launch_ms": round(mean_ms, 4),
         "alg_bytes_per_launch": int(alg_per_launch),

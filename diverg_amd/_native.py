@@ -44,6 +44,8 @@ class dv_build_stats(C.Structure):
         ("deferred_rows", C.c_uint64),
         ("planned", C.c_uint32),
         ("retried", C.c_uint32),
+        ("layout", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

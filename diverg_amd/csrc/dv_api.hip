@@ -624,6 +624,7 @@ int dv_create_distance_index(dv_context *ctx, const dv_rcrs *a, uint32_t dlo, ui
   st.deferred_rows = c->stats_mid_rows + c->stats_big_rows;
   st.planned = c->stats_planned;
   st.retried = c->stats_retried;
+  st.layout = c->stats_layout;
   // everything the context holds at the high-water mark of the build: the
   // resident input, cached plan buffers, scratch and the output
   st.peak_device_bytes = c->peak_bytes + input_bytes;
@@ -671,6 +672,7 @@ int dv_create_distance_index_batched(dv_context *ctx, const dv_rcrs *comps, uint
   st.deferred_rows = c->stats_mid_rows + c->stats_big_rows;
   st.planned = c->stats_planned;
   st.retried = c->stats_retried;
+  st.layout = c->stats_layout;
   st.peak_device_bytes = c->peak_bytes + input_bytes;
   for (uint32_t i = 0; i < ncomp; ++i) {
     if ((rc = export_mat(c, douts[i], loc, &outs[i], &st.d2h_ms))) {

@@ -183,11 +183,12 @@ struct Ctx {
   uint64_t stats_alg_bytes = 0, stats_overflow_rows = 0, stats_mid_rows = 0, stats_big_rows = 0;
   uint32_t stats_n_spgemm = 0, stats_n_spadd = 0;
   double stats_max_ratio = 0;
-  uint32_t stats_planned = 0, stats_retried = 0;
+  uint32_t stats_planned = 0, stats_retried = 0, stats_layout = 1;
   void reset_stats() {
     stats_alg_bytes = stats_overflow_rows = stats_mid_rows = stats_big_rows = 0;
     stats_n_spgemm = stats_n_spadd = 0;
     stats_planned = stats_retried = 0;
+    stats_layout = 1;
   }
 
   bool hip(hipError_t e, const char *what);

@@ -264,6 +264,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     }
   }
   stats_planned = 1;
+  stats_layout = 0;
   release_all();
   // Rows of several ranges cost the slot form a dependent read each (the
   // side array), where range CRS reads them with the rest of a slice: when

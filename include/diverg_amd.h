@@ -86,6 +86,8 @@ typedef struct dv_build_stats {
   uint32_t planned;        /* 1: enqueued as one planned sequence, one readback */
   uint32_t retried;        /* 1: the plan overflowed a predicted capacity and the
                               build was repeated with a readback per operation */
+  uint32_t layout;         /* products' intermediate form: 0 slots, 1 range CRS */
+  uint32_t reserved;
 } dv_build_stats;
 
 /* --- runtime ------------------------------------------------------------
