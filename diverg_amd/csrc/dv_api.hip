@@ -443,7 +443,7 @@ int dv_set_tuning(dv_context *c, const dv_tuning *t) {
   if (n.graphs > 1) return C(c)->fail("graphs must be 0 or 1");
   if (n.autotune > 1) return C(c)->fail("autotune must be 0 or 1");
   if (n.schedule > 1) return C(c)->fail("schedule must be 0 or 1");
-  if (n.layout > 1) return C(c)->fail("layout must be 0 or 1");
+  if (n.layout > 2) return C(c)->fail("layout must be 0, 1 or 2");
   if (n.summary_span == 1) return C(c)->fail("summary_span must be 0 or >= 2");
   c->tuning = n;
   return DV_OK;

@@ -488,7 +488,8 @@ bool Ctx::plan_events_reserve(int nops) {
 }
 
 int Ctx::dindex_planned(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
-  if (tuning.layout == 0 && !rcrs_shapes.count({a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup}))
+  if (tuning.layout == 2 ||
+      (tuning.layout == 0 && !rcrs_shapes.count({a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup})))
     return dindex_slots(a, dlo, dup, d);
   Plan p;
   p.schedule = tuning.schedule;

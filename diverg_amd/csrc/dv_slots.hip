@@ -271,7 +271,8 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   // they become common (a product with more side entries than SIDE_SHARE of
   // its rows), later builds of this shape use range CRS.
   constexpr double SIDE_SHARE = 0.05;
-  if (use > SIDE_SHARE) rcrs_shapes[{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup}] = true;
+  if (tuning.layout == 0 && use > SIDE_SHARE)
+    rcrs_shapes[{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup}] = true;
   (void)in_ext;
   return DV_OK;
 }

@@ -141,11 +141,15 @@ void *dv_stream(dv_context *ctx);
  *                   (m = min(d1, d2 - d1)) needs fewer SpGEMMs -- the same
  *                   index, as the factors commute; 1: always the reference's
  *                   order (dindex.hpp:131-177)
- *   layout          0: the products of a planned index build keep their
+ *   layout          2: the products of a planned index build keep their
  *                   results in slot form (one range per row in place, rows of
  *                   several ranges in a side array), so no product needs a
  *                   row-pointer scan; only the index is converted to range
- *                   CRS; 1: every product writes range CRS */
+ *                   CRS; 1: every product writes range CRS; 0: slot form,
+ *                   except that a build shape whose products had side
+ *                   entries for more than 5% of their rows is built in range
+ *                   CRS from then on (a row of several ranges costs the slot
+ *                   form a dependent read) */
 typedef struct dv_tuning {
   uint32_t lds_words;
   uint32_t stage_entries;

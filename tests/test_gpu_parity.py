@@ -143,11 +143,13 @@ def test_distance_index_autotuned(ctx, cfg1):
     assert t["lanes_small"] in (8, 16, 32) and t["stage_entries"] in (4, 8, 16)
 
 
+@pytest.mark.parametrize("layout", [1, 2])
 @pytest.mark.parametrize("summary_span", [0, 2, 16, 64, 200])
-def test_distance_index_block_summaries(ctx, summary_span):
+def test_distance_index_block_summaries(ctx, summary_span, layout):
     """Block summaries on and off, on a graph whose bubbles make some 32-row
-    blocks fail the single-interval test (they fall back to their ranges)."""
-    ctx.set_tuning(summary_span=summary_span)
+    blocks fail the single-interval test (they fall back to their ranges), in
+    range-CRS and slot-form builds."""
+    ctx.set_tuning(summary_span=summary_span, layout=layout)
     a = dv.generate_graph(3, backbone_bp=60000, seed=8, snp_rate=0.05, indel_rate=0.01,
                           sv_rate=2e-3, sv_min=10, sv_max=300)
     print(f"dindex summaries span={summary_span}", flush=True)
@@ -156,10 +158,12 @@ def test_distance_index_block_summaries(ctx, summary_span):
                          f"D[{dlo},{dup}] summary_span={summary_span}")
 
 
+@pytest.mark.parametrize("layout", [1, 2])
 @pytest.mark.parametrize("dlo,dup", [(0, 2100), (900, 2100)])
-def test_distance_index_wide_summary_blocks(ctx, dlo, dup):
+def test_distance_index_wide_summary_blocks(ctx, dlo, dup, layout):
     """Windows wide enough that the last products summarise in blocks of 64,
     128 and 256 rows (a quarter to an eighth of the ranges' span)."""
+    ctx.set_tuning(layout=layout)
     a = dv.generate_graph(3, backbone_bp=12000, seed=9, snp_rate=0.03, indel_rate=0.005,
                           sv_rate=1e-3, sv_min=10, sv_max=200)
     print(f"dindex wide summary blocks [{dlo},{dup}] {a.nrows} loci", flush=True)
@@ -176,29 +180,34 @@ def test_distance_index_many_scan_tiles(ctx):
     assert_identical(ctx.create_distance_index(a, 5, 24), orc.dindex(a, 5, 24), "D[5,24] 9e6 bp")
 
 
+@pytest.mark.parametrize("layout", [1, 2])
 @pytest.mark.parametrize("groups_only", [False, True])
 @pytest.mark.parametrize("schedule", [0, 1])
 @pytest.mark.parametrize("dlo,dup", [(300, 600), (40, 300), (200, 210), (1, 8), (3, 4)])
-def test_distance_index_schedules(ctx, schedule, dlo, dup, groups_only):
+def test_distance_index_schedules(ctx, schedule, dlo, dup, groups_only, layout):
     """The paired schedule (A (A + I))^m . rest, whose first product adds the
     identity inside the SpGEMM, and the reference's order A^d1 (A + I)^(d2 - d1)
-    give the same index, in the lane tier and in the group tiers."""
+    give the same index, in the lane tier and in the group tiers, in range-CRS
+    and slot-form builds."""
     a = dv.generate_graph(2, backbone_bp=40000, seed=33, size_bytes=4)
-    ctx.set_tuning(schedule=schedule, lane_slice_max=0 if groups_only else 4096)
+    ctx.set_tuning(schedule=schedule, lane_slice_max=0 if groups_only else 4096, layout=layout)
     print(f"dindex schedule={schedule} [{dlo},{dup}] groups_only={groups_only}", flush=True)
     d, st = ctx.create_distance_index(a, dlo, dup, stats=True)
     assert_identical(d, orc.dindex(a, dlo, dup), f"D[{dlo},{dup}] schedule {schedule}")
+    assert st["layout"] == (0 if layout == 2 else 1)
     if schedule == 0 and (dlo, dup) == (300, 600):
         assert st["n_spgemm"] == 12 and st["n_spadd"] == 0  # M = A (A + I), then M^300
 
 
+@pytest.mark.parametrize("layout", [1, 2])
 @pytest.mark.parametrize("size_bytes", [4, 8])
 @pytest.mark.parametrize("dlo,dup", [(150, 450), (0, 250), (7, 7), (1, 8), (5, 5), (0, 1)])
 @pytest.mark.parametrize("plan_mode", [0, 1, 2])
-def test_distance_index_cfg1(ctx, cfg1, size_bytes, dlo, dup, plan_mode):
+def test_distance_index_cfg1(ctx, cfg1, size_bytes, dlo, dup, plan_mode, layout):
     """Planned builds (one readback), exact builds (a readback per op) and
-    planned builds forced through the overflow fallback agree with the oracle."""
-    ctx.set_tuning(plan_mode=plan_mode)
+    planned builds forced through the overflow fallback agree with the oracle,
+    with range-CRS and slot-form products."""
+    ctx.set_tuning(plan_mode=plan_mode, layout=layout)
     a = cfg1.as_size(size_bytes)
     print(f"dindex cfg1 [{dlo},{dup}] S={size_bytes} plan_mode={plan_mode}", flush=True)
     d, st = ctx.create_distance_index(a, dlo, dup, stats=True)
@@ -206,6 +215,30 @@ def test_distance_index_cfg1(ctx, cfg1, size_bytes, dlo, dup, plan_mode):
     assert st["n_spadd"] <= 1 and st["n_spgemm"] > 0
     assert st["planned"] == (plan_mode == 0)
     assert st["retried"] == (plan_mode == 2)
+
+
+def test_distance_index_layout_switch(ctx):
+    """Automatic layout: a graph whose products have many rows of several
+    ranges is built in slot form once, then in range CRS; both are exact."""
+    a = dv.generate_graph(3, backbone_bp=30000, seed=5, snp_rate=0.08, indel_rate=0.02,
+                          sv_rate=3e-3, sv_min=10, sv_max=200)
+    ctx.set_tuning(layout=0)
+    want = orc.dindex(a, 40, 160)
+    d1, st1 = ctx.create_distance_index(a, 40, 160, stats=True)
+    d2, st2 = ctx.create_distance_index(a, 40, 160, stats=True)
+    assert_identical(d1, want, "first build")
+    assert_identical(d2, want, "second build")
+    assert st1["layout"] == 0 and st2["layout"] == 1
+
+
+def test_distance_index_slot_side_overflow(ctx):
+    """Slot-form products whose side arrays run out (a tiny prediction, as in
+    plan_mode 2) fall back to the exact build."""
+    ctx.set_tuning(layout=2, plan_mode=2)
+    a = dv.generate_graph(1, backbone_bp=20000, seed=3)
+    d, st = ctx.create_distance_index(a, 10, 60, stats=True)
+    assert_identical(d, orc.dindex(a, 10, 60), "D[10,60]")
+    assert st["retried"] == 1
 
 
 def test_distance_index_bad_bounds(ctx, cfg1):
