@@ -1786,7 +1786,17 @@ __global__ __launch_bounds__(BLOCK) void k_slot_lane(
     need(n_a.x);
     need(n_a.y);
     uint32_t nruns = 0;
-    if (live) {
+    // live slots form a prefix: with slot 0 empty the row is [cl, ch] alone
+    const bool single = ok && r.lo[0] > r.hi[0];
+    if (live && single) {
+      if (cl <= ch) {
+        r.lo[0] = cl;
+        r.hi[0] = ch;
+        nnz_acc += (unsigned long long)(ch - cl) + 1;
+        nruns = 1;
+      }
+      ranges_acc += nruns;
+    } else if (live) {
       if (ok && cl <= ch && !insert(r, cl, ch)) {
         compact(r);
         ok = insert(r, cl, ch);
