@@ -555,7 +555,7 @@ int Ctx::autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning 
   for (size_t d = 0; !rc && d < dims.size(); ++d) {
     const uint32_t keep = best.*(dims[d].first);
     for (uint32_t v : dims[d].second) {
-      if (v == keep || v == 0 && dims[d].first != &dv_tuning::summary_span) continue;
+      if (v == keep || (v == 0 && dims[d].first != &dv_tuning::summary_span)) continue;
       dv_tuning cand = best;
       cand.*(dims[d].first) = v;
       double ms = 0;

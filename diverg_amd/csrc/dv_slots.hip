@@ -49,14 +49,19 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
 
   // records: one per operation, then the input conversion and the output scan
   if (!plan_ctr_reserve(nops + 2)) return fail("device allocation failed (plan counters)", DV_ERR_NOMEM);
-  OpCounters *c_in = d_plan_ctr + nops, *c_out = d_plan_ctr + nops + 1;
+  OpCounters *c_out = d_plan_ctr + nops + 1;
   uint64_t nmax = a.nrows;
   for (const PlanOp &o : p.ops) nmax = std::max(nmax, p.slots[o.c].nrows);
   OpScratch s;
   if (!op_scratch(nmax, s)) return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
   const uint32_t per_op = s.parts_cap;  // lane-tier blocks per operation at most
-  if (!slot_parts_reserve(3ull * per_op * nops))
+  // per-block partial sums of every operation, then the side-partition
+  // counters of every operation and of the input conversion
+  const uint64_t parts_words = 3ull * per_op * nops;
+  const uint64_t ctr_words = (uint64_t)SLOT_SIDE_PARTS * (nops + 1);
+  if (!slot_parts_reserve(parts_words + ctr_words))
     return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
+  unsigned long long *side_ctr = slot_parts + parts_words;
 
   // buffers: slot array (8 bytes per row) in `rowmap`, side array in `entries`
   auto release_all = [&]() {
@@ -76,11 +81,11 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     return true;
   };
   // the input: every row of several ranges takes its ranges plus a header
+  // (1.5 entries per range at most), with room for uneven partitions
   PlanSlot &in = p.slots[x];
   const uint64_t in_ranges = a.n_entries / 2;
-  const PlanSlot in_ext = in;  // the caller's matrix (rowmap / entries)
   in.external = false;
-  if (!acquire(in, in_ranges + in_ranges / 2 + 16)) {
+  if (!acquire(in, 3 * in_ranges + 4096ull * SLOT_SIDE_PARTS)) {
     release_all();
     return fail("device allocation failed (plan buffers)", DV_ERR_NOMEM);
   }
@@ -89,7 +94,8 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     PlanSlot &c = p.slots[p.ops[i].c];
     const uint64_t cap = p.ops[i].kind == P_IDENT
                              ? 0
-                             : std::max<uint64_t>(4096, (uint64_t)(f * (double)c.nrows));
+                             : std::max<uint64_t>(4096ull * SLOT_SIDE_PARTS,
+                                                  (uint64_t)(f * (double)c.nrows));
     if (!acquire(c, cap)) {
       release_all();
       return fail("device allocation failed (plan buffers)", DV_ERR_NOMEM);
@@ -119,8 +125,9 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
            (uint32_t)((nops + 2) * sizeof(OpCounters) / 4), 0u);
   fill_u32(stream, d_status, 1, 0u);
-  fill_u32(stream, reinterpret_cast<uint32_t *>(slot_parts), 6ull * per_op * nops, 0u);
-  slots_from_rcrs(stream, a.sb, view(a), in.rowmap, in.entries, in.cap, c_in, d_status);
+  fill_u32(stream, reinterpret_cast<uint32_t *>(slot_parts), 2 * (parts_words + ctr_words), 0u);
+  slots_from_rcrs(stream, a.sb, view(a), in.rowmap, in.entries, in.cap,
+                  side_ctr + (uint64_t)SLOT_SIDE_PARTS * nops, d_status);
   for (int i = 0; i < nops; ++i) {
     const PlanOp &o = p.ops[i];
     PlanSlot &c = p.slots[o.c];
@@ -134,7 +141,8 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     }
     if (o.kind == P_SPADD) {
       const PlanSlot &xs = p.slots[p.slots[o.a].kind == P_IDENT ? o.b : o.a];
-      slots_plus_identity(stream, view_of(xs), c.nrows, c.rowmap, c.entries, c.cap, ctr, d_status,
+      slots_plus_identity(stream, view_of(xs), c.nrows, c.rowmap, c.entries, c.cap,
+                          side_ctr + (uint64_t)SLOT_SIDE_PARTS * i, d_status,
                           slot_parts + 3ull * per_op * i, per_op);
       if (timing)
         for (int e = 1; e < 5; ++e) (void)hipEventRecord(plan_ev[5 * i + e], stream);
@@ -151,22 +159,22 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     const uint64_t resident = (uint64_t)num_cu * slot_lane_blocks_per_cu(summ_rows != 0);
     const uint32_t grid =
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(n + 255) / 256, resident, per_op}));
+    unsigned long long *op_side = side_ctr + (uint64_t)SLOT_SIDE_PARTS * i;
     slots_lane(stream, view_of(A), view_of(B), n, B.nrows, tuning.lane_slice_max,
-               tuning.small_slice_max, grid, c.rowmap, c.entries, c.cap, s.mid, s.big, ctr,
-               d_status, summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows,
+               tuning.small_slice_max, grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big,
+               ctr, d_status, summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows,
                o.plus_i ? 1u : 0u, slot_parts + 3ull * per_op * i);
     if (timing) (void)hipEventRecord(plan_ev[5 * i + 2], stream);
     const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);
-    slots_group(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.rows_per_group, gm.grid,
-                s.mid, &ctr->n_mid, c.rowmap, c.entries, c.cap, ctr, d_status, o.plus_i ? 1u : 0u);
-    slots_group(stream, view_of(A), view_of(B), 64, gb.lds_words, gb.rows_per_group, gb.grid, s.big,
-                &ctr->n_big, c.rowmap, c.entries, c.cap, ctr, d_status, o.plus_i ? 1u : 0u);
+    slots_groups(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.grid, s.mid, &ctr->n_mid,
+                 gb.lds_words, gb.grid, s.big, &ctr->n_big, gm.rows_per_group, c.rowmap, c.entries,
+                 c.cap, op_side, ctr, d_status, o.plus_i ? 1u : 0u);
     if (timing) {
       (void)hipEventRecord(plan_ev[5 * i + 3], stream);
       (void)hipEventRecord(plan_ev[5 * i + 4], stream);
     }
   }
-  slots_reduce_parts(stream, slot_parts, per_op, (uint32_t)nops, d_plan_ctr);
+  slots_reduce_parts(stream, slot_parts, per_op, side_ctr, (uint32_t)nops, d_plan_ctr);
   const PlanSlot &O = p.slots[p.out];
   slots_counts(stream, view_of(O), O.nrows, s.cnt, d_status);
   scan_rowmap(stream, d.sb, s.cnt, O.nrows, d.rowmap, s.tmp, c_out, ~0ull, d_status, nullptr, 0,
@@ -187,11 +195,15 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   }
   // plan_mode 2 always takes the fallback (tests)
   const uint32_t st = *h_status | (tuning.plan_mode == 2 ? 1u : 0u);
-  // side use per row, for the next prediction
-  double use = 0;
+  // side use per row (all entries, and the fullest partition's share), for
+  // the layout choice and the next prediction
+  double use = 0, part_use = 0;
   for (int i = 0; i < nops; ++i)
-    if (p.slots[p.ops[i].c].nrows)
-      use = std::max(use, (double)h_plan_ctr[i].total / (double)p.slots[p.ops[i].c].nrows);
+    if (p.slots[p.ops[i].c].nrows) {
+      const double n = (double)p.slots[p.ops[i].c].nrows;
+      use = std::max(use, (double)h_plan_ctr[i].total / n);
+      part_use = std::max(part_use, (double)h_plan_ctr[i].pad * SLOT_SIDE_PARTS / n);
+    }
   if (st & 2) {
     release_all();
     release(d);
@@ -203,7 +215,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     plan_retries += 1;
     const int rc = exact(d);
     stats_retried = 1;
-    if (!rc && tuning.plan_mode != 2) slot_side_factor = std::max(2.0 * f, 1.5 * use);
+    if (!rc && tuning.plan_mode != 2) slot_side_factor = std::max(2.0 * f, 1.5 * part_use);
     return rc;
   }
   // ---- the index's entries
@@ -273,7 +285,6 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   constexpr double SIDE_SHARE = 0.05;
   if (tuning.layout == 0 && use > SIDE_SHARE)
     rcrs_shapes[{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup}] = true;
-  (void)in_ext;
   return DV_OK;
 }
 

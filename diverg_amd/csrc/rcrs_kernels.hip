@@ -1606,6 +1606,14 @@ struct Slots {
   const uint2 *__restrict__ side;
 };
 
+// Side entries are reserved in SLOT_SIDE_PARTS partitions of pcap entries,
+// each with its own counter (ctr[p]), so that the reservations of a launch do
+// not all serialise on one word; a wave (or a group) uses one partition.
+struct SideAlloc {
+  unsigned long long *ctr;
+  unsigned long long pcap;
+};
+
 // Reserve `need` entries of a side array for each lane (0: none) with one
 // atomic per wave; returns the lane's first entry.  Every lane of the wave
 // must call it.
@@ -1668,19 +1676,24 @@ __device__ __forceinline__ void fold_slots(const Slots &B, uint32_t s, uint32_t 
     for (int k = 0; k < LANE_BATCH; ++k)
       v[k] = (uint32_t)k <= rem ? B.slot[j + k] : make_uint2(SLOT_EMPTY, 0u);
     fold_batch(v, cl, ch, r, ok);  // markers and empty rows are not ranges: skipped
-    uint32_t mk = 0, offs[LANE_BATCH];
+    uint32_t anym = 0xFFFFFFFFu;  // 0 when some slot of the batch is marked (rare)
 #pragma unroll
-    for (int k = 0; k < LANE_BATCH; ++k) {
-      mk |= v[k].x == SLOT_MARK ? 1u << k : 0u;
-      offs[k] = v[k].y;
-    }
-    while (ok && mk) {
-      const uint32_t k = (uint32_t)__builtin_ctz(mk);
-      mk &= mk - 1;
-      uint32_t off = offs[0];
+    for (int k = 0; k < LANE_BATCH; ++k) anym = min(anym, v[k].x ^ SLOT_MARK);
+    if (anym == 0) {
+      uint32_t mk = 0, offs[LANE_BATCH];
 #pragma unroll
-      for (int q = 1; q < LANE_BATCH; ++q) off = k == (uint32_t)q ? offs[q] : off;
-      fold_side(B.side, off, cl, ch, r, ok, gathered);
+      for (int k = 0; k < LANE_BATCH; ++k) {
+        mk |= v[k].x == SLOT_MARK ? 1u << k : 0u;
+        offs[k] = v[k].y;
+      }
+      while (ok && mk) {
+        const uint32_t k = (uint32_t)__builtin_ctz(mk);
+        mk &= mk - 1;
+        uint32_t off = offs[0];
+#pragma unroll
+        for (int q = 1; q < LANE_BATCH; ++q) off = k == (uint32_t)q ? offs[q] : off;
+        fold_side(B.side, off, cl, ch, r, ok, gathered);
+      }
     }
     j = rem >= (uint32_t)LANE_BATCH ? j + LANE_BATCH : e + 1;
     if (j == 0) break;  // (e + 1 wrapped: e was the last ordinal)
@@ -1713,7 +1726,7 @@ __device__ __forceinline__ void slot_store_runs(const Runs &r, uint32_t nruns, u
 template <bool SUMM>
 __global__ __launch_bounds__(BLOCK) void k_slot_lane(
     Slots A, Slots B, uint64_t nrows, uint32_t lane_max, uint32_t mid_max,
-    uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side, unsigned long long side_cap,
+    uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side, SideAlloc sa,
     uint32_t *__restrict__ mid_list, uint32_t *__restrict__ big_list,
     OpCounters *__restrict__ ctr, unsigned int *__restrict__ status,
     const uint2 *__restrict__ pre, const uint2 *__restrict__ suf,
@@ -1815,12 +1828,13 @@ __global__ __launch_bounds__(BLOCK) void k_slot_lane(
       }
     }
     const uint32_t need_side = live && ok && nruns > 1 ? nruns + 1 : 0u;
-    const unsigned long long off = wave_alloc(need_side, &ctr->total);
+    const uint32_t part = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) & (SLOT_SIDE_PARTS - 1);
+    const unsigned long long loc = wave_alloc(need_side, &sa.ctr[part]);
     if (live && ok) {
-      if (need_side && off + need_side > side_cap) {
-        atomicOr(status, 1u);  // side array full: the build falls back
+      if (need_side && loc + need_side > sa.pcap) {
+        atomicOr(status, 1u);  // side partition full: the build falls back
       } else {
-        slot_store_runs(r, nruns, row, c_slot, c_side, off);
+        slot_store_runs(r, nruns, row, c_slot, c_side, part * sa.pcap + loc);
       }
     }
     wave_append(cls == 1, (uint32_t)row, mid_list, &ctr->n_mid);
@@ -1843,7 +1857,7 @@ template <int G>
 __device__ __forceinline__ void slot_row(uint64_t row, const Slots &A, const Slots &B,
                                          uint64_t *lds, uint32_t W, uint32_t lane,
                                          uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side,
-                                         unsigned long long side_cap, OpCounters *ctr,
+                                         const SideAlloc &sa, OpCounters *ctr,
                                          unsigned int *status, unsigned long long &nnz_acc,
                                          unsigned long long &ranges_acc, uint32_t plus_i) {
   const uint2 a = A.slot[row];
@@ -1958,14 +1972,16 @@ __device__ __forceinline__ void slot_row(uint64_t row, const Slots &A, const Slo
     if (lane == 0) c_slot[row] = make_uint2(lo, hi);
     return;
   }
-  unsigned long long off = 0;
-  if (lane == 0) off = atomicAdd(&ctr->total, (unsigned long long)runs + 1);
-  off = ((unsigned long long)(uint32_t)__shfl((int)(uint32_t)(off >> 32), 0, G) << 32) |
-        (uint32_t)__shfl((int)(uint32_t)off, 0, G);
-  if (off + runs + 1 > side_cap) {
+  const uint32_t part = blockIdx.x & (SLOT_SIDE_PARTS - 1);
+  unsigned long long loc = 0;
+  if (lane == 0) loc = atomicAdd(&sa.ctr[part], (unsigned long long)runs + 1);
+  loc = ((unsigned long long)(uint32_t)__shfl((int)(uint32_t)(loc >> 32), 0, G) << 32) |
+        (uint32_t)__shfl((int)(uint32_t)loc, 0, G);
+  if (loc + runs + 1 > sa.pcap) {
     if (lane == 0) atomicOr(status, 1u);
     return;
   }
+  const unsigned long long off = part * sa.pcap + loc;
   // a one-chunk window still holds the row's bits
   pass(true, 2 * (off + 1), nchunks > 1);
   if (lane == 0) {
@@ -1974,31 +1990,54 @@ __device__ __forceinline__ void slot_row(uint64_t row, const Slots &A, const Slo
   }
 }
 
+// One row list by groups of G lanes over blocks b of nb (nb a multiple of 8).
 template <int G>
-__global__ __launch_bounds__(BLOCK) void k_slot_group(
-    Slots A, Slots B, uint32_t W, uint32_t RPG, const uint32_t *__restrict__ rows,
+__device__ __forceinline__ void slot_group_rows(
+    const Slots &A, const Slots &B, uint32_t W, uint32_t RPG, const uint32_t *__restrict__ rows,
     const unsigned int *__restrict__ n_dev, uint2 *__restrict__ c_slot,
-    uint2 *__restrict__ c_side, unsigned long long side_cap, OpCounters *__restrict__ ctr,
-    unsigned int *__restrict__ status, uint32_t plus_i) {
-  extern __shared__ uint64_t lds_all[];
-  if (*status) return;
+    uint2 *__restrict__ c_side, const SideAlloc &sa, OpCounters *__restrict__ ctr,
+    unsigned int *__restrict__ status, uint32_t plus_i, uint64_t *lds_all, uint32_t b,
+    uint32_t nb) {
   constexpr uint32_t GPB = BLOCK / G;
   const uint32_t grp = threadIdx.x / G, lane = threadIdx.x % G;
   uint64_t *lds = lds_all + (size_t)grp * W;
   const uint64_t nl = *n_dev;
   unsigned long long nnz_acc = 0, ranges_acc = 0;
-  const uint64_t groups = (uint64_t)gridDim.x * GPB;
-  const uint64_t gid = (uint64_t)xcd_tile(blockIdx.x, gridDim.x) * GPB + grp;
+  const uint64_t groups = (uint64_t)nb * GPB;
+  const uint64_t gid = (uint64_t)xcd_tile(b, nb) * GPB + grp;
   for (uint64_t i0 = gid * RPG; i0 < nl; i0 += groups * RPG) {
     const uint64_t i1 = min(i0 + RPG, nl);
     for (uint64_t i = uniform_u64<G>(i0); i < i1; ++i) {
       const uint64_t row = uniform_u64<G>(rows[i]);
-      slot_row<G>(row, A, B, lds, W, lane, c_slot, c_side, side_cap, ctr, status, nnz_acc,
+      slot_row<G>(row, A, B, lds, W, lane, c_slot, c_side, sa, ctr, status, nnz_acc,
                   ranges_acc, plus_i);
     }
   }
   block_add_u64(nnz_acc, &ctr->nnz);
   block_add_u64(ranges_acc, &ctr->ranges);
+}
+
+// Both group tiers of a product in one launch: blocks [0, nb1) take the
+// medium rows with groups of G1 lanes, the next nb2 blocks the large rows
+// with 64-lane groups.
+template <int G1>
+__global__ __launch_bounds__(BLOCK) void k_slot_groups(
+    Slots A, Slots B, uint32_t W1, uint32_t W2, uint32_t RPG, const uint32_t *__restrict__ rows1,
+    const unsigned int *__restrict__ n1, uint32_t nb1, const uint32_t *__restrict__ rows2,
+    const unsigned int *__restrict__ n2, uint32_t nb2, uint2 *__restrict__ c_slot,
+    uint2 *__restrict__ c_side, SideAlloc sa, OpCounters *__restrict__ ctr,
+    unsigned int *__restrict__ status, uint32_t plus_i) {
+  extern __shared__ uint64_t lds_all[];
+  if (*status) return;
+  if (blockIdx.x < nb1) {
+    if (*n1 == 0) return;
+    slot_group_rows<G1>(A, B, W1, RPG, rows1, n1, c_slot, c_side, sa, ctr, status, plus_i,
+                        lds_all, blockIdx.x, nb1);
+  } else {
+    if (*n2 == 0) return;
+    slot_group_rows<64>(A, B, W2, RPG, rows2, n2, c_slot, c_side, sa, ctr, status, plus_i,
+                        lds_all, blockIdx.x - nb1, nb2);
+  }
 }
 
 // Interval summaries of a slot matrix (as k_interval_summary).
@@ -2038,36 +2077,51 @@ __global__ __launch_bounds__(BLOCK) void k_slot_summary(Slots B, uint64_t nrows,
   }
 }
 
-// rCRS -> slots (a row of several ranges into the side array).
+// rCRS -> slots (a row of several ranges into the side array).  Four rows
+// per lane: their bounds, then their first ranges, each loaded at once.
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_rcrs_to_slots(Mat<S> A, uint64_t nrows,
                                                          uint2 *__restrict__ slot,
-                                                         uint2 *__restrict__ side,
-                                                         unsigned long long side_cap,
-                                                         OpCounters *__restrict__ ctr,
+                                                         uint2 *__restrict__ side, SideAlloc sa,
                                                          unsigned int *__restrict__ status) {
-  for (uint64_t b0 = (uint64_t)blockIdx.x * BLOCK; b0 < nrows; b0 += (uint64_t)gridDim.x * BLOCK) {
-    const uint64_t row = b0 + threadIdx.x;
-    const bool live = row < nrows;
-    uint64_t p0 = 0, p1 = 0;
-    if (live) {
-      p0 = A.rm[row];
-      p1 = A.rm[row + 1];
+  constexpr int K = 4;
+  for (uint64_t b0 = (uint64_t)blockIdx.x * BLOCK * K; b0 < nrows;
+       b0 += (uint64_t)gridDim.x * BLOCK * K) {
+    const uint64_t r0 = b0 + (uint64_t)threadIdx.x * K;
+    S rb[K + 1];
+#pragma unroll
+    for (int k = 0; k <= K; ++k) rb[k] = r0 + k <= nrows ? A.rm[r0 + k] : (k ? rb[k - 1] : (S)0);
+    uint2 x[K];
+    uint64_t need64 = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint64_t nr = (uint64_t)(rb[k + 1] - rb[k]) / 2;
+      x[k] = nr ? ld_range(A.en, rb[k]) : make_uint2(SLOT_EMPTY, 0u);
+      need64 += nr > 1 ? nr + 1 : 0;
     }
-    const uint64_t nr = (p1 - p0) / 2;
-    const uint32_t need_side = nr > 1 ? (uint32_t)nr + 1 : 0u;
-    const unsigned long long off = wave_alloc(need_side, &ctr->total);
-    if (!live) continue;
-    if (nr == 0) {
-      slot[row] = make_uint2(SLOT_EMPTY, 0u);
-    } else if (nr == 1) {
-      slot[row] = ld_range(A.en, p0);
-    } else if (off + need_side > side_cap) {
+    // (more than a side partition holds: flagged below without a reservation)
+    const uint32_t need = need64 <= sa.pcap ? (uint32_t)need64 : 0u;
+    const uint32_t part = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) & (SLOT_SIDE_PARTS - 1);
+    const unsigned long long loc = wave_alloc(need, &sa.ctr[part]);
+    if (need64 > sa.pcap || (need && loc + need > sa.pcap)) {
       atomicOr(status, 1u);
-    } else {
-      side[off] = make_uint2((uint32_t)nr, 0u);
-      for (uint64_t q = 0; q < nr; ++q) side[off + 1 + q] = ld_range(A.en, p0 + 2 * q);
-      slot[row] = make_uint2(SLOT_MARK, (uint32_t)off);
+      continue;
+    }
+    unsigned long long off = part * sa.pcap + loc;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint64_t row = r0 + k;
+      if (row >= nrows) continue;
+      const uint64_t nr = (uint64_t)(rb[k + 1] - rb[k]) / 2;
+      if (nr <= 1) {
+        slot[row] = x[k];
+      } else {
+        side[off] = make_uint2((uint32_t)nr, 0u);
+        side[off + 1] = x[k];
+        for (uint64_t q = 1; q < nr; ++q) side[off + 1 + q] = ld_range(A.en, rb[k] + 2 * q);
+        slot[row] = make_uint2(SLOT_MARK, (uint32_t)off);
+        off += nr + 1;
+      }
     }
   }
 }
@@ -2082,8 +2136,7 @@ __global__ __launch_bounds__(BLOCK) void k_slot_identity(uint64_t n, uint2 *__re
 // union of A's row and {i}).
 __global__ __launch_bounds__(BLOCK) void k_slot_plus_identity(
     Slots A, uint64_t nrows, uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side,
-    unsigned long long side_cap, OpCounters *__restrict__ ctr, unsigned int *__restrict__ status,
-    unsigned long long *__restrict__ parts) {
+    SideAlloc sa, unsigned int *__restrict__ status, unsigned long long *__restrict__ parts) {
   if (*status) return;
   unsigned long long nnz_acc = 0, ranges_acc = 0;
   for (uint64_t b0 = (uint64_t)blockIdx.x * BLOCK; b0 < nrows; b0 += (uint64_t)gridDim.x * BLOCK) {
@@ -2107,7 +2160,9 @@ __global__ __launch_bounds__(BLOCK) void k_slot_plus_identity(
       u = make_uint2(min(a.x, i), max(a.y, i));
     }
     const uint32_t need_side = live && !one ? na + 2 : 0u;
-    const unsigned long long off = wave_alloc(need_side, &ctr->total);
+    const uint32_t part = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) & (SLOT_SIDE_PARTS - 1);
+    const unsigned long long loc = wave_alloc(need_side, &sa.ctr[part]);
+    const unsigned long long off = part * sa.pcap + loc;
     if (!live) continue;
     if (one) {
       c_slot[row] = u;
@@ -2115,7 +2170,7 @@ __global__ __launch_bounds__(BLOCK) void k_slot_plus_identity(
       ranges_acc += 1;
       continue;
     }
-    if (off + need_side > side_cap) {
+    if (loc + need_side > sa.pcap) {
       atomicOr(status, 1u);
       continue;
     }
@@ -2173,6 +2228,7 @@ __global__ __launch_bounds__(BLOCK) void k_slot_plus_identity(
 // blocks): one block per operation adds them to its counters.
 __global__ __launch_bounds__(BLOCK) void k_reduce_parts(const unsigned long long *__restrict__ parts,
                                                         uint32_t per_op,
+                                                        const unsigned long long *__restrict__ side_ctr,
                                                         OpCounters *__restrict__ ctr) {
   const unsigned long long *p = parts + 3ull * per_op * blockIdx.x;
   unsigned long long a = 0, b = 0, c = 0;
@@ -2184,10 +2240,18 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_parts(const unsigned long long
   a = block_sum_u64(a);
   b = block_sum_u64(b);
   c = block_sum_u64(c);
+  // side entries used: in total, and in the fullest partition
+  unsigned long long used = 0, most = 0;
+  if (threadIdx.x < SLOT_SIDE_PARTS) used = most = side_ctr[SLOT_SIDE_PARTS * blockIdx.x + threadIdx.x];
+  used = block_sum_u64(used);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) most = max(most, (unsigned long long)__shfl_xor(most, o, 64));
   if (threadIdx.x == 0) {
     ctr[blockIdx.x].nnz += a;
     ctr[blockIdx.x].slice += b;
     ctr[blockIdx.x].ranges += c;
+    ctr[blockIdx.x].total = used;
+    ctr[blockIdx.x].pad = (unsigned int)min(most, 0xFFFFFFFFull);
   }
 }
 
@@ -2505,15 +2569,16 @@ Slots slots_of(SlotView v) {
 }  // namespace
 
 void slots_from_rcrs(hipStream_t s, uint32_t sb, DView a, void *slot, void *side,
-                     uint64_t side_cap, OpCounters *ctr, unsigned int *status) {
+                     uint64_t side_cap, unsigned long long *side_ctr, unsigned int *status) {
   if (!a.nrows) return;
   uint2 *sl = static_cast<uint2 *>(slot), *sd = static_cast<uint2 *>(side);
+  const SideAlloc sa{side_ctr, side_cap / SLOT_SIDE_PARTS};
   if (sb == 4)
-    hipLaunchKernelGGL(k_rcrs_to_slots<uint32_t>, dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
-                       mat<uint32_t>(a), a.nrows, sl, sd, (unsigned long long)side_cap, ctr, status);
+    hipLaunchKernelGGL(k_rcrs_to_slots<uint32_t>, dim3(grid_for(a.nrows, BLOCK * 4)), dim3(BLOCK), 0, s,
+                       mat<uint32_t>(a), a.nrows, sl, sd, sa, status);
   else
-    hipLaunchKernelGGL(k_rcrs_to_slots<uint64_t>, dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
-                       mat<uint64_t>(a), a.nrows, sl, sd, (unsigned long long)side_cap, ctr, status);
+    hipLaunchKernelGGL(k_rcrs_to_slots<uint64_t>, dim3(grid_for(a.nrows, BLOCK * 4)), dim3(BLOCK), 0, s,
+                       mat<uint64_t>(a), a.nrows, sl, sd, sa, status);
 }
 
 void slots_identity(hipStream_t s, uint64_t n, void *slot) {
@@ -2523,12 +2588,12 @@ void slots_identity(hipStream_t s, uint64_t n, void *slot) {
 }
 
 void slots_plus_identity(hipStream_t s, SlotView a, uint64_t n, void *c_slot, void *c_side,
-                         uint64_t side_cap, OpCounters *ctr, unsigned int *status,
+                         uint64_t side_cap, unsigned long long *side_ctr, unsigned int *status,
                          unsigned long long *parts, uint32_t max_blocks) {
   if (!n) return;
   hipLaunchKernelGGL(k_slot_plus_identity, dim3(grid_for(n, BLOCK, max_blocks)), dim3(BLOCK), 0, s,
                      slots_of(a), n, static_cast<uint2 *>(c_slot), static_cast<uint2 *>(c_side),
-                     (unsigned long long)side_cap, ctr, status, parts);
+                     SideAlloc{side_ctr, side_cap / SLOT_SIDE_PARTS}, status, parts);
 }
 
 void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
@@ -2551,10 +2616,12 @@ void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
 
 void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t b_nrows,
                 uint32_t lane_max, uint32_t mid_max, uint32_t grid, void *c_slot, void *c_side,
-                uint64_t side_cap, uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
-                unsigned int *status, const void *summ, uint32_t summ_min, uint32_t summ_rows,
-                uint32_t plus_i, unsigned long long *parts) {
+                uint64_t side_cap, unsigned long long *side_ctr, uint32_t *mid_list,
+                uint32_t *big_list, OpCounters *ctr, unsigned int *status, const void *summ,
+                uint32_t summ_min, uint32_t summ_rows, uint32_t plus_i,
+                unsigned long long *parts) {
   if (!nrows) return;
+  const SideAlloc sa{side_ctr, side_cap / SLOT_SIDE_PARTS};
   const uint2 *pre = static_cast<const uint2 *>(summ);
   const uint2 *suf = pre ? pre + b_nrows : nullptr;
   const uint4 *pre2 = pre ? reinterpret_cast<const uint4 *>(pre + 2 * b_nrows) : nullptr;
@@ -2563,24 +2630,28 @@ void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t 
 #define DV_SLANE(M_)                                                                          \
   hipLaunchKernelGGL((k_slot_lane<M_>), dim3(grid), dim3(BLOCK), 0, s, slots_of(a), slots_of(b), \
                      nrows, lane_max, mid_max, static_cast<uint2 *>(c_slot),                  \
-                     static_cast<uint2 *>(c_side), (unsigned long long)side_cap, mid_list,    \
+                     static_cast<uint2 *>(c_side), sa, mid_list,                              \
                      big_list, ctr, status, pre, suf, pre2, suf2, summ_min, sh, plus_i, parts)
   if (pre) DV_SLANE(true);
   else DV_SLANE(false);
 #undef DV_SLANE
 }
 
-void slots_group(hipStream_t s, SlotView a, SlotView b, uint32_t lanes, uint32_t lds_words,
-                 uint32_t rows_per_group, uint32_t grid, const uint32_t *rows,
-                 const unsigned int *n_dev, void *c_slot, void *c_side, uint64_t side_cap,
-                 OpCounters *ctr, unsigned int *status, uint32_t plus_i) {
+void slots_groups(hipStream_t s, SlotView a, SlotView b, uint32_t lanes1, uint32_t lds1,
+                  uint32_t grid1, const uint32_t *rows1, const unsigned int *n1, uint32_t lds2,
+                  uint32_t grid2, const uint32_t *rows2, const unsigned int *n2,
+                  uint32_t rows_per_group, void *c_slot, void *c_side, uint64_t side_cap,
+                  unsigned long long *side_ctr, OpCounters *ctr, unsigned int *status,
+                  uint32_t plus_i) {
   uint2 *cs = static_cast<uint2 *>(c_slot), *cd = static_cast<uint2 *>(c_side);
-#define DV_SGRP(G_)                                                                           \
-  hipLaunchKernelGGL((k_slot_group<G_>), dim3(grid), dim3(BLOCK),                             \
-                     (size_t)(BLOCK / G_) * lds_words * 8, s, slots_of(a), slots_of(b), lds_words, \
-                     rows_per_group, rows, n_dev, cs, cd, (unsigned long long)side_cap, ctr,   \
-                     status, plus_i)
-  switch (lanes) {
+  const SideAlloc sa{side_ctr, side_cap / SLOT_SIDE_PARTS};
+  const size_t lds2_bytes = (size_t)(BLOCK / 64) * lds2 * 8;
+#define DV_SGRP(G_)                                                                             \
+  hipLaunchKernelGGL((k_slot_groups<G_>), dim3(grid1 + grid2), dim3(BLOCK),                     \
+                     std::max((size_t)(BLOCK / G_) * lds1 * 8, lds2_bytes), s, slots_of(a),      \
+                     slots_of(b), lds1, lds2, rows_per_group, rows1, n1, grid1, rows2, n2, grid2, \
+                     cs, cd, sa, ctr, status, plus_i)
+  switch (lanes1) {
     case 8: DV_SGRP(8); break;
     case 16: DV_SGRP(16); break;
     case 32: DV_SGRP(32); break;
@@ -2590,9 +2661,9 @@ void slots_group(hipStream_t s, SlotView a, SlotView b, uint32_t lanes, uint32_t
 }
 
 void slots_reduce_parts(hipStream_t s, const unsigned long long *parts, uint32_t per_op,
-                        uint32_t nops, OpCounters *ctr) {
+                        const unsigned long long *side_ctr, uint32_t nops, OpCounters *ctr) {
   if (!nops) return;
-  hipLaunchKernelGGL(k_reduce_parts, dim3(nops), dim3(BLOCK), 0, s, parts, per_op, ctr);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(nops), dim3(BLOCK), 0, s, parts, per_op, side_ctr, ctr);
 }
 
 void slots_counts(hipStream_t s, SlotView a, uint64_t n, uint32_t *cnt,

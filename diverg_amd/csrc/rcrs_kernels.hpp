@@ -165,20 +165,23 @@ void slice_lengths(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *len);
 // empty row, (0xFFFFFFFE, off) for a row of several ranges in the side array
 // (side[off] = (count, 0), then the ranges).  Side arrays hold side_cap
 // entries plus SLOT_SIDE_PAD spare ones (side_cap below 2^32 - 16: offsets
-// are 32-bit); ctr->total counts the entries used (a product that would pass
-// side_cap sets bit 0 of *status, and every later kernel of the build then
-// does nothing), ctr->ranges the ranges written.
+// are 32-bit), reserved in SLOT_SIDE_PARTS partitions of side_cap /
+// SLOT_SIDE_PARTS entries, each counted in its own word of side_ctr (a
+// product that would overfill a partition sets bit 0 of *status, and every
+// later kernel of the build then does nothing); ctr->ranges counts the ranges
+// written.
 constexpr uint64_t SLOT_SIDE_PAD = 8;
+constexpr uint32_t SLOT_SIDE_PARTS = 64;
 struct SlotView {
   const void *slot;
   const void *side;
 };
 void slots_from_rcrs(hipStream_t s, uint32_t sb, DView a, void *slot, void *side,
-                     uint64_t side_cap, OpCounters *ctr, unsigned int *status);
+                     uint64_t side_cap, unsigned long long *side_ctr, unsigned int *status);
 void slots_identity(hipStream_t s, uint64_t n, void *slot);
 // C = I + A; parts as slots_lane (at most max_blocks blocks)
 void slots_plus_identity(hipStream_t s, SlotView a, uint64_t n, void *c_slot, void *c_side,
-                         uint64_t side_cap, OpCounters *ctr, unsigned int *status,
+                         uint64_t side_cap, unsigned long long *side_ctr, unsigned int *status,
                          unsigned long long *parts, uint32_t max_blocks);
 // Interval summaries of B (layout and block_rows as block_summary).
 void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
@@ -188,16 +191,24 @@ void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
 // for slots_group.  parts receives 3 words per block (nnz, slice, ranges).
 void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t b_nrows,
                 uint32_t lane_max, uint32_t mid_max, uint32_t grid, void *c_slot, void *c_side,
-                uint64_t side_cap, uint32_t *mid_list, uint32_t *big_list, OpCounters *ctr,
-                unsigned int *status, const void *summ, uint32_t summ_min, uint32_t summ_rows,
-                uint32_t plus_i, unsigned long long *parts);
-void slots_group(hipStream_t s, SlotView a, SlotView b, uint32_t lanes, uint32_t lds_words,
-                 uint32_t rows_per_group, uint32_t grid, const uint32_t *rows,
-                 const unsigned int *n_dev, void *c_slot, void *c_side, uint64_t side_cap,
-                 OpCounters *ctr, unsigned int *status, uint32_t plus_i);
-// ctr[i] += the sums of parts[3 per_op i ..] (one block per operation).
+                uint64_t side_cap, unsigned long long *side_ctr, uint32_t *mid_list,
+                uint32_t *big_list, OpCounters *ctr, unsigned int *status, const void *summ,
+                uint32_t summ_min, uint32_t summ_rows, uint32_t plus_i,
+                unsigned long long *parts);
+// Both group tiers of a product in one launch: rows1 (*n1 of them) by groups
+// of lanes1 lanes (LDS window lds1 words) on grid1 blocks, rows2 by 64-lane
+// groups (lds2) on grid2 blocks; grids multiples of 8.
+void slots_groups(hipStream_t s, SlotView a, SlotView b, uint32_t lanes1, uint32_t lds1,
+                  uint32_t grid1, const uint32_t *rows1, const unsigned int *n1, uint32_t lds2,
+                  uint32_t grid2, const uint32_t *rows2, const unsigned int *n2,
+                  uint32_t rows_per_group, void *c_slot, void *c_side, uint64_t side_cap,
+                  unsigned long long *side_ctr, OpCounters *ctr, unsigned int *status,
+                  uint32_t plus_i);
+// ctr[i] += the sums of parts[3 per_op i ..]; ctr[i].total = the side
+// entries used (side_ctr[SLOT_SIDE_PARTS i ..]), ctr[i].pad those of the
+// fullest partition (one block per operation).
 void slots_reduce_parts(hipStream_t s, const unsigned long long *parts, uint32_t per_op,
-                        uint32_t nops, OpCounters *ctr);
+                        const unsigned long long *side_ctr, uint32_t nops, OpCounters *ctr);
 // Slots -> rCRS: per-row entry counts, and the entries at their row pointers.
 void slots_counts(hipStream_t s, SlotView a, uint64_t n, uint32_t *cnt,
                   const unsigned int *status);
