@@ -280,9 +280,12 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   release_all();
   // Rows of several ranges cost the slot form a dependent read each (the
   // side array), where range CRS reads them with the rest of a slice: when
-  // they become common (a product with more side entries than SIDE_SHARE of
-  // its rows), later builds of this shape use range CRS.
-  constexpr double SIDE_SHARE = 0.05;
+  // they dominate (a product with more side entries than SIDE_SHARE of its
+  // rows), later builds of this shape use range CRS.  Measured: slot form
+  // is faster up to ~0.6 (the 1e5-locus graph of [150, 450]: 0.70 vs
+  // 0.97 ms; the chromosome graph, up to 0.43: 29 vs 32 ms) and slower at
+  // 0.82 (a graph of 8% SNPs and 2% indels: 1.64 vs 1.40 ms).
+  constexpr double SIDE_SHARE = 0.75;
   if (tuning.layout == 0 && use > SIDE_SHARE)
     rcrs_shapes[{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup}] = true;
   return DV_OK;

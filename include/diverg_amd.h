@@ -147,9 +147,9 @@ void *dv_stream(dv_context *ctx);
  *                   row-pointer scan; only the index is converted to range
  *                   CRS; 1: every product writes range CRS; 0: slot form,
  *                   except that a build shape whose products had side
- *                   entries for more than 5% of their rows is built in range
- *                   CRS from then on (a row of several ranges costs the slot
- *                   form a dependent read) */
+ *                   entries for more than 75% of their rows is built in
+ *                   range CRS from then on (a row of several ranges costs the
+ *                   slot form a dependent read) */
 typedef struct dv_tuning {
   uint32_t lds_words;
   uint32_t stage_entries;
