@@ -132,20 +132,19 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     const PlanOp &o = p.ops[i];
     PlanSlot &c = p.slots[o.c];
     OpCounters *ctr = d_plan_ctr + i;
-    if (timing) (void)hipEventRecord(plan_ev[5 * i], stream);
+    // timing: two events per SpAdd / SpGEMM, around the kernel it is
+    // priced on (every timed marker holds the queue for a few microseconds)
     if (o.kind == P_IDENT) {
       slots_identity(stream, c.nrows, c.rowmap);
-      if (timing)
-        for (int e = 1; e < 5; ++e) (void)hipEventRecord(plan_ev[5 * i + e], stream);
       continue;
     }
     if (o.kind == P_SPADD) {
       const PlanSlot &xs = p.slots[p.slots[o.a].kind == P_IDENT ? o.b : o.a];
+      if (timing) (void)hipEventRecord(plan_ev[2 * i], stream);
       slots_plus_identity(stream, view_of(xs), c.nrows, c.rowmap, c.entries, c.cap,
                           side_ctr + (uint64_t)SLOT_SIDE_PARTS * i, d_status,
                           slot_parts + 3ull * per_op * i, per_op);
-      if (timing)
-        for (int e = 1; e < 5; ++e) (void)hipEventRecord(plan_ev[5 * i + e], stream);
+      if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
       continue;
     }
     const PlanSlot &A = p.slots[o.a], &B = p.slots[o.b];
@@ -154,7 +153,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
                              : 0u;
     if (B.nrows > s.rows) summ_rows = 0;
     if (summ_rows) slots_summary(stream, view_of(B), B.nrows, s.summ, d_status, summ_rows);
-    if (timing) (void)hipEventRecord(plan_ev[5 * i + 1], stream);
+    if (timing) (void)hipEventRecord(plan_ev[2 * i], stream);
     const uint64_t n = c.nrows;
     const uint64_t resident = (uint64_t)num_cu * slot_lane_blocks_per_cu(summ_rows != 0);
     const uint32_t grid =
@@ -164,15 +163,11 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
                tuning.small_slice_max, grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big,
                ctr, d_status, summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows,
                o.plus_i ? 1u : 0u, slot_parts + 3ull * per_op * i);
-    if (timing) (void)hipEventRecord(plan_ev[5 * i + 2], stream);
+    if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
     const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);
     slots_groups(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.grid, s.mid, &ctr->n_mid,
                  gb.lds_words, gb.grid, s.big, &ctr->n_big, gm.rows_per_group, c.rowmap, c.entries,
                  c.cap, op_side, ctr, d_status, o.plus_i ? 1u : 0u);
-    if (timing) {
-      (void)hipEventRecord(plan_ev[5 * i + 3], stream);
-      (void)hipEventRecord(plan_ev[5 * i + 4], stream);
-    }
   }
   slots_reduce_parts(stream, slot_parts, per_op, side_ctr, (uint32_t)nops, d_plan_ctr);
   const PlanSlot &O = p.slots[p.out];
@@ -242,6 +237,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   for (int i = 0; i < nops; ++i)
     ranges[p.ops[i].c] = p.ops[i].kind == P_IDENT ? p.slots[p.ops[i].c].nrows : h_plan_ctr[i].ranges;
   auto bytes = [&](int sl) { return (uint64_t)p.slots[sl].sb * (p.slots[sl].nrows + 1) + 8 * ranges[sl]; };
+  int prev_ev = -1;  // the last timed event recorded, after a SpAdd or a lane
   for (int i = 0; i < nops; ++i) {
     const PlanOp &o = p.ops[i];
     if (o.kind == P_IDENT) continue;
@@ -258,19 +254,22 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
       stats_n_spadd += 1;
     }
     if (timing) {
-      float t[4] = {0, 0, 0, 0};
-      for (int e = 0; e < 4; ++e)
-        (void)hipEventElapsedTime(&t[e], plan_ev[5 * i + e], plan_ev[5 * i + e + 1]);
+      // own kernel, and the kernels since the previous timed one (the
+      // previous product's group tiers, this product's summaries)
+      float own = 0, between = 0;
+      (void)hipEventElapsedTime(&own, plan_ev[2 * i], plan_ev[2 * i + 1]);
+      if (prev_ev >= 0) (void)hipEventElapsedTime(&between, plan_ev[prev_ev], plan_ev[2 * i]);
+      prev_ev = 2 * i + 1;
+      prof_t[2].total_ms += between;
       if (o.kind == P_SPGEMM) {
         prof_t[0].launches += 1;
-        prof_t[0].total_ms += t[1];
+        prof_t[0].total_ms += own;
         prof_t[0].alg_bytes += alg;
         prof_t[0].slice_bytes += 8ull * h.slice;
         prof_t[2].launches += 1;
-        prof_t[2].total_ms += t[0] + t[2];
       } else {
         prof_t[3].launches += 1;
-        prof_t[3].total_ms += t[0] + t[1] + t[2] + t[3];
+        prof_t[3].total_ms += own;
         prof_t[3].alg_bytes += alg;
       }
     }
