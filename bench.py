@@ -248,7 +248,8 @@ def main():
         "bit_exact": bit_exact,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "kernels_ms": {k: round(v["total_ms"] / args.steps, 4) for k, v in prof.items()},
+        # per build, over the profiled builds
+        "kernels_ms": {k: round(v["total_ms"] / nprof, 4) for k, v in prof.items()},
     }
     legs = set(args.legs.split(","))
     if "chromosome" in legs or "serving" in legs or "queries" in legs:
