@@ -125,7 +125,15 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
            (uint32_t)((nops + 2) * sizeof(OpCounters) / 4), 0u);
   fill_u32(stream, d_status, 1, 0u);
-  fill_u32(stream, reinterpret_cast<uint32_t *>(slot_parts), 2 * (parts_words + ctr_words), 0u);
+  // per-block parts: each launched block writes its own, so only the side
+  // counters are cleared (all parts too when the operations are too many
+  // to pass their block counts)
+  PartBlocks used{};
+  used.all = nops > (int)PART_BLOCKS_MAX_OPS ? 1u : 0u;
+  if (used.all)
+    fill_u32(stream, reinterpret_cast<uint32_t *>(slot_parts), 2 * (parts_words + ctr_words), 0u);
+  else
+    fill_u32(stream, reinterpret_cast<uint32_t *>(side_ctr), 2 * ctr_words, 0u);
   slots_from_rcrs(stream, a.sb, view(a), in.rowmap, in.entries, in.cap,
                   side_ctr + (uint64_t)SLOT_SIDE_PARTS * nops, d_status);
   for (int i = 0; i < nops; ++i) {
@@ -144,6 +152,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
       slots_plus_identity(stream, view_of(xs), c.nrows, c.rowmap, c.entries, c.cap,
                           side_ctr + (uint64_t)SLOT_SIDE_PARTS * i, d_status,
                           slot_parts + 3ull * per_op * i, per_op);
+      if (!used.all) used.n[i] = slots_plus_identity_blocks(c.nrows, per_op);
       if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
       continue;
     }
@@ -163,13 +172,14 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
                tuning.small_slice_max, grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big,
                ctr, d_status, summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows,
                o.plus_i ? 1u : 0u, slot_parts + 3ull * per_op * i);
+    if (!used.all) used.n[i] = n ? grid : 0u;
     if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
     const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);
     slots_groups(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.grid, s.mid, &ctr->n_mid,
                  gb.lds_words, gb.grid, s.big, &ctr->n_big, gm.rows_per_group, c.rowmap, c.entries,
                  c.cap, op_side, ctr, d_status, o.plus_i ? 1u : 0u);
   }
-  slots_reduce_parts(stream, slot_parts, per_op, side_ctr, (uint32_t)nops, d_plan_ctr);
+  slots_reduce_parts(stream, slot_parts, per_op, used, side_ctr, (uint32_t)nops, d_plan_ctr);
   const PlanSlot &O = p.slots[p.out];
   slots_counts(stream, view_of(O), O.nrows, s.cnt, d_status);
   scan_rowmap(stream, d.sb, s.cnt, O.nrows, d.rowmap, s.tmp, c_out, ~0ull, d_status, nullptr, 0,

@@ -2224,15 +2224,17 @@ __global__ __launch_bounds__(BLOCK) void k_slot_plus_identity(
 }
 
 // Per-block (nnz, slice, ranges) sums of the lane-tier launches, per_op
-// triples for each of nops operations (zero where a launch had fewer
-// blocks): one block per operation adds them to its counters.
+// triples for each of nops operations, of which the first blocks.n[i] were
+// written by operation i's launch (every block of it writes its triple):
+// one block per operation adds them to its counters.
 __global__ __launch_bounds__(BLOCK) void k_reduce_parts(const unsigned long long *__restrict__ parts,
-                                                        uint32_t per_op,
+                                                        uint32_t per_op, PartBlocks blocks,
                                                         const unsigned long long *__restrict__ side_ctr,
                                                         OpCounters *__restrict__ ctr) {
   const unsigned long long *p = parts + 3ull * per_op * blockIdx.x;
+  const uint32_t nb = blocks.all ? per_op : blocks.n[blockIdx.x];
   unsigned long long a = 0, b = 0, c = 0;
-  for (uint32_t j = threadIdx.x; j < per_op; j += BLOCK) {
+  for (uint32_t j = threadIdx.x; j < nb; j += BLOCK) {
     a += p[3 * j];
     b += p[3 * j + 1];
     c += p[3 * j + 2];
@@ -2587,11 +2589,15 @@ void slots_identity(hipStream_t s, uint64_t n, void *slot) {
                      static_cast<uint2 *>(slot));
 }
 
+uint32_t slots_plus_identity_blocks(uint64_t n, uint32_t max_blocks) {
+  return n ? grid_for(n, BLOCK, max_blocks) : 0u;
+}
+
 void slots_plus_identity(hipStream_t s, SlotView a, uint64_t n, void *c_slot, void *c_side,
                          uint64_t side_cap, unsigned long long *side_ctr, unsigned int *status,
                          unsigned long long *parts, uint32_t max_blocks) {
   if (!n) return;
-  hipLaunchKernelGGL(k_slot_plus_identity, dim3(grid_for(n, BLOCK, max_blocks)), dim3(BLOCK), 0, s,
+  hipLaunchKernelGGL(k_slot_plus_identity, dim3(slots_plus_identity_blocks(n, max_blocks)), dim3(BLOCK), 0, s,
                      slots_of(a), n, static_cast<uint2 *>(c_slot), static_cast<uint2 *>(c_side),
                      SideAlloc{side_ctr, side_cap / SLOT_SIDE_PARTS}, status, parts);
 }
@@ -2661,9 +2667,11 @@ void slots_groups(hipStream_t s, SlotView a, SlotView b, uint32_t lanes1, uint32
 }
 
 void slots_reduce_parts(hipStream_t s, const unsigned long long *parts, uint32_t per_op,
-                        const unsigned long long *side_ctr, uint32_t nops, OpCounters *ctr) {
+                        const PartBlocks &used, const unsigned long long *side_ctr,
+                        uint32_t nops, OpCounters *ctr) {
   if (!nops) return;
-  hipLaunchKernelGGL(k_reduce_parts, dim3(nops), dim3(BLOCK), 0, s, parts, per_op, side_ctr, ctr);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(nops), dim3(BLOCK), 0, s, parts, per_op, used,
+                     side_ctr, ctr);
 }
 
 void slots_counts(hipStream_t s, SlotView a, uint64_t n, uint32_t *cnt,

@@ -179,7 +179,9 @@ struct SlotView {
 void slots_from_rcrs(hipStream_t s, uint32_t sb, DView a, void *slot, void *side,
                      uint64_t side_cap, unsigned long long *side_ctr, unsigned int *status);
 void slots_identity(hipStream_t s, uint64_t n, void *slot);
-// C = I + A; parts as slots_lane (at most max_blocks blocks)
+// C = I + A; parts as slots_lane (at most max_blocks blocks: the launch
+// has slots_plus_identity_blocks(n, max_blocks))
+uint32_t slots_plus_identity_blocks(uint64_t n, uint32_t max_blocks);
 void slots_plus_identity(hipStream_t s, SlotView a, uint64_t n, void *c_slot, void *c_side,
                          uint64_t side_cap, unsigned long long *side_ctr, unsigned int *status,
                          unsigned long long *parts, uint32_t max_blocks);
@@ -204,11 +206,20 @@ void slots_groups(hipStream_t s, SlotView a, SlotView b, uint32_t lanes1, uint32
                   uint32_t rows_per_group, void *c_slot, void *c_side, uint64_t side_cap,
                   unsigned long long *side_ctr, OpCounters *ctr, unsigned int *status,
                   uint32_t plus_i);
-// ctr[i] += the sums of parts[3 per_op i ..]; ctr[i].total = the side
-// entries used (side_ctr[SLOT_SIDE_PARTS i ..]), ctr[i].pad those of the
-// fullest partition (one block per operation).
+// Blocks that wrote parts, per operation (a kernel argument): n[i] for
+// operation i, or per_op for every operation when `all` is set (then the
+// unused parts must have been cleared).
+constexpr uint32_t PART_BLOCKS_MAX_OPS = 64;
+struct PartBlocks {
+  uint32_t all;
+  uint32_t n[PART_BLOCKS_MAX_OPS];
+};
+// ctr[i] += the sums of parts[3 per_op i ..] (the first used.n[i] blocks'
+// words); ctr[i].total = the side entries used (side_ctr[SLOT_SIDE_PARTS i
+// ..]), ctr[i].pad those of the fullest partition (one block per operation).
 void slots_reduce_parts(hipStream_t s, const unsigned long long *parts, uint32_t per_op,
-                        const unsigned long long *side_ctr, uint32_t nops, OpCounters *ctr);
+                        const PartBlocks &used, const unsigned long long *side_ctr,
+                        uint32_t nops, OpCounters *ctr);
 // Slots -> rCRS: per-row entry counts, and the entries at their row pointers.
 void slots_counts(hipStream_t s, SlotView a, uint64_t n, uint32_t *cnt,
                   const unsigned int *status);
