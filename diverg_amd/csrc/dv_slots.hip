@@ -164,14 +164,16 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     if (summ_rows) slots_summary(stream, view_of(B), B.nrows, s.summ, d_status, summ_rows);
     if (timing) (void)hipEventRecord(plan_ev[2 * i], stream);
     const uint64_t n = c.nrows;
-    const uint64_t resident = (uint64_t)num_cu * slot_lane_blocks_per_cu(summ_rows != 0);
+    // slice loads per round trip: about the widest A range
+    const uint32_t batch = A.span <= 2 ? 2u : A.span <= 4 ? 4u : 8u;
+    const uint64_t resident = (uint64_t)num_cu * slot_lane_blocks_per_cu(summ_rows != 0, batch);
     const uint32_t grid =
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(n + 255) / 256, resident, per_op}));
     unsigned long long *op_side = side_ctr + (uint64_t)SLOT_SIDE_PARTS * i;
     slots_lane(stream, view_of(A), view_of(B), n, B.nrows, tuning.lane_slice_max,
                tuning.small_slice_max, grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big,
                ctr, d_status, summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows,
-               o.plus_i ? 1u : 0u, slot_parts + 3ull * per_op * i);
+               o.plus_i ? 1u : 0u, slot_parts + 3ull * per_op * i, batch);
     if (!used.all) used.n[i] = n ? grid : 0u;
     if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
     const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);

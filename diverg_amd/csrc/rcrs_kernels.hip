@@ -1700,6 +1700,59 @@ __device__ __forceinline__ void fold_slots(const Slots &B, uint32_t s, uint32_t 
   }
 }
 
+// fold_slots for narrow slices, NB (2 or 4) loads per round trip, with
+// fewer instructions per slot: loads past e repeat slot e (a union takes a
+// repeated range unchanged, so no load is conditional) and one test per
+// slot checks that it is a range touching the running interval as it
+// grows.  A batch with any other slot (an empty or marked row, a gap) is
+// redone from the same state through fold(), its marked rows' side ranges
+// after it, so the result is fold()'s either way.  (For 8-slot batches
+// this form takes more registers than the lane kernel can spare.)
+template <int NB>
+__device__ __forceinline__ void fold_slots_tight(const Slots &B, uint32_t s, uint32_t e,
+                                                 uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
+                                                 unsigned long long &gathered) {
+  gathered += (unsigned long long)(e - s) + 1;
+  for (uint32_t j = s; ok && j <= e;) {
+    const uint32_t rem = e - j;  // rows after j in the slice (no index past e is formed)
+    uint2 v[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) v[k] = B.slot[j + min((uint32_t)k, rem)];
+    // an empty running interval starts as the first slot
+    const bool none = cl > ch;
+    uint32_t ncl = none ? v[0].x : cl, nch = none ? v[0].y : ch;
+    bool all = true;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      all = all && v[k].x <= v[k].y && v[k].x <= nch + 1u && v[k].y + 1u >= ncl;
+      ncl = min(ncl, v[k].x);
+      nch = max(nch, v[k].y);
+    }
+    if (all) {
+      cl = ncl;
+      ch = nch;
+    } else {
+      uint32_t mk = 0, offs[NB];
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        if ((uint32_t)k <= rem) fold(v[k].x, v[k].y, cl, ch, r, ok);
+        mk |= (uint32_t)k <= rem && v[k].x == SLOT_MARK ? 1u << k : 0u;
+        offs[k] = v[k].y;
+      }
+      while (ok && mk) {
+        const uint32_t k = (uint32_t)__builtin_ctz(mk);
+        mk &= mk - 1;
+        uint32_t off = offs[0];
+#pragma unroll
+        for (int q = 1; q < NB; ++q) off = k == (uint32_t)q ? offs[q] : off;
+        fold_side(B.side, off, cl, ch, r, ok, gathered);
+      }
+    }
+    j = rem >= (uint32_t)NB ? j + NB : e + 1;
+    if (j == 0) break;  // (e + 1 wrapped: e was the last ordinal)
+  }
+}
+
 // Write a lane-tier row (slot) from its compacted runs: one range in place,
 // several in the side array (reserved by the caller: off, `need` entries).
 __device__ __forceinline__ void slot_store_runs(const Runs &r, uint32_t nruns, uint64_t row,
@@ -1723,15 +1776,22 @@ __device__ __forceinline__ void slot_store_runs(const Runs &r, uint32_t nruns, u
 // whose union has more than LANE_SLOTS runs or whose slices are long go to
 // the group lists as in k_spgemm_lane.  A two-stage pipeline loads the A
 // slot of the row after next while this row folds.
-template <bool SUMM>
-__global__ __launch_bounds__(BLOCK) void k_slot_lane(
-    Slots A, Slots B, uint64_t nrows, uint32_t lane_max, uint32_t mid_max,
-    uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side, SideAlloc sa,
-    uint32_t *__restrict__ mid_list, uint32_t *__restrict__ big_list,
-    OpCounters *__restrict__ ctr, unsigned int *__restrict__ status,
-    const uint2 *__restrict__ pre, const uint2 *__restrict__ suf,
-    const uint4 *__restrict__ pre2, const uint4 *__restrict__ suf2, uint32_t summ_min,
-    uint32_t summ_shift, uint32_t plus_i, unsigned long long *__restrict__ parts) {
+#define DV_SLOT_LANE_PARAMS                                                                \
+  Slots A, Slots B, uint64_t nrows, uint32_t lane_max, uint32_t mid_max,                   \
+      uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side, SideAlloc sa,                \
+      uint32_t *__restrict__ mid_list, uint32_t *__restrict__ big_list,                    \
+      OpCounters *__restrict__ ctr, unsigned int *__restrict__ status,                     \
+      const uint2 *__restrict__ pre, const uint2 *__restrict__ suf,                        \
+      const uint4 *__restrict__ pre2, const uint4 *__restrict__ suf2, uint32_t summ_min,   \
+      uint32_t summ_shift, uint32_t plus_i, unsigned long long *__restrict__ parts
+#define DV_SLOT_LANE_ARGS                                                                  \
+  A, B, nrows, lane_max, mid_max, c_slot, c_side, sa, mid_list, big_list, ctr, status, pre, \
+      suf, pre2, suf2, summ_min, summ_shift, plus_i, parts
+
+// NB: slice loads per round trip (LANE_BATCH, or 2 / 4 through
+// fold_slots_tight for products of narrow slices).
+template <bool SUMM, int NB>
+__device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
   if (*status) return;
   unsigned long long nnz_acc = 0, slice_acc = 0, ranges_acc = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -1790,6 +1850,8 @@ __global__ __launch_bounds__(BLOCK) void k_slot_lane(
             pre, suf, pre2, suf2, s, e, summ_shift,
             [&](uint32_t r0, uint32_t r1) { fold_slots(B, r0, r1 - 1, cl, ch, r, ok, slice_acc); },
             cl, ch, r, ok, slice_acc);
+      } else if constexpr (NB < LANE_BATCH) {
+        fold_slots_tight<NB>(B, s, e, cl, ch, r, ok, slice_acc);
       } else {
         fold_slots(B, s, e, cl, ch, r, ok, slice_acc);
       }
@@ -1848,6 +1910,19 @@ __global__ __launch_bounds__(BLOCK) void k_slot_lane(
     parts[3 * blockIdx.x + 1] = slice_acc;
     parts[3 * blockIdx.x + 2] = ranges_acc;
   }
+}
+
+template <bool SUMM>
+__global__ __launch_bounds__(BLOCK) void k_slot_lane(DV_SLOT_LANE_PARAMS) {
+  slot_lane_body<SUMM, LANE_BATCH>(DV_SLOT_LANE_ARGS);
+}
+
+// Narrow slices (2 or 4 slots a batch): six waves per SIMD measured faster
+// than five, even with the few bytes the register cap spills.
+template <int NB>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_slot_lane_narrow(
+    DV_SLOT_LANE_PARAMS) {
+  slot_lane_body<false, NB>(DV_SLOT_LANE_ARGS);
 }
 
 // Group tiers on slots: one row of C by G lanes with an LDS bit window, as
@@ -2625,7 +2700,7 @@ void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t 
                 uint64_t side_cap, unsigned long long *side_ctr, uint32_t *mid_list,
                 uint32_t *big_list, OpCounters *ctr, unsigned int *status, const void *summ,
                 uint32_t summ_min, uint32_t summ_rows, uint32_t plus_i,
-                unsigned long long *parts) {
+                unsigned long long *parts, uint32_t batch) {
   if (!nrows) return;
   const SideAlloc sa{side_ctr, side_cap / SLOT_SIDE_PARTS};
   const uint2 *pre = static_cast<const uint2 *>(summ);
@@ -2633,13 +2708,15 @@ void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t 
   const uint4 *pre2 = pre ? reinterpret_cast<const uint4 *>(pre + 2 * b_nrows) : nullptr;
   const uint4 *suf2 = pre ? pre2 + b_nrows : nullptr;
   const uint32_t sh = (uint32_t)__builtin_ctz(summ_rows ? summ_rows : 1);
-#define DV_SLANE(M_)                                                                          \
-  hipLaunchKernelGGL((k_slot_lane<M_>), dim3(grid), dim3(BLOCK), 0, s, slots_of(a), slots_of(b), \
-                     nrows, lane_max, mid_max, static_cast<uint2 *>(c_slot),                  \
-                     static_cast<uint2 *>(c_side), sa, mid_list,                              \
-                     big_list, ctr, status, pre, suf, pre2, suf2, summ_min, sh, plus_i, parts)
-  if (pre) DV_SLANE(true);
-  else DV_SLANE(false);
+#define DV_SLANE(K_)                                                                       \
+  hipLaunchKernelGGL(K_, dim3(grid), dim3(BLOCK), 0, s, slots_of(a), slots_of(b), nrows,     \
+                     lane_max, mid_max, static_cast<uint2 *>(c_slot),                        \
+                     static_cast<uint2 *>(c_side), sa, mid_list, big_list, ctr, status, pre, \
+                     suf, pre2, suf2, summ_min, sh, plus_i, parts)
+  if (pre) DV_SLANE(k_slot_lane<true>);
+  else if (batch <= 2) DV_SLANE(k_slot_lane_narrow<2>);
+  else if (batch <= 4) DV_SLANE(k_slot_lane_narrow<4>);
+  else DV_SLANE(k_slot_lane<false>);
 #undef DV_SLANE
 }
 
@@ -2692,13 +2769,16 @@ void slots_emit(hipStream_t s, SlotView a, uint64_t n, uint32_t sb, const void *
                        static_cast<const uint64_t *>(rowmap), entries);
 }
 
-uint32_t slot_lane_blocks_per_cu(bool summ) {
-  static int cache[2] = {0, 0};
-  const int k = summ ? 1 : 0;
+uint32_t slot_lane_blocks_per_cu(bool summ, uint32_t batch) {
+  static int cache[4] = {0, 0, 0, 0};
+  const int k = summ ? 3 : batch <= 2 ? 0 : batch <= 4 ? 1 : 2;
   if (!cache[k]) {
     int nb = 0;
-    const hipError_t e = summ ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<true>, BLOCK, 0)
-                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<false>, BLOCK, 0);
+    const hipError_t e =
+        k == 3   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<true>, BLOCK, 0)
+        : k == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane_narrow<2>, BLOCK, 0)
+        : k == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane_narrow<4>, BLOCK, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<false>, BLOCK, 0);
     if (e != hipSuccess || nb < 1) {
       (void)hipGetLastError();
       nb = 4;

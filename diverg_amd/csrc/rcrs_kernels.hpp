@@ -191,12 +191,14 @@ void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
 // Lane tier of C = A.B (A (B + I) with plus_i): rows it finishes are
 // written; others go to mid_list / big_list (counts in ctr->n_mid, n_big)
 // for slots_group.  parts receives 3 words per block (nnz, slice, ranges).
+// batch (without summaries): slice loads per round trip, 2, 4 or 8, about
+// the widest A range (narrow products then run fewer instructions).
 void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t b_nrows,
                 uint32_t lane_max, uint32_t mid_max, uint32_t grid, void *c_slot, void *c_side,
                 uint64_t side_cap, unsigned long long *side_ctr, uint32_t *mid_list,
                 uint32_t *big_list, OpCounters *ctr, unsigned int *status, const void *summ,
                 uint32_t summ_min, uint32_t summ_rows, uint32_t plus_i,
-                unsigned long long *parts);
+                unsigned long long *parts, uint32_t batch);
 // Both group tiers of a product in one launch: rows1 (*n1 of them) by groups
 // of lanes1 lanes (LDS window lds1 words) on grid1 blocks, rows2 by 64-lane
 // groups (lds2) on grid2 blocks; grids multiples of 8.
@@ -225,6 +227,6 @@ void slots_counts(hipStream_t s, SlotView a, uint64_t n, uint32_t *cnt,
                   const unsigned int *status);
 void slots_emit(hipStream_t s, SlotView a, uint64_t n, uint32_t sb, const void *rowmap,
                 uint32_t *entries);
-uint32_t slot_lane_blocks_per_cu(bool summaries);
+uint32_t slot_lane_blocks_per_cu(bool summaries, uint32_t batch);
 
 }  // namespace dv
