@@ -13,7 +13,7 @@ Fixtures (all small, JSON or npz without pickles):
                         loop, two components) with D for several [d1, d2],
                         computed by dense walk-length layering and checked
                         equal to the oracle before writing
-  cfg1_2000bp.npz       A of the preset-1 generator at 2000 bp and its D for
+  preset1_2000bp.npz       A of the preset-1 generator at 2000 bp and its D for
                         [15, 45], computed by sparse walk-length layering
                         (scipy) and checked equal to the oracle
 Each expected output is produced by a method independent of the oracle (the
@@ -149,7 +149,7 @@ def main():
     exp = sparse_walk_index(a, 15, 45)
     o = orc.dindex(a, 15, 45)
     assert same(o, exp)
-    np.savez(os.path.join(HERE, "cfg1_2000bp.npz"), a_rowmap=a.rowmap, a_entries=a.entries,
+    np.savez(os.path.join(HERE, "preset1_2000bp.npz"), a_rowmap=a.rowmap, a_entries=a.entries,
              a_ncols=a.ncols, a_nnz=a.nnz, d_rowmap=np.asarray(exp.rowmap, np.uint64),
              d_entries=exp.entries, d_nnz=exp.nnz, dlo=15, dup=45)
     print("golden fixtures written")

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def cfg1():
+def graph_1e5():
     return dv.generate_graph(1)
 
 
@@ -68,13 +68,13 @@ TIERS = [
 
 
 @pytest.mark.parametrize("tier", TIERS, ids=[str(t) for t in TIERS])
-def test_spgemm_tiers_chunks_and_stage(ctx, cfg1, tier):
+def test_spgemm_tiers_chunks_and_stage(ctx, graph_1e5, tier):
     """Every tier, tiny LDS windows (chunked path) and tiny stages (direct
     recompute) give the same product."""
     lds, stage, lanes, mid_max, lane_max = tier
     ctx.set_tuning(lds_words=lds, stage_entries=stage, lanes_small=lanes,
                    small_slice_max=mid_max, lane_slice_max=lane_max)
-    ai = orc.spadd(orc.identity(cfg1.nrows), cfg1)
+    ai = orc.spadd(orc.identity(graph_1e5.nrows), graph_1e5)
     ai = dv.RangeCRS(ai.ncols, ai.entries, ai.rowmap, ai.nnz)
     print(f"spgemm tiers {tier}", flush=True)
     for k in (1, 4, 37):
@@ -88,8 +88,8 @@ def test_spgemm_tiers_chunks_and_stage(ctx, cfg1, tier):
 
 
 @pytest.mark.parametrize("size_bytes", [4, 8])
-def test_spadd_identity_plus_a(ctx, cfg1, size_bytes):
-    a = cfg1.as_size(size_bytes)
+def test_spadd_identity_plus_a(ctx, graph_1e5, size_bytes):
+    a = graph_1e5.as_size(size_bytes)
     i = ctx.identity(a.nrows, size_bytes)
     assert_identical(i, orc.identity(a.nrows), "identity")
     c = ctx.range_spadd(i, a)
@@ -115,10 +115,10 @@ def test_spadd_random_and_noncanonical(ctx):
 
 @pytest.mark.parametrize("k", [0, 1, 2, 5, 16, 150])
 @pytest.mark.parametrize("plan_mode", [0, 1, 2])
-def test_power(ctx, cfg1, k, plan_mode):
+def test_power(ctx, graph_1e5, k, plan_mode):
     ctx.set_tuning(plan_mode=plan_mode)
     print(f"power k={k} plan_mode={plan_mode}", flush=True)
-    assert_identical(ctx.range_power(cfg1, k), orc.power(cfg1, k), f"A^{k}")
+    assert_identical(ctx.range_power(graph_1e5, k), orc.power(graph_1e5, k), f"A^{k}")
 
 
 def test_power_noncanonical_input(ctx):
@@ -130,14 +130,14 @@ def test_power_noncanonical_input(ctx):
         assert_identical(ctx.range_power(a, k), orc.power(a, k), f"A^{k} non-canonical")
 
 
-def test_distance_index_autotuned(ctx, cfg1):
+def test_distance_index_autotuned(ctx, graph_1e5):
     """The autotuner tries many launch shapes; every one must give the same
     index, and the cached winner is reused for the same shape."""
-    want = orc.dindex(cfg1, 150, 450)
+    want = orc.dindex(graph_1e5, 150, 450)
     ctx.set_tuning(autotune=1)
-    print("dindex autotune cfg1 [150,450]", flush=True)
+    print("dindex autotune 1e5-bp graph [150,450]", flush=True)
     for _ in range(2):
-        assert_identical(ctx.create_distance_index(cfg1, 150, 450), want, "autotuned D")
+        assert_identical(ctx.create_distance_index(graph_1e5, 150, 450), want, "autotuned D")
     t = ctx.last_tuning()
     assert t["autotune"] == 1
     assert t["lanes_small"] in (8, 16, 32) and t["stage_entries"] in (4, 8, 16)
@@ -203,13 +203,13 @@ def test_distance_index_schedules(ctx, schedule, dlo, dup, groups_only, layout):
 @pytest.mark.parametrize("size_bytes", [4, 8])
 @pytest.mark.parametrize("dlo,dup", [(150, 450), (0, 250), (7, 7), (1, 8), (5, 5), (0, 1)])
 @pytest.mark.parametrize("plan_mode", [0, 1, 2])
-def test_distance_index_cfg1(ctx, cfg1, size_bytes, dlo, dup, plan_mode, layout):
+def test_distance_index_graph_1e5(ctx, graph_1e5, size_bytes, dlo, dup, plan_mode, layout):
     """Planned builds (one readback), exact builds (a readback per op) and
     planned builds forced through the overflow fallback agree with the oracle,
     with range-CRS and slot-form products."""
     ctx.set_tuning(plan_mode=plan_mode, layout=layout)
-    a = cfg1.as_size(size_bytes)
-    print(f"dindex cfg1 [{dlo},{dup}] S={size_bytes} plan_mode={plan_mode}", flush=True)
+    a = graph_1e5.as_size(size_bytes)
+    print(f"dindex 1e5-bp graph [{dlo},{dup}] S={size_bytes} plan_mode={plan_mode}", flush=True)
     d, st = ctx.create_distance_index(a, dlo, dup, stats=True)
     assert_identical(d, orc.dindex(a, dlo, dup), f"D[{dlo},{dup}]")
     assert st["n_spadd"] <= 1 and st["n_spgemm"] > 0
@@ -241,24 +241,24 @@ def test_distance_index_slot_side_overflow(ctx):
     assert st["retried"] == 1
 
 
-def test_distance_index_bad_bounds(ctx, cfg1):
+def test_distance_index_bad_bounds(ctx, graph_1e5):
     for dlo, dup in [(5, 4), (0, 0)]:
         with pytest.raises(dv.DiVerGError) as e:
-            ctx.create_distance_index(cfg1, dlo, dup)
+            ctx.create_distance_index(graph_1e5, dlo, dup)
         assert e.value.status == 1
 
 
-def test_device_resident_chain(ctx, cfg1):
-    da = ctx.to_device(cfg1)
+def test_device_resident_chain(ctx, graph_1e5):
+    da = ctx.to_device(graph_1e5)
     sq = ctx.range_spgemm(da, da, location="device")
     p4 = ctx.range_spgemm(sq, sq, location="device")
-    assert_identical(p4.to_host(), orc.power(cfg1, 4), "A^4 on device")
+    assert_identical(p4.to_host(), orc.power(graph_1e5, 4), "A^4 on device")
     d = ctx.create_distance_index(da, 3, 9, location="device")
-    assert_identical(d.to_host(), orc.dindex(cfg1, 3, 9), "D on device")
+    assert_identical(d.to_host(), orc.dindex(graph_1e5, 3, 9), "D on device")
 
 
-def test_query_batch(ctx, cfg1):
-    d = orc.dindex(cfg1, 150, 450)
+def test_query_batch(ctx, graph_1e5):
+    d = orc.dindex(graph_1e5, 150, 450)
     d = dv.RangeCRS(d.ncols, d.entries, d.rowmap, d.nnz)
     rng = np.random.default_rng(11)
     n = 200000

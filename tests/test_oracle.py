@@ -191,8 +191,8 @@ def test_walk_rows_matches_dindex():
                               w.entries[w.rowmap[k]:w.rowmap[k + 1]])
 
 
-def test_golden_cfg1_2000bp():
-    z = np.load(os.path.join(GOLD, "cfg1_2000bp.npz"))
+def test_golden_preset1_2000bp():
+    z = np.load(os.path.join(GOLD, "preset1_2000bp.npz"))
     a = dv.generate_graph(1, backbone_bp=2000)
     assert np.array_equal(a.rowmap, z["a_rowmap"]) and np.array_equal(a.entries, z["a_entries"])
     d = orc.dindex(a, int(z["dlo"]), int(z["dup"]))
