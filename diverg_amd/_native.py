@@ -143,6 +143,7 @@ SIGNATURES = [
     ("dv_status_string", C.c_char_p, [C.c_int]),
     ("dv_init", C.c_int, [C.c_int, P(VP)]),
     ("dv_finalize", C.c_int, [VP]),
+    ("dv_synchronize", C.c_int, [VP]),
     ("dv_last_error", C.c_char_p, [VP]),
     ("dv_stream", VP, [VP]),
     ("dv_get_tuning", C.c_int, [VP, P(dv_tuning)]),

@@ -383,6 +383,11 @@ int dv_init(int device, dv_context **out) {
   return DV_OK;
 }
 
+int dv_synchronize(dv_context *c) {
+  if (!c) return DV_ERR_INVALID;
+  return c->hip(hipStreamSynchronize(c->stream), "synchronize") ? DV_OK : DV_ERR_DEVICE;
+}
+
 int dv_finalize(dv_context *c) {
   if (!c) return DV_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);

@@ -234,6 +234,15 @@ void Ctx::plan_launch(Plan &p, bool timing) {
   fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
            (uint32_t)(nops * sizeof(OpCounters) / 4), 0u);
   fill_u32(stream, d_status, 1, 0u);
+  // A (A + I) computed from A equals the reference's A . SpAdd(I, A) only for
+  // canonical rows of A (the reference merge of an unsorted row is not the
+  // set union): a caller's A is checked first, and a non-canonical one sets
+  // bit 2 of the status, so no later kernel runs
+  for (const PlanOp &o : p.ops)
+    if (o.plus_i && p.slots[o.a].external) {
+      check_canonical(stream, p.slots[o.a].sb, view(p.slots[o.a].as_mat()), d_status);
+      break;
+    }
   for (int i = 0; i < nops; ++i) {
     const PlanOp &o = p.ops[i];
     PlanSlot &c = p.slots[o.c];
@@ -416,6 +425,14 @@ int Ctx::plan_run(Plan &p, DevMat &out, const std::function<int(DevMat &)> &exac
     release_all();
     return fail("entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
   }
+  if (st & 4) {  // a non-canonical input under the paired schedule
+    release_all();
+    reset_stats();
+    if (p.ref_key_set) ref_order_shapes.insert(p.ref_key);
+    rc = exact(out);
+    stats_retried = 1;
+    return rc;
+  }
   if (st & 1) {
     release_all();
     plan_retries += 1;
@@ -488,11 +505,17 @@ bool Ctx::plan_events_reserve(int nops) {
 }
 
 int Ctx::dindex_planned(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
-  if (tuning.layout == 2 ||
-      (tuning.layout == 0 && !rcrs_shapes.count({a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup})))
+  const std::array<uint64_t, 6> key{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup};
+  // a shape whose input had a non-canonical row takes range CRS in the
+  // reference's order (slot-form I + A and the paired schedule need
+  // canonical rows)
+  const bool ref_order = ref_order_shapes.count(key) != 0;
+  if (!ref_order && (tuning.layout == 2 || (tuning.layout == 0 && !rcrs_shapes.count(key))))
     return dindex_slots(a, dlo, dup, d);
   Plan p;
-  p.schedule = tuning.schedule;
+  p.schedule = ref_order ? 1u : tuning.schedule;
+  p.ref_key = key;
+  p.ref_key_set = true;
   p.out = p.dindex(p.external(a), dlo, dup);
   return plan_run(p, d, [&](DevMat &o) { return dindex_exact(a, dlo, dup, o); });
 }

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <functional>
 #include <map>
+#include <set>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -101,6 +102,10 @@ struct Plan {
   int power(int p, uint32_t k);
   int dindex(int a, uint32_t dlo, uint32_t dup);
   uint32_t schedule = 0;  // 0: fewer SpGEMMs; 1: the reference's order
+  // index builds: the build shape, remembered when the input turns out not
+  // to be canonical
+  std::array<uint64_t, 6> ref_key{};
+  bool ref_key_set = false;
 };
 
 // A rowmap + entries buffer pair kept by the context across builds.
@@ -150,6 +155,9 @@ struct Ctx {
   // build shapes (rows, entries, nnz, size type, d1, d2) whose slot-form
   // products had many rows of several ranges: built in range CRS from then on
   std::map<std::array<uint64_t, 6>, bool> rcrs_shapes;
+  // build shapes whose input had a non-canonical row: built in range CRS in
+  // the reference's order (I + A by the reference merge) from then on
+  std::set<std::array<uint64_t, 6>> ref_order_shapes;
   std::vector<hipEvent_t> plan_ev;
   // hipGraph of the last planned build (replayed while its fingerprint holds)
   hipGraphExec_t graph_exec = nullptr;

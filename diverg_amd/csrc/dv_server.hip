@@ -82,6 +82,10 @@ struct dv_server {
   dv_context *qctx = nullptr;  // query stream owner
   dv_context *bctx = nullptr;  // builds
   hipStream_t qstream = nullptr;
+  // serialises the callers' copies through qctx (its allocation map and
+  // counters are not thread-safe); the loop thread enqueues on qstream too,
+  // which HIP allows from any thread
+  std::mutex copy_mu;
 
   // index table
   std::mutex idx_mu;
@@ -391,10 +395,7 @@ int dv_server_add_index(dv_server *s, const dv_rcrs *index, uint32_t *id) {
   dv_rcrs dev{};
   int rc;
   {
-    // dv_copy runs on the query context; the loop thread only touches qstream
-    // through its own slots and kernels, never this context's stream
-    static std::mutex add_mu;
-    std::lock_guard<std::mutex> lk(add_mu);
+    std::lock_guard<std::mutex> lk(s->copy_mu);
     const uint64_t bytes = (uint64_t)index->size_bytes * (index->nrows + 1) + 4 * index->n_entries;
     if (s->index_bytes.load() + bytes > s->opts.hbm_budget_bytes) return DV_ERR_BUDGET;
     rc = dv_copy(s->qctx, index, DV_DEVICE, &dev);
@@ -481,8 +482,7 @@ int dv_server_get_index(dv_server *s, uint32_t id, dv_rcrs *out) {
     m = s->idxs[id].m;
   }
   m.owner = nullptr;  // a view: dv_copy makes the host copy
-  static std::mutex get_mu;
-  std::lock_guard<std::mutex> lk(get_mu);
+  std::lock_guard<std::mutex> lk(s->copy_mu);
   return dv_copy(s->qctx, &m, DV_HOST, out);
 }
 

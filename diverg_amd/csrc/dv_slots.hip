@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <vector>
 
 #include "dv_runtime.hpp"
@@ -74,7 +75,12 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   auto acquire = [&](PlanSlot &sl, uint64_t side_cap) -> bool {
     side_cap = std::min<uint64_t>(side_cap, 0xFFFFFFF0ull);  // 32-bit offsets in the slots
     sl.cap = side_cap;
-    sl.buf = buf_acquire(8 * std::max<uint64_t>(1, sl.nrows), 2 * (side_cap + SLOT_SIDE_PAD));
+    const uint64_t rm = 8 * std::max<uint64_t>(1, sl.nrows), ent = 2 * (side_cap + SLOT_SIDE_PAD);
+    sl.buf = buf_acquire(rm, ent);
+    if (sl.buf < 0) {  // free the cached buffers no slot holds, then try once more
+      buf_trim();
+      sl.buf = buf_acquire(rm, ent);
+    }
     if (sl.buf < 0) return false;
     sl.rowmap = bufs[sl.buf].rowmap;
     sl.entries = bufs[sl.buf].entries;
@@ -216,13 +222,43 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     release(d);
     return fail("entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
   }
+  const std::array<uint64_t, 6> key{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup};
+  if (st & 12) {
+    // the input does not suit the slot form: a row that is not canonical (4;
+    // the reference's I + A merge of such a row is not the set union) or
+    // rows of more ranges than a side partition holds (8).  Built in range
+    // CRS (in the reference's order for 4), now and for later builds of this
+    // shape.
+    release_all();
+    release(d);
+    if (st & 4) ref_order_shapes.insert(key);
+    rcrs_shapes[key] = true;
+    if (tuning.layout == 2 && !(st & 4)) {  // slot form forced: the exact path
+      reset_stats();
+      stats_retried = 1;
+      return exact(d);
+    }
+    const int rc = dindex_planned(a, dlo, dup, d);
+    stats_retried = 1;
+    return rc;
+  }
   if (st & 1) {
     release_all();
     release(d);
     plan_retries += 1;
     const int rc = exact(d);
     stats_retried = 1;
-    if (!rc && tuning.plan_mode != 2) slot_side_factor = std::max(2.0 * f, 1.5 * part_use);
+    if (!rc && tuning.plan_mode != 2) {
+      // a product ran out of side entries: predict more, up to
+      // SIDE_FACTOR_MAX entries per row; a shape that needs more is built in
+      // range CRS from now on
+      constexpr double SIDE_FACTOR_MAX = 8.0;
+      const double want = std::max(2.0 * f, 1.5 * part_use);
+      if (want > SIDE_FACTOR_MAX)
+        rcrs_shapes[key] = true;
+      else
+        slot_side_factor = want;
+    }
     return rc;
   }
   // ---- the index's entries

@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 
 #include "rcrs_kernels.hpp"
@@ -1502,6 +1503,27 @@ __global__ __launch_bounds__(BLOCK) void k_validate(Mat<S> M, uint64_t nrows, ui
   }
 }
 
+// Sets bit 2 of *status (4) when a row's ranges are not canonical: each range
+// must end at least two columns before the next one starts
+// (e_k + 1 < s_(k+1)).  Rows of one range are canonical.
+template <typename S>
+__global__ __launch_bounds__(BLOCK) void k_check_canonical(Mat<S> M, uint64_t nrows,
+                                                           unsigned int *__restrict__ status) {
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  bool canonical = true;
+  for (uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; row < nrows; row += stride) {
+    const uint64_t b = M.rm[row], e = M.rm[row + 1];
+    if (e - b <= 2) continue;
+    uint32_t prev_end = M.en[b + 1];
+    for (uint64_t k = b + 2; k < e; k += 2) {
+      const uint2 v = ld_range(M.en, k);
+      canonical = canonical && (uint64_t)prev_end + 1 < (uint64_t)v.x;
+      prev_end = v.y;
+    }
+  }
+  if (!canonical) atomicOr(status, 4u);
+}
+
 // ---------------------------------------------------------------------------
 // Block-diagonal assembly of independent components and the split back.
 // Component rows [row0, row0 + n) of the assembly hold the component's rows
@@ -2154,6 +2176,10 @@ __global__ __launch_bounds__(BLOCK) void k_slot_summary(Slots B, uint64_t nrows,
 
 // rCRS -> slots (a row of several ranges into the side array).  Four rows
 // per lane: their bounds, then their first ranges, each loaded at once.
+// Status bits: 8 when a wave's rows need more side entries than a partition
+// holds (the input does not fit the slot form), 4 when a row's ranges are
+// not canonical (each range must end at least two columns before the next
+// starts): slot-form I + A and the paired schedule assume canonical rows.
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_rcrs_to_slots(Mat<S> A, uint64_t nrows,
                                                          uint2 *__restrict__ slot,
@@ -2179,10 +2205,11 @@ __global__ __launch_bounds__(BLOCK) void k_rcrs_to_slots(Mat<S> A, uint64_t nrow
     const uint32_t part = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) & (SLOT_SIDE_PARTS - 1);
     const unsigned long long loc = wave_alloc(need, &sa.ctr[part]);
     if (need64 > sa.pcap || (need && loc + need > sa.pcap)) {
-      atomicOr(status, 1u);
+      atomicOr(status, 8u);
       continue;
     }
     unsigned long long off = part * sa.pcap + loc;
+    bool canonical = true;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const uint64_t row = r0 + k;
@@ -2193,11 +2220,18 @@ __global__ __launch_bounds__(BLOCK) void k_rcrs_to_slots(Mat<S> A, uint64_t nrow
       } else {
         side[off] = make_uint2((uint32_t)nr, 0u);
         side[off + 1] = x[k];
-        for (uint64_t q = 1; q < nr; ++q) side[off + 1 + q] = ld_range(A.en, rb[k] + 2 * q);
+        uint32_t prev_end = x[k].y;
+        for (uint64_t q = 1; q < nr; ++q) {
+          const uint2 v = ld_range(A.en, rb[k] + 2 * q);
+          canonical = canonical && (uint64_t)prev_end + 1 < (uint64_t)v.x;
+          prev_end = v.y;
+          side[off + 1 + q] = v;
+        }
         slot[row] = make_uint2(SLOT_MARK, (uint32_t)off);
         off += nr + 1;
       }
     }
+    if (!canonical) atomicOr(status, 4u);
   }
 }
 
@@ -2480,10 +2514,12 @@ void spgemm_lane(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t stage, u
 }
 
 uint32_t lane_blocks_per_cu(uint32_t sb, bool summ) {
-  // resident blocks of the lane kernel per CU (occupancy), per variant
-  static int cache[4] = {0, 0, 0, 0};
+  // resident blocks of the lane kernel per CU (occupancy), per variant; the
+  // cache is shared by every context and thread (a racing fill stores the
+  // same value)
+  static std::atomic<int> cache[4] = {0, 0, 0, 0};
   const int k = (sb == 4 ? 0 : 2) + (summ ? 1 : 0);
-  if (!cache[k]) {
+  if (!cache[k].load(std::memory_order_relaxed)) {
     int nb = 0;
     hipError_t e;
     if (sb == 4)
@@ -2624,6 +2660,16 @@ void validate_matrix(hipStream_t s, uint32_t sb, DView m, uint64_t n_entries, ui
   else
     hipLaunchKernelGGL(k_validate<uint64_t>, dim3(grid_for(work)), dim3(BLOCK), 0, s,
                        mat<uint64_t>(m), m.nrows, m.ncols, n_entries, flag);
+}
+
+void check_canonical(hipStream_t s, uint32_t sb, DView m, unsigned int *status) {
+  if (!m.nrows) return;
+  if (sb == 4)
+    hipLaunchKernelGGL(k_check_canonical<uint32_t>, dim3(grid_for(m.nrows)), dim3(BLOCK), 0, s,
+                       mat<uint32_t>(m), m.nrows, status);
+  else
+    hipLaunchKernelGGL(k_check_canonical<uint64_t>, dim3(grid_for(m.nrows)), dim3(BLOCK), 0, s,
+                       mat<uint64_t>(m), m.nrows, status);
 }
 
 void slice_lengths(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *len) {
@@ -2770,9 +2816,9 @@ void slots_emit(hipStream_t s, SlotView a, uint64_t n, uint32_t sb, const void *
 }
 
 uint32_t slot_lane_blocks_per_cu(bool summ, uint32_t batch) {
-  static int cache[4] = {0, 0, 0, 0};
+  static std::atomic<int> cache[4] = {0, 0, 0, 0};  // shared by every context and thread
   const int k = summ ? 3 : batch <= 2 ? 0 : batch <= 4 ? 1 : 2;
-  if (!cache[k]) {
+  if (!cache[k].load(std::memory_order_relaxed)) {
     int nb = 0;
     const hipError_t e =
         k == 3   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<true>, BLOCK, 0)

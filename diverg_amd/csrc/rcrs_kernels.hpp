@@ -154,6 +154,9 @@ void query_multi(hipStream_t s, const IndexDesc *table, uint32_t ntable, const v
 // Sets *flag when rowmap[0] != 0, rowmap[n] != n_entries, a row length is
 // odd or negative, or a range is not s <= e < ncols.
 void validate_matrix(hipStream_t s, uint32_t sb, DView m, uint64_t n_entries, uint32_t *flag);
+// Sets bit 2 (4) of *status when a row of m is not canonical (its ranges are
+// not ascending with a gap of at least one column between neighbours).
+void check_canonical(hipStream_t s, uint32_t sb, DView m, unsigned int *status);
 
 // Number of B ranges each row of A gathers (sum over the row's ranges of the
 // B slice length) -- the work measure behind the size classes.

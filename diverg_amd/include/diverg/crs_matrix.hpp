@@ -54,8 +54,11 @@ namespace diverg {
 
   template< typename TOrdinal, typename TSize >
   class CRSMatrix< crs_matrix::RangeDynamic, bool, TOrdinal, TSize > {
-    static_assert( std::is_same< TOrdinal, uint32_t >::value,
-                   "the engine stores uint32_t column ranges" );
+    // the engine stores 32-bit column ranges; int32_t ordinals (the
+    // reference's tests use them) hold the same non-negative values
+    static_assert( std::is_same< TOrdinal, uint32_t >::value
+                       || std::is_same< TOrdinal, int32_t >::value,
+                   "the engine stores 32-bit column ranges" );
     static_assert( std::is_same< TSize, uint32_t >::value || std::is_same< TSize, uint64_t >::value,
                    "the size type is uint32_t or uint64_t" );
   public:

@@ -1,19 +1,22 @@
 /**
  *  @file  diverg/range_sparse.hpp
  *  @brief Range SpAdd / SpGEMM / power on the MI355X engine, call-compatible
- *         with DiVerG's range_sparse.hpp (RangeSpAdd::spadd :430-477,
- *         RangeSpGEMM::spgemm :2283-2332, RangePower::power :2334-2368,
- *         create_range_identity_matrix :91-110).
+ *         with DiVerG's range_sparse.hpp: the operator classes RangeSpAdd
+ *         (:430-468), RangeSpGEMM (:2283-2324) and RangePower (:2334-2360),
+ *         the free functions range_spadd (:470-477), range_spgemm
+ *         (:2326-2332), range_power (:2362-2368) that forward to them, and
+ *         create_range_identity_matrix (:91-110).
  *
- *  The reference's SparseConfig / execution-space arguments select Kokkos
- *  schedules; here they are accepted and ignored (launch shapes are the
- *  library's tuning, dv_set_tuning).  Operands are copied to HBM, the result
- *  is copied back into a host matrix, as the reference's matrix-level calls
- *  do with Kokkos views.
+ *  The SparseConfig / execution-space arguments are accepted and carried
+ *  (range_sparse_base.hpp): launch shapes are the library's tuning
+ *  (dv_set_tuning).  Operands are copied to HBM and the result is copied back
+ *  into a host matrix, as the reference's matrix-level calls do with Kokkos
+ *  views.
  */
 #ifndef DIVERG_AMD_RANGE_SPARSE_HPP__
 #define DIVERG_AMD_RANGE_SPARSE_HPP__
 
+#include <cassert>
 #include <cstdint>
 #include <cstdlib>
 #include <limits>
@@ -22,12 +25,10 @@
 #include <vector>
 
 #include "crs_matrix.hpp"
+#include "range_sparse_base.hpp"
 #include "runtime.hpp"
 
 namespace diverg {
-  /** Accepted for call-site compatibility with range_sparse_base.hpp:480-498. */
-  struct DefaultSparseConfiguration { };
-
   namespace amd {
     template< typename TMatrix >
     inline dv_rcrs
@@ -39,7 +40,9 @@ namespace diverg {
       d.nrows = rm.empty() ? 0 : rm.size() - 1;
       d.ncols = m.numCols();
       d.rowmap = const_cast< typename TMatrix::size_type* >( rm.data() );
-      d.entries = const_cast< uint32_t* >( en.data() );
+      // int32_t ordinals hold non-negative columns: the same bits as uint32_t
+      d.entries = reinterpret_cast< uint32_t* >(
+          const_cast< typename TMatrix::ordinal_type* >( en.data() ) );
       d.n_entries = en.size();
       d.nnz = m.nnz();
       d.size_bytes = sizeof( typename TMatrix::size_type );
@@ -65,9 +68,13 @@ namespace diverg {
     }
   }  // namespace amd
 
-  template< typename TRCRSMatrix >
+  /**
+   *  The identity of order n in range CRS (range_sparse.hpp:91-110): row i
+   *  holds the one range [i, i].
+   */
+  template< typename TRCRSMatrix, typename TExecSpace = amd::DeviceSpace >
   inline TRCRSMatrix
-  create_range_identity_matrix( typename TRCRSMatrix::ordinal_type n )
+  create_range_identity_matrix( typename TRCRSMatrix::ordinal_type n, TExecSpace = {} )
   {
     dv_rcrs out{};
     amd::check( dv_range_identity( amd::context(), n, sizeof( typename TRCRSMatrix::size_type ),
@@ -76,20 +83,47 @@ namespace diverg {
     return amd::adopt< TRCRSMatrix >( out );
   }
 
-  template< typename TRCRSMatrix, typename TExecSpace = DefaultSparseConfiguration >
+  /* === range_spadd: c = a + b (range_sparse.hpp:430-477) === */
+  template< typename TRCRSMatrix, typename TExecSpace >
+  class RangeSpAdd {
+  public:
+    static TRCRSMatrix spadd( TRCRSMatrix const& a, TRCRSMatrix const& b, TExecSpace space );
+  };
+
+  template< typename TRCRSMatrix, typename TExecSpace >
   inline TRCRSMatrix
-  range_spadd( TRCRSMatrix const& a, TRCRSMatrix const& b, TExecSpace = {} )
+  RangeSpAdd< TRCRSMatrix, TExecSpace >::spadd( TRCRSMatrix const& a, TRCRSMatrix const& b,
+                                                TExecSpace )
   {
+    assert( a.numCols() == b.numCols() );
+    assert( a.numRows() == b.numRows() );
     dv_rcrs da = amd::describe( a ), db = amd::describe( b ), out{};
     amd::check( dv_range_spadd( amd::context(), &da, &db, DV_HOST, &out ), "range_spadd" );
     return amd::adopt< TRCRSMatrix >( out );
   }
 
-  template< typename TRCRSMatrix, typename TSparseConfig = DefaultSparseConfiguration >
+  template< typename TRCRSMatrix, typename TExecSpace = amd::DeviceSpace >
   inline TRCRSMatrix
-  range_spgemm( TRCRSMatrix const& a, TRCRSMatrix const& b, TSparseConfig = {} )
+  range_spadd( TRCRSMatrix const& a, TRCRSMatrix const& b, TExecSpace space = {} )
+  {
+    return RangeSpAdd< TRCRSMatrix, TExecSpace >::spadd( a, b, space );
+  }
+
+  /* === range_spgemm: c = a . b (range_sparse.hpp:2283-2332) === */
+  template< typename TRCRSMatrix, typename TSparseConfig >
+  class RangeSpGEMM {
+  public:
+    static TRCRSMatrix spgemm( TRCRSMatrix const& a, TRCRSMatrix const& b,
+                               TSparseConfig config );
+  };
+
+  template< typename TRCRSMatrix, typename TSparseConfig >
+  inline TRCRSMatrix
+  RangeSpGEMM< TRCRSMatrix, TSparseConfig >::spgemm( TRCRSMatrix const& a, TRCRSMatrix const& b,
+                                                     TSparseConfig )
   {
     typedef typename TRCRSMatrix::ordinal_type ordinal_type;
+    assert( a.numCols() == b.numRows() );
     // range_sparse.hpp:2299-2300
     DIVERG_AMD_ASSERT( a.numCols() <= std::numeric_limits< ordinal_type >::max() - 1 );
     DIVERG_AMD_ASSERT( b.numCols() <= std::numeric_limits< ordinal_type >::max() - 1 );
@@ -100,11 +134,33 @@ namespace diverg {
 
   template< typename TRCRSMatrix, typename TSparseConfig = DefaultSparseConfiguration >
   inline TRCRSMatrix
-  range_power( TRCRSMatrix const& a, unsigned int k, TSparseConfig = {} )
+  range_spgemm( TRCRSMatrix const& a, TRCRSMatrix const& b, TSparseConfig config = {} )
+  {
+    return RangeSpGEMM< TRCRSMatrix, TSparseConfig >::spgemm( a, b, config );
+  }
+
+  /* === range_power: c = a^k (range_sparse.hpp:2334-2368) === */
+  template< typename TRCRSMatrix, typename TSparseConfig >
+  class RangePower {
+  public:
+    static TRCRSMatrix power( TRCRSMatrix const& a, unsigned int k, TSparseConfig config );
+  };
+
+  template< typename TRCRSMatrix, typename TSparseConfig >
+  inline TRCRSMatrix
+  RangePower< TRCRSMatrix, TSparseConfig >::power( TRCRSMatrix const& a, unsigned int k,
+                                                   TSparseConfig )
   {
     dv_rcrs da = amd::describe( a ), out{};
     amd::check( dv_range_power( amd::context(), &da, k, DV_HOST, &out ), "range_power" );
     return amd::adopt< TRCRSMatrix >( out );
+  }
+
+  template< typename TRCRSMatrix, typename TSparseConfig = DefaultSparseConfiguration >
+  inline TRCRSMatrix
+  range_power( TRCRSMatrix const& a, unsigned int k, TSparseConfig config = {} )
+  {
+    return RangePower< TRCRSMatrix, TSparseConfig >::power( a, k, config );
   }
 }  // namespace diverg
 

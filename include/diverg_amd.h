@@ -95,6 +95,9 @@ typedef struct dv_build_stats {
  * (runtime.hpp:25-55, src/eti/runtime.cpp:22-49). */
 int dv_init(int device, dv_context **ctx);
 int dv_finalize(dv_context *ctx);
+/* Wait for every operation the context has enqueued (the reference's
+ * execution_space{}.fence(), e.g. dindex.hpp:181-187). */
+int dv_synchronize(dv_context *ctx);
 const char *dv_last_error(const dv_context *ctx);
 const char *dv_status_string(int status);
 int dv_abi_version(void);

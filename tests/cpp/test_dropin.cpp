@@ -1,12 +1,16 @@
 // Drop-in check of the C++ wrapper headers: the reference's call sites
-// (diverg::util::create_distance_index, diverg::range_spgemm / range_spadd /
-// range_power, CRSMatrix<RangeDynamic,...>::operator(), serialize/load,
+// (diverg::util::create_distance_index with and without its config and
+// timers, diverg::range_spgemm / range_spadd / range_power and their
+// RangeSpGEMM / RangeSpAdd / RangePower classes, SparseConfig with every
+// accumulator/partition pair, CRSMatrix<RangeDynamic,...>::operator(), serialize/load,
 // util::range_adjacency_matrix, upper_node_rank_in_component,
 // util::merge_distance_index) compiled
 // unchanged against diverg_amd/include and compared with the CPU oracle, in the
 // manner of check_identical (test_dindex.cpp:34-53).  Needs a GPU.
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <cstdlib>
 #include <sstream>
 #include <string>
@@ -143,6 +147,118 @@ static void scenario(const char *name) {
   orc_free(&e);
 }
 
+// A seeded random Boolean matrix of order n with about nnz set cells (rows
+// sorted and duplicate-free, so the ranges are canonical), as the
+// reference's create_random_binary_matrix gives test_range_sparse.cpp:512-545.
+template <typename TMatrix>
+static TMatrix random_matrix(uint32_t n, uint64_t nnz, uint64_t seed) {
+  typedef typename TMatrix::ordinal_type O;
+  typedef typename TMatrix::size_type S;
+  std::vector<std::vector<uint32_t>> rows(n);
+  uint64_t x = seed;
+  auto next = [&]() {  // splitmix64
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  for (uint64_t k = 0; k < nnz; ++k) rows[next() % n].push_back((uint32_t)(next() % n));
+  typename TMatrix::entries_type en;
+  typename TMatrix::rowmap_type rm(1, 0);
+  for (auto &r : rows) {
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+    for (std::size_t k = 0; k < r.size();) {
+      std::size_t e = k;
+      while (e + 1 < r.size() && r[e + 1] == r[e] + 1) ++e;
+      en.push_back((O)r[k]);
+      en.push_back((O)r[e]);
+      k = e + 1;
+    }
+    rm.push_back((S)en.size());
+  }
+  return TMatrix((O)n, std::move(en), std::move(rm));
+}
+
+// test_range_sparse.cpp:512-545: SparseConfig< grid::Suggested, TAccumulator,
+// TPartition > for every accumulator/partition pair the reference tests,
+// range_spgemm( m, m, config_type{} ) on an int32_t-ordinal matrix, and the
+// config's execution space fenced.
+template <typename TAccumulator, typename TPartition>
+static void config_scenario(const char *name) {
+  using namespace diverg;
+  std::printf("scenario: range_spgemm with SparseConfig< grid::Suggested, %s >\n", name);
+  typedef SparseConfig<grid::Suggested, TAccumulator, TPartition> config_type;
+  typedef typename config_type::execution_space execution_space;
+  typedef CRSMatrix<crs_matrix::RangeDynamic, bool, int32_t, uint64_t> rcrsmatrix_t;
+  rcrsmatrix_t rrand_mat = random_matrix<rcrsmatrix_t>(6521, 200000, 0xD1E5);
+  auto rc = range_spgemm(rrand_mat, rrand_mat, config_type{});
+  execution_space space;
+  space.fence();
+  // oracle on the same matrix
+  std::vector<uint64_t> rm(rrand_mat.get_rowmap().begin(), rrand_mat.get_rowmap().end());
+  std::vector<uint32_t> en(rrand_mat.get_entries().begin(), rrand_mat.get_entries().end());
+  orc_rcrs o{(uint64_t)rrand_mat.numRows(), (uint64_t)rrand_mat.numCols(), rm.data(), en.data(),
+             rrand_mat.nnz()};
+  orc_rcrs e;
+  REQUIRE(orc_spgemm(&o, &o, &e) == 0);
+  REQUIRE((uint64_t)rc.numRows() == e.nrows && (uint64_t)rc.numCols() == e.ncols);
+  REQUIRE(rc.nnz() == e.nnz);
+  bool same = rc.get_entries().size() == e.rowmap[e.nrows];
+  for (std::size_t i = 0; same && i < rc.get_entries().size(); ++i)
+    same = (uint32_t)rc.get_entries()[i] == e.entries[i];
+  for (std::size_t i = 0; same && i <= e.nrows; ++i) same = rc.get_rowmap()[i] == e.rowmap[i];
+  REQUIRE(same);
+  orc_free(&e);
+}
+
+// rcrs_benchmark.cpp:449-450 and :685-720: a RunTime grid whose sizes the
+// caller sets, create_distance_index( ra, dlo, dup, config, &timer,
+// inner_timer.get() ), and the ETI operator classes RangeSpGEMM::spgemm,
+// RangeSpAdd::spadd, RangePower::power (range_sparse.hpp:430-468,
+// 2283-2368) with create_range_identity_matrix( n, space ).
+template <typename TSize>
+static void benchmark_scenario() {
+  using namespace diverg;
+  std::printf("scenario: benchmark call shapes, size type %zu bytes\n", sizeof(TSize));
+  typedef rcrs_t<TSize> range_crsmatrix_t;
+  typedef SparseConfig<grid::RunTime, HBitVectorAccumulator<>, TeamSequentialPartition> config_type;
+  typedef typename config_type::execution_space execution_space;
+  config_type config;
+  config.grid.m_team_work_size = 16;
+  config.grid.m_team_size = 4;
+  config.grid.m_vector_size = 8;
+  REQUIRE(config.grid.team_size() == 4 && config.grid.vector_size() == 8);
+  auto ra = graph<TSize>(1, 20000);
+  OracleView<TSize> ov(ra);
+  amd::Timer timer;
+  std::shared_ptr<amd::Timer> inner_timer = std::make_shared<amd::Timer>();
+  const unsigned dlo = 15, dup = 45;
+  auto rc = util::create_distance_index(ra, dlo, dup, config, &timer, inner_timer.get());
+  REQUIRE(timer.seconds() > 0.0);
+  orc_rcrs e;
+  REQUIRE(orc_dindex(&ov.o, dlo, dup, &e) == 0);
+  check_identical(rc, e);
+  orc_free(&e);
+  // the ETI operator classes
+  execution_space space;
+  auto i = create_range_identity_matrix<range_crsmatrix_t>(ra.numRows(), space);
+  OracleView<TSize> oi(i);
+  auto ai = RangeSpAdd<range_crsmatrix_t, execution_space>::spadd(i, ra, space);
+  REQUIRE(orc_spadd(&oi.o, &ov.o, &e) == 0);
+  check_identical(ai, e);
+  orc_free(&e);
+  auto sq = RangeSpGEMM<range_crsmatrix_t, config_type>::spgemm(ra, ra, config);
+  REQUIRE(orc_spgemm(&ov.o, &ov.o, &e) == 0);
+  check_identical(sq, e);
+  orc_free(&e);
+  auto p = RangePower<range_crsmatrix_t, config_type>::power(ra, 13, config);
+  REQUIRE(orc_power(&ov.o, 13, &e) == 0);
+  check_identical(p, e);
+  orc_free(&e);
+}
+
 // test_dindex.cpp:55-240 on the reference's multi.gfa: the adjacency of the
 // whole graph and of each component (walked with upper_node_rank_in_component)
 // and distance indices of it, against the oracle.
@@ -193,6 +309,18 @@ int main(int argc, char **argv) {
   diverg::ScopedRuntime runtime;
   scenario<uint64_t>("uint64_t");
   scenario<uint32_t>("uint32_t");
+  config_scenario<diverg::BTreeAccumulator, diverg::ThreadRangePolicyPartition>(
+      "BTreeAccumulator, ThreadRangePolicyPartition");
+  config_scenario<diverg::HBitVectorAccumulator<>, diverg::ThreadSequentialPartition>(
+      "HBitVectorAccumulator<>, ThreadSequentialPartition");
+  config_scenario<diverg::HBitVectorAccumulator<>, diverg::TeamSequentialPartition>(
+      "HBitVectorAccumulator<>, TeamSequentialPartition");
+  config_scenario<diverg::HBitVectorAccumulator<>, diverg::ThreadRangeParallelPartition>(
+      "HBitVectorAccumulator<>, ThreadRangeParallelPartition");
+  config_scenario<diverg::HBitVectorAccumulator<>, diverg::ThreadParallelPartition>(
+      "HBitVectorAccumulator<>, ThreadParallelPartition");
+  benchmark_scenario<uint64_t>();
+  benchmark_scenario<uint32_t>();
   const std::string gfa = argc > 1 ? argv[1] : "tests/golden/gfa/multi.gfa";
   graph_scenario<uint64_t>(gfa, 1, 8);   // test_dindex.cpp:180-206
   graph_scenario<uint32_t>(gfa, 1, 3);   // test_dindex.cpp:208-240

@@ -47,6 +47,56 @@ def test_struct_sizes_match_header():
     assert ctypes.sizeof(N.dv_build_stats) == 8 * 5 + 4 * 2 + 8 * 2 + 4 * 4
     assert ctypes.sizeof(N.dv_tuning) == 4 * 12
     assert ctypes.sizeof(N.dv_kernel_time) == 8 * 4
+    assert ctypes.sizeof(N.dv_server_opts) == 8 + 4 * 4
+    assert ctypes.sizeof(N.dv_request_info) == 4 * 4 + 8 * 4
+
+
+STRUCTS = ["dv_rcrs", "dv_build_stats", "dv_graph_info", "dv_tuning", "dv_kernel_time",
+           "dv_graph_params", "dv_server_opts", "dv_request_info"]
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes struct has the size, field names and field offsets the C
+    compiler gives the header's struct (a C probe prints them)."""
+    lines = ["#include <stddef.h>", "#include <stdio.h>", "#include <diverg_amd.h>",
+             "#include <diverg_amd_gen.h>", "#include <diverg_amd_graph.h>", "int main(void) {"]
+    for name in STRUCTS:
+        cls = getattr(N, name)
+        lines.append(f'  printf("{name}:size %zu\\n", sizeof({name}));')
+        for f in cls._fields_:
+            lines.append(f'  printf("{name}.{f[0]} %zu %zu\\n", offsetof({name}, {f[0]}), '
+                         f'sizeof((({name} *)0)->{f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    res = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                          "-o", str(exe), str(src)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                               check=True).stdout.splitlines():
+        key, *vals = line.split()
+        got[key] = tuple(int(v) for v in vals)
+    for name in STRUCTS:
+        cls = getattr(N, name)
+        assert got[f"{name}:size"] == (ctypes.sizeof(cls),), name
+        for f in cls._fields_:
+            fd = getattr(cls, f[0])
+            assert got[f"{name}.{f[0]}"] == (fd.offset, fd.size), f"{name}.{f[0]}"
+
+
+def test_server_rejects_cu_masked_query_stream():
+    """A CU-masked query stream is refused before any HIP call (so this runs
+    without a GPU): the mask the earlier code built placed every chosen CU on
+    one XCD."""
+    o = N.dv_server_opts(0, 0, 0, 16, 0)
+    h = ctypes.c_void_p()
+    rc = N.lib().dv_server_start(0, ctypes.byref(o), ctypes.byref(h))
+    assert rc == 1 and not h.value   # DV_ERR_INVALID, no server
+    with pytest.raises(dv.DiVerGError) as e:
+        dv.Server(query_cus=4)
+    assert e.value.status == 1
 
 
 def _canonical(m):
@@ -136,7 +186,28 @@ def test_cpp_wrapper_headers_compile(tmp_path):
         "using M32 = diverg::CRSMatrix<diverg::crs_matrix::RangeDynamic, bool, uint32_t, uint32_t>;\n"
         "M f(M const& a) { return diverg::util::create_distance_index(a, 150, 450); }\n"
         "M32 g(M32 const& a) { return diverg::range_power(diverg::range_spadd(a, a), 3); }\n"
-        "M h(M const& a) { return diverg::range_spgemm(a, a, diverg::DefaultSparseConfiguration{}); }\n")
+        "M h(M const& a) { return diverg::range_spgemm(a, a, diverg::DefaultSparseConfiguration{}); }\n"
+        # rcrs_benchmark.cpp:449-450, 685-720
+        "using RT = diverg::SparseConfig<diverg::grid::RunTime, diverg::HBitVectorAccumulator<>,\n"
+        "                                diverg::TeamSequentialPartition>;\n"
+        "M k(M const& a, diverg::amd::Timer* t1, diverg::amd::Timer* t2) {\n"
+        "  RT config; config.grid.m_team_size = 4; config.grid.m_vector_size = 8;\n"
+        "  return diverg::util::create_distance_index(a, 150, 450, config, t1, t2); }\n"
+        # test_range_sparse.cpp:522-545 (int32_t ordinals, every tag pair)
+        "using MI = diverg::CRSMatrix<diverg::crs_matrix::RangeDynamic, bool, int32_t, uint64_t>;\n"
+        "template <typename A, typename P> MI r(MI const& m) {\n"
+        "  typedef diverg::SparseConfig<diverg::grid::Suggested, A, P> config_type;\n"
+        "  typedef typename config_type::execution_space execution_space;\n"
+        "  auto rc = diverg::range_spgemm(m, m, config_type{});\n"
+        "  execution_space space; space.fence(); return rc; }\n"
+        "template MI r<diverg::BTreeAccumulator, diverg::ThreadRangePolicyPartition>(MI const&);\n"
+        "template MI r<diverg::HBitVectorAccumulator<>, diverg::ThreadParallelPartition>(MI const&);\n"
+        # range_sparse.hpp:430-468, 2283-2368 (the ETI operator classes)
+        "M32 s(M32 const& a) { using S = diverg::amd::DeviceSpace;\n"
+        "  auto i = diverg::create_range_identity_matrix<M32>(a.numRows(), S{});\n"
+        "  auto c = diverg::RangeSpAdd<M32, S>::spadd(i, a, S{});\n"
+        "  c = diverg::RangeSpGEMM<M32, diverg::DefaultSparseConfiguration>::spgemm(c, a, {});\n"
+        "  return diverg::RangePower<M32, diverg::DefaultSparseConfiguration>::power(c, 5, {}); }\n")
     res = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
                           "-I" + os.path.join(ROOT, "include"),
                           "-I" + os.path.join(ROOT, "diverg_amd", "include"), str(src)],

@@ -8,13 +8,7 @@ import diverg_amd as dv
 from oracle import oracle as orc
 from tests.rcrs_util import assert_identical
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="dv_server_start ended in a segmentation fault (main thread) in the "
-                     "first GPU run of this test, which created its query stream with "
-                     "hipExtStreamCreateWithCUMask; no cause in this library's code is shown, "
-                     "so that path is switched off and these tests stay off the GPU"),
-]
+pytestmark = pytest.mark.gpu
 
 
 def as_rcrs(o):
