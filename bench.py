@@ -80,12 +80,50 @@ def measured_traffic(args, kernel):
     return t.get("traffic_bytes_per_launch")
 
 
-def cpu_threads():
+def host_cores():
+    """The host's cores as the CPU baseline counts them: logical CPUs in this
+    process's affinity mask (nproc), the physical cores they belong to
+    (sysfs topology), lscpu's summary, and the CPU share the machine grants
+    the job (OMP_NUM_THREADS when the environment sets it)."""
+    import subprocess
     try:
-        n = len(os.sched_getaffinity(0))
+        cpus = sorted(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        cpus = list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology"
+        try:
+            with open(base + "/physical_package_id") as f:
+                pkg = f.read().strip()
+            with open(base + "/core_id") as f:
+                core = f.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("cpu", c))
+    lscpu = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("CPU(s)", "Thread(s) per core", "Core(s) per socket", "Socket(s)",
+                             "Model name"):
+                lscpu[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    return {"nproc": len(cpus), "physical_cores_in_mask": len(cores), "lscpu": lscpu,
+            "cpu_share": int(share) if share.isdigit() and int(share) > 0 else None}
+
+
+def cpu_threads(cores=None):
+    """Threads of the CPU baseline: one per physical core in the affinity
+    mask, within the CPU share granted to the job."""
+    c = cores or host_cores()
+    n = c["physical_cores_in_mask"]
+    if c["cpu_share"]:
+        n = min(n, c["cpu_share"])
+    return max(1, n)
 
 
 def main():
@@ -122,6 +160,11 @@ def main():
               "(components shard across GPUs with no exchange; see the design notes). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    # the CPU baseline's OpenMP threads: one per core, packed (read by the
+    # OpenMP runtime when the oracle library loads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
 
     import diverg_amd as dv
 
@@ -193,28 +236,63 @@ def main():
     result_host = d.to_host()
     d.free()
 
+    # A^d1 on its own (the reference's intermediate rA^dlo, dindex.hpp:151-157)
+    pw = ctx.range_power(da, args.dlo, location="device")
+    power_host = pw.to_host()
+    pw.free()
+
     cpu = None
     bit_exact = None
+    power_exact = None
     if not args.no_cpu:
         from oracle import oracle as orc
-        th = cpu_threads()
+        cores = host_cores()
+        th = cpu_threads(cores)
         orc.set_threads(th)
-        t = time.perf_counter()
-        e = orc.dindex(a, args.dlo, args.dup)
-        cpu_s = time.perf_counter() - t
+        # one warm-up build, then the median of three, all on `th` threads
+        runs = []
+        e = None
+        for r in range(4):
+            t = time.perf_counter()
+            e = orc.dindex(a, args.dlo, args.dup)
+            if r:
+                runs.append(time.perf_counter() - t)
+            print(f"# cpu oracle build {r + 1}/4 ({'warm-up' if r == 0 else 'timed'}) on "
+                  f"{th} threads", file=sys.stderr, flush=True)
+        cpu_s = float(np.median(runs))
         bit_exact = bool(result_host.nnz == e.nnz % (1 << 32)
                          and np.array_equal(result_host.rowmap.astype(np.uint64), e.rowmap)
                          and np.array_equal(result_host.entries, e.entries))
+        del e
+        ep = orc.power(a, args.dlo)
+        power_exact = bool(power_host.nnz == ep.nnz % (1 << 32)
+                           and np.array_equal(power_host.rowmap.astype(np.uint64), ep.rowmap)
+                           and np.array_equal(power_host.entries, ep.entries))
+        del ep
+        # one thread: a single timed build
+        orc.set_threads(1)
+        t = time.perf_counter()
+        orc.dindex(a, args.dlo, args.dup)
+        one_s = time.perf_counter() - t
+        orc.set_threads(th)
         cpu = {
             "value": round(a.nrows / cpu_s, 1),
             "unit": "loci/s",
             "cores": th,
             "kind": "port",
-            "sample": (f"the full workload: one build of the same {a.nrows}-locus index "
-                       f"[{args.dlo},{args.dup}], {cpu_s:.2f} s"),
+            "sample": (f"the full workload: builds of the same {a.nrows}-locus index "
+                       f"[{args.dlo},{args.dup}]; 1 warm-up, median of 3 on {th} threads "
+                       f"(OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
+                       f"OMP_PLACES={os.environ.get('OMP_PLACES')}), and one 1-thread build"),
             "build_s": round(cpu_s, 3),
+            "runs_s": [round(x, 3) for x in runs],
+            "threads": th,
+            "one_thread_s": round(one_s, 3),
+            "one_thread_loci_per_s": round(a.nrows / one_s, 1),
+            "host": cores,
         }
-        print(f"# cpu oracle: {cpu_s:.2f}s on {th} threads, bit_exact={bit_exact}",
+        print(f"# cpu oracle: median {cpu_s:.2f}s on {th} threads, 1 thread {one_s:.2f}s, "
+              f"bit_exact={bit_exact}, A^{args.dlo} bit_exact={power_exact}",
               file=sys.stderr, flush=True)
 
     out = {
@@ -246,6 +324,8 @@ def main():
         "spgemms_per_build": stats[-1]["n_spgemm"],
         "peak_device_bytes": stats[-1]["peak_device_bytes"],
         "bit_exact": bit_exact,
+        "a_d1_bit_exact": power_exact,
+        "a_d1_ranges": len(power_host.entries) // 2,
         "roofline": roofline,
         "cpu_baseline": cpu,
         # per build, over the profiled builds
@@ -283,6 +363,16 @@ def leg_chromosome(ctx):
     dh = d.to_host()
     d.free()
     del da
+    # the whole index against the oracle, and sampled rows against the
+    # oracle's independent walk-length BFS
+    orc.set_threads(cpu_threads())
+    log("chromosome: oracle build of the whole index")
+    t = time.perf_counter()
+    e = orc.dindex(a, 200, 1000)
+    oracle_s = time.perf_counter() - t
+    full_ok = bool(dh.nnz == e.nnz and np.array_equal(dh.rowmap, e.rowmap)
+                   and np.array_equal(dh.entries, e.entries))
+    del e
     rng = np.random.default_rng(0xD1E5)
     rows = np.sort(rng.choice(a.nrows, 1500, replace=False)).astype(np.uint32)
     w = orc.walk_rows(a, rows, 200, 1000)
@@ -299,10 +389,13 @@ def leg_chromosome(ctx):
         "peak_hbm_bytes": st["peak_device_bytes"],
         "index_ranges": len(dh.entries) // 2,
         "index_nnz": dh.nnz,
+        "bit_exact_vs_oracle": full_ok,
+        "oracle_s": round(oracle_s, 2),
         "rows_checked_vs_walk_bfs": len(rows),
         "rows_bit_exact": bool(ok),
     }
-    log(f"chromosome build {res['build_s']}s, rows bit-exact {ok}")
+    log(f"chromosome build {res['build_s']}s, index bit-exact {full_ok} (oracle "
+        f"{oracle_s:.1f}s), BFS rows bit-exact {ok}")
     return res, (a, vpos, dh)
 
 
@@ -478,7 +571,6 @@ def leg_batched(ctx):
     total = sum(c.nrows for c in comps)
     log(f"batched: 24 components, {total} loci ({time.time() - t:.1f}s to generate)")
     dcomps = [ctx.to_device(c) for c in comps]
-    del comps
     ctx.set_tuning(graphs=1)
     outs = ctx.create_distance_index_batched(dcomps, 300, 600, location="device")  # warm
     for o in outs:
@@ -488,6 +580,21 @@ def leg_batched(ctx):
     outs, st = ctx.create_distance_index_batched(dcomps, 300, 600, location="device", stats=True)
     hip_sync()
     batched_s = time.perf_counter() - t
+    # per size class, each class with its autotuned launch shape (the first
+    # call tunes, untimed)
+    ctx.set_tuning(autotune=1)
+    t = time.perf_counter()
+    for o in ctx.create_distance_index_batched(dcomps, 300, 600, location="device"):
+        o.free()
+    hip_sync()
+    tune_s = time.perf_counter() - t
+    log(f"batched: per-class autotuning took {tune_s:.1f}s")
+    t = time.perf_counter()
+    touts, tst = ctx.create_distance_index_batched(dcomps, 300, 600, location="device",
+                                                   stats=True)
+    hip_sync()
+    tuned_s = time.perf_counter() - t
+    ctx.set_tuning(autotune=0)
     ctx.set_tuning(graphs=0)
     hip_sync()
     t = time.perf_counter()
@@ -496,25 +603,41 @@ def leg_batched(ctx):
         plain.append(ctx.create_distance_index(dc, 300, 600, location="device"))
     hip_sync()
     plain_s = time.perf_counter() - t
-    exact = True
-    for o, p in zip(outs, plain):
-        exact = exact and dv.identical(o.to_host(), p.to_host())
+    from oracle import oracle as orc
+    orc.set_threads(cpu_threads())
+    exact = oracle_exact = True
+    t = time.perf_counter()
+    for c, (comp, o, to, p) in enumerate(zip(comps, outs, touts, plain)):
+        got = o.to_host()
+        exact = exact and dv.identical(got, p.to_host()) and dv.identical(got, to.to_host())
+        oracle_exact = oracle_exact and dv.identical(got, orc.dindex(comp, 300, 600))
         o.free()
+        to.free()
         p.free()
+        if c % 6 == 5:
+            log(f"batched: {c + 1}/24 components checked against the oracle")
+    oracle_s = time.perf_counter() - t
     res = {
         "workload": "24 components, log-normal lengths (median 5e6 bp, sigma 1, clipped "
                     "1e5-1.2e8), SNP 0.01, indel 0.001, uint64 rowmap, [300,600]",
         "components": 24,
         "loci": total,
         "batched_s": round(batched_s, 4),
+        "size_class_tuned_s": round(tuned_s, 4),
+        "size_class_builds": tst["builds"],
+        "size_class_tuning_s": round(tune_s, 2),
         "sequential_plain_s": round(plain_s, 4),
         "speedup": round(plain_s / batched_s, 3),
         "batched_loci_per_s": round(total / batched_s, 1),
         "spgemms": st["n_spgemm"],
         "peak_hbm_bytes": st["peak_device_bytes"],
         "bit_exact_per_component": bool(exact),
+        "bit_exact_vs_oracle_per_component": bool(oracle_exact),
+        "oracle_s": round(oracle_s, 2),
     }
-    log(f"batched {batched_s:.3f}s vs sequential {plain_s:.3f}s, exact {exact}")
+    log(f"batched {batched_s:.3f}s (per size class, tuned: {tuned_s:.3f}s in "
+        f"{tst['builds']} builds) vs sequential {plain_s:.3f}s, exact {exact}, "
+        f"oracle exact {oracle_exact}")
     return res
 
 

@@ -45,7 +45,7 @@ class dv_build_stats(C.Structure):
         ("planned", C.c_uint32),
         ("retried", C.c_uint32),
         ("layout", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("builds", C.c_uint32),
     ]
 
 
@@ -74,6 +74,7 @@ class dv_tuning(C.Structure):
         ("summary_span", C.c_uint32),
         ("schedule", C.c_uint32),
         ("layout", C.c_uint32),
+        ("chain_span", C.c_uint32),
     ]
 
 

@@ -407,6 +407,7 @@ int dv_finalize(dv_context *c) {
   if (c->h_plan_ctr) (void)hipHostFree(c->h_plan_ctr);
   if (c->d_plan_ctr) (void)hipFree(c->d_plan_ctr);
   if (c->slot_parts) (void)hipFree(c->slot_parts);
+  if (c->slot_defer) (void)hipFree(c->slot_defer);
   if (c->h_flag) (void)hipHostFree(c->h_flag);
   if (c->d_ctr) (void)hipFree(c->d_ctr);
   if (c->d_flag) (void)hipFree(c->d_flag);
@@ -630,6 +631,7 @@ int dv_create_distance_index(dv_context *ctx, const dv_rcrs *a, uint32_t dlo, ui
   st.planned = c->stats_planned;
   st.retried = c->stats_retried;
   st.layout = c->stats_layout;
+  st.builds = 1;
   // everything the context holds at the high-water mark of the build: the
   // resident input, cached plan buffers, scratch and the output
   st.peak_device_bytes = c->peak_bytes + input_bytes;
@@ -678,6 +680,7 @@ int dv_create_distance_index_batched(dv_context *ctx, const dv_rcrs *comps, uint
   st.planned = c->stats_planned;
   st.retried = c->stats_retried;
   st.layout = c->stats_layout;
+  st.builds = c->stats_classes;
   st.peak_device_bytes = c->peak_bytes + input_bytes;
   for (uint32_t i = 0; i < ncomp; ++i) {
     if ((rc = export_mat(c, douts[i], loc, &outs[i], &st.d2h_ms))) {

@@ -11,6 +11,10 @@
 // component's index is the corresponding diagonal block shifted by its
 // offset: bit-identical to a plain build of that component (the block
 // assembly the reference itself checks in test_dindex.cpp:111-173).
+// The launch shape of a build is the reference's SparseConfig / ExecGrid
+// choice (range_sparse_base.hpp:99-498); with dv_tuning.autotune the
+// components are grouped in size classes and each class is one build with
+// its own autotuned shape.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,12 +24,89 @@
 
 namespace dv {
 
+// Size class of a component for the per-class tuned batched build: by rows,
+// below 2^20, below 2^24, and larger.
+static int size_class(uint64_t nrows) { return nrows < (1ull << 20) ? 0 : nrows < (1ull << 24) ? 1 : 2; }
+
 int Ctx::dindex_batched(const std::vector<DevMat> &comps, uint32_t dlo, uint32_t dup,
                         std::vector<DevMat> &outs) {
+  if (!(dlo <= dup && dup != 0)) return fail("create_distance_index: need dlo <= dup and dup != 0");
+  // A component with a non-canonical row is built on its own: the reference
+  // clips such rows to an accumulator window placed by row and column index
+  // (range_sparse.hpp:1192-1242), which the assembly would shift.
+  std::vector<size_t> odd, even;
+  for (size_t c = 0; c < comps.size(); ++c) {
+    const uint8_t k = comps[c].nrows ? canonical(comps[c]) : 1;
+    if (k == 0) return DV_ERR_DEVICE;
+    (k == 2 ? odd : even).push_back(c);
+  }
+  if (!odd.empty()) {
+    outs.assign(comps.size(), DevMat{});
+    std::vector<DevMat> sub, sub_out;
+    for (size_t c : even) sub.push_back(comps[c]);
+    int rc = dindex_batched_classes(sub, dlo, dup, sub_out);
+    const uint32_t n_spgemm = stats_n_spgemm, n_spadd = stats_n_spadd,
+                   builds = stats_classes;
+    const uint64_t alg = stats_alg_bytes;
+    if (!rc)
+      for (size_t k = 0; k < even.size(); ++k) outs[even[k]] = sub_out[k];
+    for (size_t k = 0; !rc && k < odd.size(); ++k) rc = dindex_dev(comps[odd[k]], dlo, dup, outs[odd[k]]);
+    if (rc) {
+      for (DevMat &o : outs) release(o);
+      return rc;
+    }
+    stats_n_spgemm += n_spgemm;
+    stats_n_spadd += n_spadd;
+    stats_alg_bytes += alg;
+    stats_classes = builds + (uint32_t)odd.size();
+    stats_retried = 1;
+    return DV_OK;
+  }
+  return dindex_batched_classes(comps, dlo, dup, outs);
+}
+
+int Ctx::dindex_batched_classes(const std::vector<DevMat> &comps, uint32_t dlo, uint32_t dup,
+                                std::vector<DevMat> &outs) {
+  // With the autotuner on, the components are built per size class: one
+  // block-diagonal build per class, each with the launch shape the
+  // autotuner picks for that class's build shape (and caches).
+  if (!tuning.autotune || tuning.plan_mode != 0 || comps.size() < 2)
+    return dindex_batched_one(comps, dlo, dup, outs);
+  std::vector<std::vector<size_t>> cls(3);
+  for (size_t c = 0; c < comps.size(); ++c) cls[size_class(comps[c].nrows)].push_back(c);
+  outs.assign(comps.size(), DevMat{});
+  uint32_t n_spgemm = 0, n_spadd = 0, retried = 0, classes = 0;
+  uint64_t alg = 0;
+  for (const auto &members : cls) {
+    if (members.empty()) continue;
+    std::vector<DevMat> sub, sub_out;
+    for (size_t c : members) sub.push_back(comps[c]);
+    const int rc = dindex_batched_one(sub, dlo, dup, sub_out);
+    if (rc) {
+      for (DevMat &o : outs) release(o);
+      return rc;
+    }
+    for (size_t k = 0; k < members.size(); ++k) outs[members[k]] = sub_out[k];
+    n_spgemm += stats_n_spgemm;
+    n_spadd += stats_n_spadd;
+    alg += stats_alg_bytes;
+    retried |= stats_retried;
+    classes += 1;
+  }
+  stats_n_spgemm = n_spgemm;
+  stats_n_spadd = n_spadd;
+  stats_alg_bytes = alg;
+  stats_retried = retried;
+  stats_classes = classes;
+  return DV_OK;
+}
+
+int Ctx::dindex_batched_one(const std::vector<DevMat> &comps, uint32_t dlo, uint32_t dup,
+                            std::vector<DevMat> &outs) {
   const size_t m = comps.size();
   outs.assign(m, DevMat{});
+  stats_classes = 1;
   if (m == 0) return DV_OK;
-  if (!(dlo <= dup && dup != 0)) return fail("create_distance_index: need dlo <= dup and dup != 0");
   std::vector<uint64_t> row0(m + 1, 0), ent0(m + 1, 0);
   for (size_t c = 0; c < m; ++c) {
     if (comps[c].nrows != comps[c].ncols) return fail("batched: a component is not square");

@@ -24,14 +24,16 @@ int Ctx::identity_dev(uint64_t n, uint32_t sb, DevMat &out) {
     return fail("device allocation failed (identity)", DV_ERR_NOMEM);
   }
   identity(stream, sb, n, out.rowmap, out.entries);
+  out.canon = 1;
   return hip(hipGetLastError(), "identity launch") ? DV_OK : DV_ERR_DEVICE;
 }
 
 bool Ctx::op_scratch(uint64_t n, OpScratch &s) {
   const uint64_t n1 = n ? n : 1;
   const uint32_t KS = tuning.stage_entries;
-  // the lane-tier grid is at most 8 blocks of 256 threads per CU
-  const uint32_t parts_cap = 8 * std::max<uint32_t>(1, num_cu);
+  // the lane-tier grid is at most 8 blocks of 256 threads per CU (16 for
+  // the two launches of the chain form)
+  const uint32_t parts_cap = 16 * std::max<uint32_t>(1, num_cu);
   void *buf[8];
   const uint64_t sz[8] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * scan_words(n),
                           8 * summary_words(n1), 16ull * parts_cap};
@@ -194,17 +196,85 @@ int Ctx::spadd_dev(const DevMat &a, const DevMat &b, DevMat &c) {
     return DV_ERR_DEVICE;
   }
   c.nnz = h.nnz;
+  // canonical inputs give a canonical sum; otherwise it is checked when used
+  c.canon = a.canon == 1 && b.canon == 1 ? 1 : 0;
   prof_add(3, 0, 1, mat_bytes(a) + mat_bytes(b) + mat_bytes(c), 0);
   stats_alg_bytes += mat_bytes(a) + mat_bytes(b) + mat_bytes(c);
   stats_n_spadd += 1;
   return DV_OK;
 }
 
+uint8_t Ctx::canonical(const DevMat &m) {
+  if (m.canon) return m.canon;
+  if (m.nrows == 0) return 1;
+  fill_u32(stream, d_status, 1, 0u);
+  check_canonical(stream, m.sb, view(m), d_status);
+  if (!hip(hipMemcpyAsync(h_status, d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
+           "canonical readback") ||
+      !hip(hipStreamSynchronize(stream), "canonical synchronize"))
+    return 0;
+  return (*h_status & 4) ? 2 : 1;
+}
+
+// C = A.B.  The reference's accumulator gives the set union of the B rows
+// named by each A row when B's rows are canonical; for a B with a row that is
+// not, its window clips ranges (spgemm_window).
 int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   if (a.ncols != b.nrows) return fail("spgemm: a.ncols != b.nrows");
   if (a.sb != b.sb) return fail("spgemm: operands differ in size type");
   if (a.ncols > 0xFFFFFFFEull || b.ncols > 0xFFFFFFFEull)
     return fail("spgemm: ncols exceeds uint32_t max - 1", DV_ERR_OVERFLOW);
+  const uint8_t cb = canonical(b);
+  if (cb == 0) return DV_ERR_DEVICE;
+  return cb == 2 ? spgemm_window(a, b, c) : spgemm_core(a, b, c);
+}
+
+int Ctx::spgemm_window(const DevMat &a, const DevMat &b, DevMat &c) {
+  const uint64_t n = a.nrows;
+  OpScratch s;
+  DevMat p;
+  p.nrows = n;
+  p.ncols = b.ncols;
+  p.sb = a.sb;
+  p.owned = true;
+  p.canon = 2;
+  const bool got = op_scratch(n, s);
+  p.rowmap = dalloc(p.sb * (n + 1));
+  if (!got || !p.rowmap) {
+    release(p);
+    return fail("device allocation failed (spgemm window)", DV_ERR_NOMEM);
+  }
+  zero_counters(stream, d_ctr);
+  fill_u32(stream, d_status, 1, 0u);
+  window_pieces(stream, a.sb, view(a), view(b), s.cnt, nullptr, nullptr, d_status);
+  scan_rowmap(stream, p.sb, s.cnt, n, p.rowmap, s.tmp, d_ctr, ~0ull, d_status, nullptr, 0,
+              nullptr);
+  OpCounters h{};
+  uint32_t st = 0;
+  if (!hip(hipGetLastError(), "window pieces launch") || !read_counters(h, &st)) {
+    release(p);
+    return DV_ERR_DEVICE;
+  }
+  if (st & 2) {
+    release(p);
+    return fail("spgemm: entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
+  }
+  p.n_entries = h.total;
+  p.entries = static_cast<uint32_t *>(dalloc(4 * std::max<uint64_t>(1, p.n_entries)));
+  if (!p.entries) {
+    release(p);
+    return fail("device allocation failed (spgemm window)", DV_ERR_NOMEM);
+  }
+  window_pieces(stream, a.sb, view(a), view(b), nullptr, p.rowmap, p.entries, d_status);
+  DevMat ident;
+  int rc = identity_dev(n, a.sb, ident);
+  if (!rc) rc = spgemm_core(ident, p, c);
+  release(ident);
+  release(p);
+  return rc;
+}
+
+int Ctx::spgemm_core(const DevMat &a, const DevMat &b, DevMat &c) {
   const uint64_t n = a.nrows;
   c = DevMat{};
   c.nrows = n;
@@ -261,6 +331,7 @@ int Ctx::spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c) {
   stats_mid_rows += h.n_mid;
   stats_big_rows += h.n_big;
   if (n) stats_max_ratio = std::max(stats_max_ratio, (double)c.n_entries / (2.0 * (double)n));
+  c.canon = 1;
   return DV_OK;
 }
 

@@ -38,6 +38,7 @@ int Plan::external(const DevMat &m) {
   slots[s].n_entries = m.n_entries;
   slots[s].nnz = m.nnz;
   slots[s].external = true;
+  slots[s].canon = m.canon;
   return s;
 }
 
@@ -234,15 +235,19 @@ void Ctx::plan_launch(Plan &p, bool timing) {
   fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
            (uint32_t)(nops * sizeof(OpCounters) / 4), 0u);
   fill_u32(stream, d_status, 1, 0u);
-  // A (A + I) computed from A equals the reference's A . SpAdd(I, A) only for
-  // canonical rows of A (the reference merge of an unsorted row is not the
-  // set union): a caller's A is checked first, and a non-canonical one sets
-  // bit 2 of the status, so no later kernel runs
-  for (const PlanOp &o : p.ops)
-    if (o.plus_i && p.slots[o.a].external) {
-      check_canonical(stream, p.slots[o.a].sb, view(p.slots[o.a].as_mat()), d_status);
-      break;
-    }
+  // The plan's products are set unions, which is the reference's product
+  // only for canonical right operands (its accumulator window clips the
+  // ranges of other rows), and A (A + I) from A equals A . SpAdd(I, A) only
+  // for a canonical A.  A caller's matrix not known to be canonical is
+  // checked first; a non-canonical one sets bit 2 of the status, so no later
+  // kernel runs, and the build is redone in the exact form.
+  for (int x = 0; x < (int)p.slots.size(); ++x) {
+    const PlanSlot &sl = p.slots[x];
+    if (!sl.external || sl.canon == 1) continue;
+    bool read = false;
+    for (const PlanOp &o : p.ops) read = read || (o.kind == P_SPGEMM && (o.b == x || o.plus_i));
+    if (read) check_canonical(stream, sl.sb, view(sl.as_mat()), d_status);
+  }
   for (int i = 0; i < nops; ++i) {
     const PlanOp &o = p.ops[i];
     PlanSlot &c = p.slots[o.c];
@@ -425,7 +430,7 @@ int Ctx::plan_run(Plan &p, DevMat &out, const std::function<int(DevMat &)> &exac
     release_all();
     return fail("entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
   }
-  if (st & 4) {  // a non-canonical input under the paired schedule
+  if (st & 4) {  // a non-canonical input: the exact form
     release_all();
     reset_stats();
     if (p.ref_key_set) ref_order_shapes.insert(p.ref_key);
@@ -506,14 +511,18 @@ bool Ctx::plan_events_reserve(int nops) {
 
 int Ctx::dindex_planned(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   const std::array<uint64_t, 6> key{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup};
-  // a shape whose input had a non-canonical row takes range CRS in the
-  // reference's order (slot-form I + A and the paired schedule need
-  // canonical rows)
-  const bool ref_order = ref_order_shapes.count(key) != 0;
-  if (!ref_order && (tuning.layout == 2 || (tuning.layout == 0 && !rcrs_shapes.count(key))))
+  // a shape whose input had a non-canonical row is built in the exact form
+  // (the reference's order, its accumulator window for non-canonical right
+  // operands)
+  if (ref_order_shapes.count(key)) {
+    reset_stats();
+    stats_retried = 1;
+    return dindex_exact(a, dlo, dup, d);
+  }
+  if (tuning.layout == 2 || (tuning.layout == 0 && !rcrs_shapes.count(key)))
     return dindex_slots(a, dlo, dup, d);
   Plan p;
-  p.schedule = ref_order ? 1u : tuning.schedule;
+  p.schedule = tuning.schedule;
   p.ref_key = key;
   p.ref_key_set = true;
   p.out = p.dindex(p.external(a), dlo, dup);
@@ -537,6 +546,7 @@ int Ctx::autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning 
     to.small_slice_max = from.small_slice_max;
     to.lane_slice_max = from.lane_slice_max;
     to.summary_span = from.summary_span;
+    to.chain_span = from.chain_span;
   };
   auto hit = tuned.find(key);
   if (hit != tuned.end()) {
@@ -570,6 +580,7 @@ int Ctx::autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning 
       {&dv_tuning::small_slice_max, {32, 64, 128, 256}},
       {&dv_tuning::lane_slice_max, {1024, 4096, 16384}},
       {&dv_tuning::summary_span, {0, 16, 32, 64}},
+      {&dv_tuning::chain_span, {0, 2, 3, 5, 9}},
       {&dv_tuning::lds_words, {lds_max / 4, lds_max / 2, lds_max}},
       {&dv_tuning::rows_per_group, {1, 4, 16}},
   };
@@ -578,7 +589,9 @@ int Ctx::autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning 
   for (size_t d = 0; !rc && d < dims.size(); ++d) {
     const uint32_t keep = best.*(dims[d].first);
     for (uint32_t v : dims[d].second) {
-      if (v == keep || (v == 0 && dims[d].first != &dv_tuning::summary_span)) continue;
+      if (v == keep || (v == 0 && dims[d].first != &dv_tuning::summary_span &&
+                        dims[d].first != &dv_tuning::chain_span))
+        continue;
       dv_tuning cand = best;
       cand.*(dims[d].first) = v;
       double ms = 0;

@@ -28,6 +28,9 @@ struct DevMat {
   void *rowmap = nullptr;
   uint32_t *entries = nullptr;
   bool owned = false;
+  // 1: every row canonical (ranges ascending, a gap of at least one column
+  // between neighbours), 2: some row is not, 0: not known
+  uint8_t canon = 0;
 };
 
 uint64_t mat_bytes(const DevMat &m);
@@ -67,6 +70,8 @@ struct PlanSlot {
   int buf = -1;             // context buffer backing the slot
   bool external = false;    // caller memory
   bool released = false;
+  bool chains = false;      // slot form: the buffer also holds the matrix's chains
+  uint8_t canon = 0;        // external: DevMat::canon of the caller's matrix
   DevMat as_mat() const {
     DevMat m;
     m.nrows = nrows;
@@ -122,7 +127,7 @@ struct Ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t num_cu = 256;
   uint32_t lds_per_block = 65536;
-  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0, 0, 32, 0, 0};
+  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0, 0, 32, 0, 0, 5};
   std::string last_error;
 
   // pool accounting
@@ -151,6 +156,9 @@ struct Ctx {
   // operation, and the predicted side-array entries per row
   unsigned long long *slot_parts = nullptr;
   uint64_t slot_parts_words = 0;
+  // rows the chain form defers to its list pass (per-block lists, counts)
+  uint32_t *slot_defer = nullptr;
+  uint64_t slot_defer_words = 0;
   double slot_side_factor = 1.0;
   // build shapes (rows, entries, nnz, size type, d1, d2) whose slot-form
   // products had many rows of several ranges: built in range CRS from then on
@@ -234,6 +242,12 @@ struct Ctx {
   int identity_dev(uint64_t n, uint32_t sb, DevMat &out);
   int spadd_dev(const DevMat &a, const DevMat &b, DevMat &c);
   int spgemm_dev(const DevMat &a, const DevMat &b, DevMat &c);
+  // set-union product (canonical rows of b, or by design)
+  int spgemm_core(const DevMat &a, const DevMat &b, DevMat &c);
+  // the reference's product for a non-canonical b (its accumulator window)
+  int spgemm_window(const DevMat &a, const DevMat &b, DevMat &c);
+  // m.canon when known, else checked on the device (1 or 2; 0 on error)
+  uint8_t canonical(const DevMat &m);
   int power_exact(const DevMat &a, uint32_t k, DevMat &out);
   int dindex_exact(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
 
@@ -254,6 +268,7 @@ struct Ctx {
   // dv_slots.hip
   int dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
   bool slot_parts_reserve(uint64_t words);
+  bool slot_defer_reserve(uint64_t words);
   int autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning &best);
 
   // dv_graph.hip
@@ -262,6 +277,11 @@ struct Ctx {
   // dv_batch.hip
   int dindex_batched(const std::vector<DevMat> &comps, uint32_t dlo, uint32_t dup,
                      std::vector<DevMat> &outs);
+  int dindex_batched_classes(const std::vector<DevMat> &comps, uint32_t dlo, uint32_t dup,
+                             std::vector<DevMat> &outs);
+  int dindex_batched_one(const std::vector<DevMat> &comps, uint32_t dlo, uint32_t dup,
+                         std::vector<DevMat> &outs);
+  uint32_t stats_classes = 1;  // builds a batched build ran (one per size class)
 };
 
 }  // namespace dv

@@ -21,6 +21,19 @@ namespace {
 SlotView view_of(const PlanSlot &s) { return SlotView{s.rowmap, s.entries}; }
 }  // namespace
 
+bool Ctx::slot_defer_reserve(uint64_t words) {
+  if (words <= slot_defer_words) return true;
+  if (slot_defer) (void)hipFree(slot_defer);
+  slot_defer = nullptr;
+  slot_defer_words = 0;
+  if (hipMalloc((void **)&slot_defer, 4 * words) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  slot_defer_words = words;
+  return true;
+}
+
 bool Ctx::slot_parts_reserve(uint64_t words) {
   if (words <= slot_parts_words) return true;
   if (slot_parts) (void)hipFree(slot_parts);
@@ -63,6 +76,10 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   if (!slot_parts_reserve(parts_words + ctr_words))
     return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
   unsigned long long *side_ctr = slot_parts + parts_words;
+  // deferred-row lists of the chain form (per block: its share of the rows)
+  const uint64_t defer_list_words = nmax + (uint64_t)per_op * 256 + 256;
+  if (!slot_defer_reserve(defer_list_words + per_op))
+    return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
 
   // buffers: slot array (8 bytes per row) in `rowmap`, side array in `entries`
   auto release_all = [&]() {
@@ -75,7 +92,11 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   auto acquire = [&](PlanSlot &sl, uint64_t side_cap) -> bool {
     side_cap = std::min<uint64_t>(side_cap, 0xFFFFFFF0ull);  // 32-bit offsets in the slots
     sl.cap = side_cap;
-    const uint64_t rm = 8 * std::max<uint64_t>(1, sl.nrows), ent = 2 * (side_cap + SLOT_SIDE_PAD);
+    // a product's slots are followed by its chains
+    const uint64_t rm = sl.kind == P_SPGEMM ? slot_chain_bytes(std::max<uint64_t>(1, sl.nrows))
+                                            : 8 * std::max<uint64_t>(1, sl.nrows);
+    const uint64_t ent = 2 * (side_cap + SLOT_SIDE_PAD);
+    sl.chains = sl.kind == P_SPGEMM;
     sl.buf = buf_acquire(rm, ent);
     if (sl.buf < 0) {  // free the cached buffers no slot holds, then try once more
       buf_trim();
@@ -126,6 +147,17 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     return fail("device allocation failed (output)", DV_ERR_NOMEM);
   }
 
+  // B operands that products read through their chains (only those get a
+  // chain pass)
+  auto reads_chains = [&](const PlanOp &o) {
+    const PlanSlot &A = p.slots[o.a], &B = p.slots[o.b];
+    return o.kind == P_SPGEMM && tuning.chain_span && A.span >= tuning.chain_span &&
+           B.kind == P_SPGEMM && (tuning.summary_span < 2 || A.span < tuning.summary_span);
+  };
+  std::vector<char> chain_reader(p.slots.size(), 0);
+  for (const PlanOp &o : p.ops)
+    if (reads_chains(o)) chain_reader[o.b] = 1;
+
   // ---- enqueue
   const bool timing = prof && plan_events_reserve(nops);
   fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
@@ -163,7 +195,10 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
       continue;
     }
     const PlanSlot &A = p.slots[o.a], &B = p.slots[o.b];
-    uint32_t summ_rows = tuning.summary_span >= 2 && A.span >= tuning.summary_span
+    // B slices through B's chains, or through interval summaries, or slot by slot
+    // (wide products, whose slices hold more chain ends, read summaries)
+    const bool chains = reads_chains(o) && B.chains;
+    uint32_t summ_rows = !chains && tuning.summary_span >= 2 && A.span >= tuning.summary_span
                              ? summary_block_rows(A.span)
                              : 0u;
     if (B.nrows > s.rows) summ_rows = 0;
@@ -172,20 +207,27 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     const uint64_t n = c.nrows;
     // slice loads per round trip: about the widest A range
     const uint32_t batch = A.span <= 2 ? 2u : A.span <= 4 ? 4u : 8u;
-    const uint64_t resident = (uint64_t)num_cu * slot_lane_blocks_per_cu(summ_rows != 0, batch);
-    const uint32_t grid =
-        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(n + 255) / 256, resident, per_op}));
+    const uint64_t resident =
+        (uint64_t)num_cu * slot_lane_blocks_per_cu(summ_rows != 0, batch, chains);
+    // the chain form's two launches write parts for twice the grid
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>({(n + 255) / 256, resident, chains ? per_op / 2 : per_op}));
     unsigned long long *op_side = side_ctr + (uint64_t)SLOT_SIDE_PARTS * i;
     slots_lane(stream, view_of(A), view_of(B), n, B.nrows, tuning.lane_slice_max,
                tuning.small_slice_max, grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big,
                ctr, d_status, summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows,
-               o.plus_i ? 1u : 0u, slot_parts + 3ull * per_op * i, batch);
-    if (!used.all) used.n[i] = n ? grid : 0u;
+               o.plus_i ? 1u : 0u, slot_parts + 3ull * per_op * i, batch,
+               chains ? slot_chains(B.rowmap, B.nrows) : ChainView{nullptr, nullptr},
+               ChainView{nullptr, nullptr}, slot_defer, slot_defer + defer_list_words);
+    if (!used.all) used.n[i] = n ? (chains ? 2 * grid : grid) : 0u;
     if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
     const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);
     slots_groups(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.grid, s.mid, &ctr->n_mid,
                  gb.lds_words, gb.grid, s.big, &ctr->n_big, gm.rows_per_group, c.rowmap, c.entries,
                  c.cap, op_side, ctr, d_status, o.plus_i ? 1u : 0u);
+    // the chains of C, for the products that read it through them
+    if (c.chains && chain_reader[o.c])
+      slots_chains(stream, c.rowmap, c.nrows, slot_chains(c.rowmap, c.nrows), d_status);
   }
   slots_reduce_parts(stream, slot_parts, per_op, used, side_ctr, (uint32_t)nops, d_plan_ctr);
   const PlanSlot &O = p.slots[p.out];
@@ -225,15 +267,15 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   const std::array<uint64_t, 6> key{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup};
   if (st & 12) {
     // the input does not suit the slot form: a row that is not canonical (4;
-    // the reference's I + A merge of such a row is not the set union) or
-    // rows of more ranges than a side partition holds (8).  Built in range
-    // CRS (in the reference's order for 4), now and for later builds of this
-    // shape.
+    // the reference's I + A merge of such a row is not the set union, and its
+    // products clip such rows to their accumulator window) or rows of more
+    // ranges than a side partition holds (8).  Built in the exact form (4)
+    // or in range CRS (8), now and for later builds of this shape.
     release_all();
     release(d);
     if (st & 4) ref_order_shapes.insert(key);
     rcrs_shapes[key] = true;
-    if (tuning.layout == 2 && !(st & 4)) {  // slot form forced: the exact path
+    if ((st & 4) || tuning.layout == 2) {  // the exact form
       reset_stats();
       stats_retried = 1;
       return exact(d);

@@ -1524,6 +1524,65 @@ __global__ __launch_bounds__(BLOCK) void k_check_canonical(Mat<S> M, uint64_t nr
   if (!canonical) atomicOr(status, 4u);
 }
 
+// The reference's accumulator window for non-canonical right operands.  Its
+// TeamSequential HBitVector<2048> product (range_sparse.hpp:1192-1242, the
+// same scatter in the numeric phase :1349-1399) keeps, per row i of C, a
+// window of 64-bit words [c_min, c_max): first the 2048-bit L1 band placed
+// around i (hbitvector.hpp:278-299), then, before each B row j it visits (A's
+// ranges in their stored order, rows in ascending order), grown down to the
+// word of B row j's first range start and up to the word after its last
+// range end.  Bits of B row j outside the window at that moment are lost (a
+// later growth clears their words; the gather reads only the window).  For
+// canonical B rows nothing is ever outside; for others the product is the
+// union of each B range clipped to its window.  This kernel writes those
+// clipped pieces (counts first, then, with WRITE, the pieces at the row
+// pointers), so that I . P (set union) gives the reference's product.
+__device__ __forceinline__ uint64_t hbv_l1_begin_bit(uint64_t centre, uint64_t ncols) {
+  constexpr uint64_t L1_BITS = 2048, L1_WORDS = L1_BITS / 64;
+  const uint64_t x_size = max((ncols + 63) & ~63ull, L1_BITS);
+  const uint64_t nb = x_size >> 6, ctr = centre >> 6;
+  if (L1_WORDS >= nb) return 0;
+  const uint64_t lflank = (L1_WORDS >> 1) - 1, rfit = nb - L1_WORDS;
+  const uint64_t pb = ctr > lflank ? ctr - lflank : 0;
+  return min(rfit, pb) << 6;
+}
+
+template <typename S, bool WRITE>
+__global__ __launch_bounds__(BLOCK) void k_window_pieces(Mat<S> A, Mat<S> B, uint64_t nrows,
+                                                         uint64_t b_ncols,
+                                                         uint32_t *__restrict__ cnt,
+                                                         const S *__restrict__ p_rm,
+                                                         uint32_t *__restrict__ p_en,
+                                                         const unsigned int *__restrict__ status) {
+  if (*status) return;
+  for (uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; row < nrows;
+       row += (uint64_t)gridDim.x * BLOCK) {
+    uint64_t c_min = hbv_l1_begin_bit(row, b_ncols), c_max = c_min + 2048;
+    uint64_t out = WRITE ? (uint64_t)p_rm[row] : 0, k = 0;
+    for (uint64_t ai = A.rm[row]; ai < (uint64_t)A.rm[row + 1]; ai += 2) {
+      const uint32_t s = A.en[ai], e = A.en[ai + 1];
+      for (uint64_t j = s; j <= e; ++j) {
+        const uint64_t bi = B.rm[j], be = B.rm[j + 1];
+        if (bi == be) continue;
+        const uint64_t b_min = B.en[bi], b_max = (uint64_t)B.en[be - 1] + 1;
+        if (b_min < c_min) c_min = b_min & ~63ull;
+        if (c_max < b_max) c_max = (b_max + 63) & ~63ull;
+        for (uint64_t q = bi; q < be; q += 2) {
+          const uint64_t x = max((uint64_t)B.en[q], c_min);
+          const uint64_t y = min((uint64_t)B.en[q + 1], c_max - 1);
+          if (x > y) continue;
+          if (WRITE) {
+            p_en[out + k] = (uint32_t)x;
+            p_en[out + k + 1] = (uint32_t)y;
+          }
+          k += 2;
+        }
+      }
+    }
+    if (!WRITE) cnt[row] = (uint32_t)min(k, (uint64_t)0xFFFFFFFFu);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Block-diagonal assembly of independent components and the split back.
 // Component rows [row0, row0 + n) of the assembly hold the component's rows
@@ -1794,10 +1853,147 @@ __device__ __forceinline__ void slot_store_runs(const Runs &r, uint32_t nruns, u
   }
 }
 
+// Chains of a slot matrix written by a product.  Rows are taken in chunks of
+// 64 (one wave of the product kernel).  Row x "chains" to row x + 1 when both
+// are single ranges in the same chunk, lo(x + 1) >= lo(x), hi(x + 1) >= hi(x)
+// and lo(x + 1) <= hi(x) + 1: then the union of rows j .. t of a chain is the
+// one interval [lo(j), hi(t)].  seg[j] holds the chunk offset (0 .. 63) of the
+// last row of the chain starting at row j (its own offset when row j chains
+// to nothing: an empty row, a row of several ranges, a row the group tiers
+// write, the chunk's last row); chunk[c] holds [lo, hi] of chunk c when its
+// 64 rows form one chain, else (1, 0).  Powers of banded graphs chain almost
+// everywhere (fewer than 1% of the rows end a chain at the 5e6-bp workload),
+// so a B slice of any width costs a product two or three loads per 64 rows.
+struct Chains {
+  const uint8_t *__restrict__ seg;
+  const uint2 *__restrict__ chunk;
+};
+constexpr uint32_t CHAIN_ROWS = 64;
+constexpr int CHUNK_BATCH = 4;
+
+// Fold B rows [j0, j1] of one chunk: v0 = slot j0, t0 = the last row of the
+// chain starting at j0, v1 = slot j1 (all loaded by the caller).  A chain
+// covering [j0, j1] folds as one interval; otherwise the rows are walked
+// chain by chain.
+__device__ __forceinline__ void fold_chain_part(const Slots &B, const uint8_t *__restrict__ seg,
+                                                uint32_t j0, uint32_t j1, uint2 v0, uint32_t t0,
+                                                uint2 v1, uint32_t &cl, uint32_t &ch, Runs &r,
+                                                bool &ok, unsigned long long &gathered) {
+  if (j1 > j0 && t0 >= j1) {
+    fold(v0.x, v1.y, cl, ch, r, ok);
+    return;
+  }
+  uint32_t j = j0, t = t0;
+  uint2 v = v0;
+  for (;;) {
+    if (t > j) {  // rows j .. b are one chain: [lo(j), hi(b)]
+      const uint32_t b = min(t, j1);
+      const uint32_t h = b == j1 ? v1.y : B.slot[b].y;
+      fold(v.x, h, cl, ch, r, ok);
+      j = b + 1;
+    } else {  // row j alone: a range, a marked row or an empty row
+      if (v.x <= v.y) fold(v.x, v.y, cl, ch, r, ok);
+      else if (v.x == SLOT_MARK) fold_side(B.side, v.y, cl, ch, r, ok, gathered);
+      j = j + 1;
+    }
+    if (!ok || j > j1) break;
+    gathered += 1;
+    v = j == j1 ? v1 : B.slot[j];
+    t = (j & ~(CHAIN_ROWS - 1)) | seg[j];
+  }
+}
+
+// Fold B rows [s, e] through the chains: the head (s to the end of its
+// chunk), whole chunks by their chunk summaries, the tail (the start of e's
+// chunk to e); the loads of the head, the tail and the first CHUNK_BATCH
+// whole chunks are issued together.
+__device__ __forceinline__ void fold_chains(const Slots &B, const Chains &C, uint32_t s,
+                                            uint32_t e, uint32_t &cl, uint32_t &ch, Runs &r,
+                                            bool &ok, unsigned long long &gathered) {
+  const uint32_t c0 = s / CHAIN_ROWS, c1 = e / CHAIN_ROWS;
+  const uint32_t e0 = c0 == c1 ? e : c0 * CHAIN_ROWS + (CHAIN_ROWS - 1);
+  const uint32_t s1 = c1 * CHAIN_ROWS;
+  const uint2 v0 = B.slot[s], v1 = B.slot[e0];
+  const uint32_t t0 = c0 * CHAIN_ROWS + C.seg[s];
+  uint2 w0 = make_uint2(1u, 0u), w1 = w0;
+  uint32_t u0 = 0;
+  uint2 m[CHUNK_BATCH];
+  if (c1 != c0) {
+    w0 = B.slot[s1];
+    u0 = s1 + C.seg[s1];
+    w1 = B.slot[e];
+#pragma unroll
+    for (int q = 0; q < CHUNK_BATCH; ++q)
+      m[q] = c0 + 1 + q < c1 ? C.chunk[c0 + 1 + q] : make_uint2(1u, 0u);
+  }
+  gathered += 3;
+  fold_chain_part(B, C.seg, s, e0, v0, t0, v1, cl, ch, r, ok, gathered);
+  if (c1 == c0) return;
+  for (uint32_t c = c0 + 1; ok && c < c1; c += CHUNK_BATCH) {
+    if (c != c0 + 1) {
+#pragma unroll
+      for (int q = 0; q < CHUNK_BATCH; ++q)
+        m[q] = c + q < c1 ? C.chunk[c + q] : make_uint2(1u, 0u);
+    }
+#pragma unroll
+    for (int q = 0; q < CHUNK_BATCH; ++q) {
+      if (!ok || c + q >= c1) continue;
+      gathered += 1;
+      if (m[q].x <= m[q].y) {
+        fold(m[q].x, m[q].y, cl, ch, r, ok);
+      } else {  // not one chain: its chains one by one
+        const uint32_t b = (c + q) * CHAIN_ROWS;
+        fold_chain_part(B, C.seg, b, b + CHAIN_ROWS - 1, B.slot[b], b + C.seg[b],
+                        B.slot[b + CHAIN_ROWS - 1], cl, ch, r, ok, gathered);
+      }
+    }
+  }
+  gathered += 3;
+  if (ok) fold_chain_part(B, C.seg, s1, e, w0, u0, w1, cl, ch, r, ok, gathered);
+}
+
+// The chains of the 64 rows of this wave (one chunk): `one` when the lane's
+// row is the single range [lo, hi].  Every lane of the wave calls it.
+__device__ __forceinline__ void chain_write(bool one, uint32_t lo, uint32_t hi, bool live,
+                                            uint64_t row, uint8_t *__restrict__ seg,
+                                            uint2 *__restrict__ chunk) {
+  const uint32_t lane = __lane_id();
+  const uint32_t nlo = (uint32_t)__shfl_down((int)lo, 1, 64);
+  const uint32_t nhi = (uint32_t)__shfl_down((int)hi, 1, 64);
+  const bool none = __shfl_down((int)one, 1, 64) != 0;
+  // columns are at most 0xFFFFFFFD, so hi + 1 does not wrap
+  const bool g = one && none && lane != 63 && nlo >= lo && nhi >= hi && nlo <= hi + 1u;
+  const unsigned long long brk = __ballot(!g);  // bit 63 is always set
+  const uint32_t t = lane + (uint32_t)__builtin_ctzll(brk >> lane);
+  const uint32_t hlast = (uint32_t)__shfl((int)hi, 63, 64);
+  if (live) seg[row] = (uint8_t)t;
+  if (lane == 0 && live) chunk[row / CHAIN_ROWS] = t == 63 ? make_uint2(lo, hlast) : make_uint2(1u, 0u);
+}
+
+// The chains of a slot matrix, from its slots (a row is single when its slot
+// is one range): one row per lane, a wave per 64-row chunk.  Run after every
+// kernel of a product has written its rows.
+__global__ __launch_bounds__(BLOCK) void k_slot_chains(const uint2 *__restrict__ slot,
+                                                       uint64_t nrows, uint8_t *__restrict__ seg,
+                                                       uint2 *__restrict__ chunk,
+                                                       const unsigned int *__restrict__ status) {
+  if (*status) return;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows;
+       base += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t row = base + threadIdx.x;
+    const bool live = row < nrows;
+    uint2 v = make_uint2(SLOT_EMPTY, 0u);
+    if (live) v = slot[row];
+    chain_write(live && v.x <= v.y, v.x, v.y, live, row, seg, chunk);
+  }
+}
+
 // Lane tier on slots: C = A.B (or A (B + I)), one lane per row of A, rows
 // whose union has more than LANE_SLOTS runs or whose slices are long go to
 // the group lists as in k_spgemm_lane.  A two-stage pipeline loads the A
-// slot of the row after next while this row folds.
+// slot of the row after next while this row folds.  B slices are read slot
+// by slot (NB at a time), through interval summaries (SUMM) or through B's
+// chains (CHAIN); with c_seg the kernel writes C's chains.
 #define DV_SLOT_LANE_PARAMS                                                                \
   Slots A, Slots B, uint64_t nrows, uint32_t lane_max, uint32_t mid_max,                   \
       uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side, SideAlloc sa,                \
@@ -1805,34 +2001,59 @@ __device__ __forceinline__ void slot_store_runs(const Runs &r, uint32_t nruns, u
       OpCounters *__restrict__ ctr, unsigned int *__restrict__ status,                     \
       const uint2 *__restrict__ pre, const uint2 *__restrict__ suf,                        \
       const uint4 *__restrict__ pre2, const uint4 *__restrict__ suf2, uint32_t summ_min,   \
-      uint32_t summ_shift, uint32_t plus_i, unsigned long long *__restrict__ parts
+      uint32_t summ_shift, uint32_t plus_i, unsigned long long *__restrict__ parts,        \
+      Chains BC, uint8_t *__restrict__ c_seg, uint2 *__restrict__ c_chunk,                 \
+      uint32_t *__restrict__ dlist, uint32_t *__restrict__ dcount, uint32_t dcap
 #define DV_SLOT_LANE_ARGS                                                                  \
   A, B, nrows, lane_max, mid_max, c_slot, c_side, sa, mid_list, big_list, ctr, status, pre, \
-      suf, pre2, suf2, summ_min, summ_shift, plus_i, parts
+      suf, pre2, suf2, summ_min, summ_shift, plus_i, parts, BC, c_seg, c_chunk, dlist,      \
+      dcount, dcap
+
+enum SlotMode { SLOT_PLAIN = 0, SLOT_SUMM = 1, SLOT_CHAIN = 2 };
 
 // NB: slice loads per round trip (LANE_BATCH, or 2 / 4 through
-// fold_slots_tight for products of narrow slices).
-template <bool SUMM, int NB>
+// fold_slots_tight for products of narrow slices) in SLOT_PLAIN mode.
+// LIST: the rows are those k_slot_fast deferred, dcount[b] of them at
+// dlist[b * dcap ..] for block b (the grid must be k_slot_fast's); the
+// parts go to parts[3 (gridDim.x + b) ..].
+template <int MODE, int NB, bool LIST = false>
 __device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
+  constexpr bool SUMM = MODE == SLOT_SUMM;
   if (*status) return;
   unsigned long long nnz_acc = 0, slice_acc = 0, ranges_acc = 0;
-  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const uint64_t stride = LIST ? BLOCK : (uint64_t)gridDim.x * BLOCK;
+  const uint64_t limit = LIST ? dcount[blockIdx.x] : nrows;
+  const uint32_t *lrows = LIST ? dlist + (uint64_t)blockIdx.x * dcap : nullptr;
+  auto row_of = [&](uint64_t i) -> uint64_t { return LIST ? (uint64_t)lrows[i] : i; };
+  const uint64_t first = LIST ? 0 : (uint64_t)blockIdx.x * BLOCK;
+  uint64_t idx = first + threadIdx.x, row = 0, c_row = LIST ? 0 : idx, n_row = LIST ? 0 : idx + stride;
   uint2 c_a = make_uint2(SLOT_EMPTY, 0u), n_a = make_uint2(SLOT_EMPTY, 0u);
-  if (row < nrows) c_a = A.slot[row];
-  if (row + stride < nrows) n_a = A.slot[row + stride];
+  if (idx < limit) {
+    c_row = row_of(idx);
+    c_a = A.slot[c_row];
+  }
+  if (idx + stride < limit) {
+    n_row = row_of(idx + stride);
+    n_a = A.slot[n_row];
+  }
   need(c_a.x);
   need(c_a.y);
   need(n_a.x);
   need(n_a.y);
-  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows; base += stride) {
-    row = base + threadIdx.x;
+  for (uint64_t base = first; base < limit; base += stride) {
+    idx = base + threadIdx.x;
+    row = c_row;
     const uint2 a = c_a;
     c_a = n_a;
+    c_row = n_row;
     n_a = make_uint2(SLOT_EMPTY, 0u);
-    if (row + 2 * stride < nrows) n_a = A.slot[row + 2 * stride];
+    n_row = LIST ? 0 : idx + 2 * stride;
+    if (idx + 2 * stride < limit) {
+      n_row = row_of(idx + 2 * stride);
+      n_a = A.slot[n_row];
+    }
     int cls = 0;
-    const bool live = row < nrows;
+    const bool live = idx < limit;
     uint64_t len = 0;
     bool ok = true;
     Runs r;
@@ -1866,7 +2087,9 @@ __device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
       if (len > lane_max) ok = false;
       if (!ok) continue;
       if (plus_i) fold(s, e, cl, ch, r, ok);  // C = A (B + I)
-      if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min &&
+      if constexpr (MODE == SLOT_CHAIN) {
+        fold_chains(B, BC, s, e, cl, ch, r, ok, slice_acc);
+      } else if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min &&
           (e >> summ_shift) - (s >> summ_shift) < 64) {
         fold_summarised(
             pre, suf, pre2, suf2, s, e, summ_shift,
@@ -1923,6 +2146,117 @@ __device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
     }
     wave_append(cls == 1, (uint32_t)row, mid_list, &ctr->n_mid);
     wave_append(cls == 2, (uint32_t)row, big_list, &ctr->n_big);
+    if (!LIST && c_seg)
+      chain_write(live && ok && nruns == 1, r.lo[0], r.hi[0], live, row, c_seg, c_chunk);
+  }
+  nnz_acc = block_sum_u64(nnz_acc);
+  slice_acc = block_sum_u64(slice_acc);
+  ranges_acc = block_sum_u64(ranges_acc);
+  if (threadIdx.x == 0) {
+    const uint64_t b = LIST ? gridDim.x + blockIdx.x : blockIdx.x;
+    parts[3 * b] = nnz_acc;
+    parts[3 * b + 1] = slice_acc;
+    parts[3 * b + 2] = ranges_acc;
+  }
+}
+
+template <bool SUMM>
+__global__ __launch_bounds__(BLOCK) void k_slot_lane(DV_SLOT_LANE_PARAMS) {
+  slot_lane_body<SUMM ? SLOT_SUMM : SLOT_PLAIN, LANE_BATCH>(DV_SLOT_LANE_ARGS);
+}
+
+// Narrow slices (2 or 4 slots a batch): six waves per SIMD measured faster
+// than five, even with the few bytes the register cap spills.
+template <int NB>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_slot_lane_narrow(
+    DV_SLOT_LANE_PARAMS) {
+  slot_lane_body<SLOT_PLAIN, NB>(DV_SLOT_LANE_ARGS);
+}
+
+
+// Phase 1 of a product over B's chains: every row of the block's share whose
+// A row is one range [s, e] covering at most FAST_CHUNKS + 1 chunks of B and
+// whose slice is one chain per part (the head s .. end of s's chunk, whole
+// chunks, the tail start of e's chunk .. e), the parts touching, is one
+// interval of C, found from a fixed set of loads issued together and a few
+// min/max operations, with no branch that differs between lanes.  The
+// block's other rows go to its list, dlist[b * dcap ..] (dcount[b] of them;
+// dcap covers the block's share), for the same grid's list pass
+// (slot_lane_body<.., LIST>).
+constexpr uint32_t FAST_CHUNKS = 4;
+
+__global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
+  __shared__ uint32_t qn;
+  if (*status) return;
+  if (threadIdx.x == 0) qn = 0;
+  __syncthreads();
+  uint32_t *lrows = dlist + (uint64_t)blockIdx.x * dcap;
+  unsigned long long nnz_acc = 0, slice_acc = 0, ranges_acc = 0;
+  const uint32_t lane = __lane_id();
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint2 c_a = make_uint2(SLOT_EMPTY, 0u), n_a = make_uint2(SLOT_EMPTY, 0u);
+  if (row < nrows) c_a = A.slot[row];
+  if (row + stride < nrows) n_a = A.slot[row + stride];
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows; base += stride) {
+    row = base + threadIdx.x;
+    const bool live = row < nrows;
+    const uint2 a = c_a;
+    c_a = n_a;
+    n_a = make_uint2(SLOT_EMPTY, 0u);
+    if (row + 2 * stride < nrows) n_a = A.slot[row + 2 * stride];
+    // lanes whose A row is not one range read row 0's data and discard it
+    const bool one_a = live && a.x <= a.y;
+    const uint32_t s = one_a ? a.x : 0u, e = one_a ? a.y : 0u;
+    const uint32_t c0 = s / CHAIN_ROWS, c1 = e / CHAIN_ROWS;
+    const bool cross = c1 != c0;
+    const uint32_t e0 = cross ? c0 * CHAIN_ROWS + (CHAIN_ROWS - 1) : e;
+    const uint32_t s1 = cross ? c1 * CHAIN_ROWS : s;
+    const uint32_t nmid = cross ? c1 - c0 - 1 : 0u;
+    const uint2 v0 = B.slot[s], v1 = B.slot[e0], w0 = B.slot[s1], w1 = B.slot[e];
+    const uint32_t t0 = c0 * CHAIN_ROWS + BC.seg[s], u0 = c1 * CHAIN_ROWS + BC.seg[s1];
+    uint2 m[FAST_CHUNKS];
+#pragma unroll
+    for (int q = 0; q < (int)FAST_CHUNKS; ++q)
+      m[q] = BC.chunk[nmid ? c0 + 1 + min((uint32_t)q, nmid - 1) : c0];
+    // a part is one chain: several rows chained to its end, or one single row
+    const bool head_ok = e0 > s ? t0 >= e0 : v0.x <= v0.y;
+    const bool tail_ok = !cross || (e > s1 ? u0 >= e : w0.x <= w0.y);
+    bool fast = one_a && nmid <= FAST_CHUNKS && head_ok && tail_ok;
+    // the union of the parts, in row order, each touching the running union
+    uint32_t cl = plus_i ? s : v0.x, ch = plus_i ? e : v1.y;
+    auto join = [&](uint32_t lo, uint32_t hi) {
+      fast = fast && lo <= ch + 1u && hi + 1u >= cl;
+      cl = min(cl, lo);
+      ch = max(ch, hi);
+    };
+    if (plus_i) join(v0.x, v1.y);
+#pragma unroll
+    for (int q = 0; q < (int)FAST_CHUNKS; ++q)
+      if ((uint32_t)q < nmid) {
+        fast = fast && m[q].x <= m[q].y;
+        join(m[q].x, m[q].y);
+      }
+    if (cross) join(w0.x, w1.y);
+    const bool empty_a = live && a.x == SLOT_EMPTY;
+    if (fast) {
+      c_slot[row] = make_uint2(cl, ch);
+      nnz_acc += (unsigned long long)(ch - cl) + 1;
+      ranges_acc += 1;
+      slice_acc += 3 + nmid + (cross ? 3 : 0);
+    } else if (empty_a) {
+      c_slot[row] = make_uint2(SLOT_EMPTY, 0u);
+    }
+    // the rest to the block's list (one LDS reservation per wave)
+    const bool defer = live && !fast && !empty_a;
+    const unsigned long long dm = __ballot(defer);
+    if (dm) {
+      const uint32_t leader = (uint32_t)__ffsll((long long)dm) - 1;
+      uint32_t qb = 0;
+      if (lane == leader) qb = atomicAdd(&qn, (uint32_t)__popcll(dm));
+      qb = (uint32_t)__shfl((int)qb, (int)leader, 64);
+      if (defer) lrows[qb + (uint32_t)__popcll(dm & ((1ull << lane) - 1))] = (uint32_t)row;
+    }
   }
   nnz_acc = block_sum_u64(nnz_acc);
   slice_acc = block_sum_u64(slice_acc);
@@ -1931,20 +2265,14 @@ __device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
     parts[3 * blockIdx.x] = nnz_acc;
     parts[3 * blockIdx.x + 1] = slice_acc;
     parts[3 * blockIdx.x + 2] = ranges_acc;
+    dcount[blockIdx.x] = qn;
   }
 }
 
-template <bool SUMM>
-__global__ __launch_bounds__(BLOCK) void k_slot_lane(DV_SLOT_LANE_PARAMS) {
-  slot_lane_body<SUMM, LANE_BATCH>(DV_SLOT_LANE_ARGS);
-}
-
-// Narrow slices (2 or 4 slots a batch): six waves per SIMD measured faster
-// than five, even with the few bytes the register cap spills.
+// Phase 2: the deferred rows, slot by slot (NB loads per round trip).
 template <int NB>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_slot_lane_narrow(
-    DV_SLOT_LANE_PARAMS) {
-  slot_lane_body<false, NB>(DV_SLOT_LANE_ARGS);
+__global__ __launch_bounds__(BLOCK) void k_slot_list(DV_SLOT_LANE_PARAMS) {
+  slot_lane_body<SLOT_PLAIN, NB, true>(DV_SLOT_LANE_ARGS);
 }
 
 // Group tiers on slots: one row of C by G lanes with an LDS bit window, as
@@ -2672,6 +3000,31 @@ void check_canonical(hipStream_t s, uint32_t sb, DView m, unsigned int *status) 
                        mat<uint64_t>(m), m.nrows, status);
 }
 
+void window_pieces(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt,
+                   const void *p_rowmap, uint32_t *p_entries, const unsigned int *status) {
+  if (!a.nrows) return;
+  const bool write = p_entries != nullptr;
+  if (sb == 4) {
+    if (write)
+      hipLaunchKernelGGL((k_window_pieces<uint32_t, true>), dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
+                         mat<uint32_t>(a), mat<uint32_t>(b), a.nrows, b.ncols, cnt,
+                         static_cast<const uint32_t *>(p_rowmap), p_entries, status);
+    else
+      hipLaunchKernelGGL((k_window_pieces<uint32_t, false>), dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
+                         mat<uint32_t>(a), mat<uint32_t>(b), a.nrows, b.ncols, cnt, nullptr, nullptr,
+                         status);
+  } else {
+    if (write)
+      hipLaunchKernelGGL((k_window_pieces<uint64_t, true>), dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
+                         mat<uint64_t>(a), mat<uint64_t>(b), a.nrows, b.ncols, cnt,
+                         static_cast<const uint64_t *>(p_rowmap), p_entries, status);
+    else
+      hipLaunchKernelGGL((k_window_pieces<uint64_t, false>), dim3(grid_for(a.nrows)), dim3(BLOCK), 0, s,
+                         mat<uint64_t>(a), mat<uint64_t>(b), a.nrows, b.ncols, cnt, nullptr, nullptr,
+                         status);
+  }
+}
+
 void slice_lengths(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *len) {
   if (!a.nrows) return;
   if (sb == 4)
@@ -2702,6 +3055,15 @@ void slots_from_rcrs(hipStream_t s, uint32_t sb, DView a, void *slot, void *side
   else
     hipLaunchKernelGGL(k_rcrs_to_slots<uint64_t>, dim3(grid_for(a.nrows, BLOCK * 4)), dim3(BLOCK), 0, s,
                        mat<uint64_t>(a), a.nrows, sl, sd, sa, status);
+}
+
+void slots_chains(hipStream_t s, const void *slot, uint64_t n, ChainView c,
+                  const unsigned int *status) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_slot_chains, dim3(grid_for(n)), dim3(BLOCK), 0, s,
+                     static_cast<const uint2 *>(slot), n,
+                     static_cast<uint8_t *>(const_cast<void *>(c.seg)),
+                     static_cast<uint2 *>(const_cast<void *>(c.chunk)), status);
 }
 
 void slots_identity(hipStream_t s, uint64_t n, void *slot) {
@@ -2746,8 +3108,14 @@ void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t 
                 uint64_t side_cap, unsigned long long *side_ctr, uint32_t *mid_list,
                 uint32_t *big_list, OpCounters *ctr, unsigned int *status, const void *summ,
                 uint32_t summ_min, uint32_t summ_rows, uint32_t plus_i,
-                unsigned long long *parts, uint32_t batch) {
+                unsigned long long *parts, uint32_t batch, ChainView b_chains, ChainView c_chains,
+                uint32_t *dlist, uint32_t *dcount) {
   if (!nrows) return;
+  const uint32_t dcap = (uint32_t)(((nrows + (uint64_t)grid * BLOCK - 1) / ((uint64_t)grid * BLOCK)) * BLOCK);
+  const Chains bc{static_cast<const uint8_t *>(b_chains.seg),
+                  static_cast<const uint2 *>(b_chains.chunk)};
+  uint8_t *c_seg = static_cast<uint8_t *>(const_cast<void *>(c_chains.seg));
+  uint2 *c_chunk = static_cast<uint2 *>(const_cast<void *>(c_chains.chunk));
   const SideAlloc sa{side_ctr, side_cap / SLOT_SIDE_PARTS};
   const uint2 *pre = static_cast<const uint2 *>(summ);
   const uint2 *suf = pre ? pre + b_nrows : nullptr;
@@ -2758,8 +3126,17 @@ void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t 
   hipLaunchKernelGGL(K_, dim3(grid), dim3(BLOCK), 0, s, slots_of(a), slots_of(b), nrows,     \
                      lane_max, mid_max, static_cast<uint2 *>(c_slot),                        \
                      static_cast<uint2 *>(c_side), sa, mid_list, big_list, ctr, status, pre, \
-                     suf, pre2, suf2, summ_min, sh, plus_i, parts)
-  if (pre) DV_SLANE(k_slot_lane<true>);
+                     suf, pre2, suf2, summ_min, sh, plus_i, parts, bc, c_seg, c_chunk, dlist,  \
+                     dcount, dcap)
+  if (bc.seg) {
+    // phase 1, then the deferred rows on the same grid (no chains written)
+    DV_SLANE(k_slot_fast);
+    c_seg = nullptr;
+    c_chunk = nullptr;
+    if (batch <= 2) DV_SLANE(k_slot_list<2>);
+    else if (batch <= 4) DV_SLANE(k_slot_list<4>);
+    else DV_SLANE(k_slot_list<8>);
+  } else if (pre) DV_SLANE(k_slot_lane<true>);
   else if (batch <= 2) DV_SLANE(k_slot_lane_narrow<2>);
   else if (batch <= 4) DV_SLANE(k_slot_lane_narrow<4>);
   else DV_SLANE(k_slot_lane<false>);
@@ -2815,13 +3192,14 @@ void slots_emit(hipStream_t s, SlotView a, uint64_t n, uint32_t sb, const void *
                        static_cast<const uint64_t *>(rowmap), entries);
 }
 
-uint32_t slot_lane_blocks_per_cu(bool summ, uint32_t batch) {
-  static std::atomic<int> cache[4] = {0, 0, 0, 0};  // shared by every context and thread
-  const int k = summ ? 3 : batch <= 2 ? 0 : batch <= 4 ? 1 : 2;
+uint32_t slot_lane_blocks_per_cu(bool summ, uint32_t batch, bool chains) {
+  static std::atomic<int> cache[5] = {0, 0, 0, 0, 0};  // shared by every context and thread
+  const int k = chains ? 4 : summ ? 3 : batch <= 2 ? 0 : batch <= 4 ? 1 : 2;
   if (!cache[k].load(std::memory_order_relaxed)) {
     int nb = 0;
     const hipError_t e =
-        k == 3   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<true>, BLOCK, 0)
+        k == 4   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_fast, BLOCK, 0)
+        : k == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<true>, BLOCK, 0)
         : k == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane_narrow<2>, BLOCK, 0)
         : k == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane_narrow<4>, BLOCK, 0)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<false>, BLOCK, 0);

@@ -157,6 +157,13 @@ void validate_matrix(hipStream_t s, uint32_t sb, DView m, uint64_t n_entries, ui
 // Sets bit 2 (4) of *status when a row of m is not canonical (its ranges are
 // not ascending with a gap of at least one column between neighbours).
 void check_canonical(hipStream_t s, uint32_t sb, DView m, unsigned int *status);
+// For C = A.B with a non-canonical B: per row of A, B's ranges clipped to the
+// reference accumulator's window as it grows (range_sparse.hpp:1192-1242,
+// hbitvector.hpp:278-299).  Without p_entries: the entry count of each row
+// into cnt; with it: the pieces at the rows of p_rowmap.  The union of a
+// row's pieces is the reference's row of C.
+void window_pieces(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt,
+                   const void *p_rowmap, uint32_t *p_entries, const unsigned int *status);
 
 // Number of B ranges each row of A gathers (sum over the row's ranges of the
 // B slice length) -- the work measure behind the size classes.
@@ -179,9 +186,28 @@ struct SlotView {
   const void *slot;
   const void *side;
 };
+// Chains of a slot matrix written by a product (seg: one byte per row, the
+// chunk offset of the last row of the chain starting there; chunk: per 64
+// rows, their union when they form one chain, else (1, 0)); null when the
+// matrix has none.
+struct ChainView {
+  const void *seg;
+  const void *chunk;
+};
+// Bytes of a product's slot buffer: slots, then chain offsets, then chunks.
+inline uint64_t slot_chain_bytes(uint64_t n) {
+  return 8 * n + ((n + 7) & ~7ull) + 8 * ((n + 63) / 64) + 64;
+}
+inline ChainView slot_chains(void *slot, uint64_t n) {
+  uint8_t *seg = static_cast<uint8_t *>(slot) + 8 * n;
+  return ChainView{seg, seg + ((n + 7) & ~7ull)};
+}
 void slots_from_rcrs(hipStream_t s, uint32_t sb, DView a, void *slot, void *side,
                      uint64_t side_cap, unsigned long long *side_ctr, unsigned int *status);
 void slots_identity(hipStream_t s, uint64_t n, void *slot);
+// The chains of a slot matrix (after every kernel of its product ran).
+void slots_chains(hipStream_t s, const void *slot, uint64_t n, ChainView c,
+                  const unsigned int *status);
 // C = I + A; parts as slots_lane (at most max_blocks blocks: the launch
 // has slots_plus_identity_blocks(n, max_blocks))
 uint32_t slots_plus_identity_blocks(uint64_t n, uint32_t max_blocks);
@@ -195,13 +221,20 @@ void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
 // written; others go to mid_list / big_list (counts in ctr->n_mid, n_big)
 // for slots_group.  parts receives 3 words per block (nnz, slice, ranges).
 // batch (without summaries): slice loads per round trip, 2, 4 or 8, about
-// the widest A range (narrow products then run fewer instructions).
+// the widest A range (narrow products then run fewer instructions).  With
+// b_chains.seg set, B slices are read through B's chains (no summaries);
+// with c_chains.seg set, the kernel writes C's chains (the group tiers' rows
+// end chains).  The chain form runs two launches on `grid` blocks: the rows
+// whose slice is a few chains, then the others from per-block lists in dlist
+// (at least nrows + grid * 256 words) counted in dcount (grid words); parts
+// then needs 6 words per block.
 void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t b_nrows,
                 uint32_t lane_max, uint32_t mid_max, uint32_t grid, void *c_slot, void *c_side,
                 uint64_t side_cap, unsigned long long *side_ctr, uint32_t *mid_list,
                 uint32_t *big_list, OpCounters *ctr, unsigned int *status, const void *summ,
                 uint32_t summ_min, uint32_t summ_rows, uint32_t plus_i,
-                unsigned long long *parts, uint32_t batch);
+                unsigned long long *parts, uint32_t batch, ChainView b_chains,
+                ChainView c_chains, uint32_t *dlist, uint32_t *dcount);
 // Both group tiers of a product in one launch: rows1 (*n1 of them) by groups
 // of lanes1 lanes (LDS window lds1 words) on grid1 blocks, rows2 by 64-lane
 // groups (lds2) on grid2 blocks; grids multiples of 8.
@@ -230,6 +263,6 @@ void slots_counts(hipStream_t s, SlotView a, uint64_t n, uint32_t *cnt,
                   const unsigned int *status);
 void slots_emit(hipStream_t s, SlotView a, uint64_t n, uint32_t sb, const void *rowmap,
                 uint32_t *entries);
-uint32_t slot_lane_blocks_per_cu(bool summaries, uint32_t batch);
+uint32_t slot_lane_blocks_per_cu(bool summaries, uint32_t batch, bool chains);
 
 }  // namespace dv

@@ -87,7 +87,8 @@ typedef struct dv_build_stats {
   uint32_t retried;        /* 1: the plan overflowed a predicted capacity and the
                               build was repeated with a readback per operation */
   uint32_t layout;         /* products' intermediate form: 0 slots, 1 range CRS */
-  uint32_t reserved;
+  uint32_t builds;         /* planned builds the call ran: 1, or for a batched call
+                              with autotune on, one per size class of components */
 } dv_build_stats;
 
 /* --- runtime ------------------------------------------------------------
@@ -152,7 +153,15 @@ void *dv_stream(dv_context *ctx);
  *                   except that a build shape whose products had side
  *                   entries for more than 75% of their rows is built in
  *                   range CRS from then on (a row of several ranges costs the
- *                   slot form a dependent read) */
+ *                   slot form a dependent read)
+ *   chain_span      slot-form products whose left operand's ranges may span
+ *                   at least this many rows (0: never) read the right operand
+ *                   through its chains, which every slot-form product writes:
+ *                   per 64-row chunk, where each row's run of consecutive
+ *                   single-range rows with non-decreasing, touching ends
+ *                   stops, and the chunk's union when all 64 rows are one
+ *                   such run; a slice costs a few loads per 64 rows instead of
+ *                   one per row (no interval summaries are computed) */
 typedef struct dv_tuning {
   uint32_t lds_words;
   uint32_t stage_entries;
@@ -166,6 +175,7 @@ typedef struct dv_tuning {
   uint32_t summary_span;
   uint32_t schedule;
   uint32_t layout;
+  uint32_t chain_span;
 } dv_tuning;
 int dv_get_tuning(const dv_context *ctx, dv_tuning *t);
 int dv_set_tuning(dv_context *ctx, const dv_tuning *t);

@@ -45,7 +45,7 @@ def test_abi_version_and_status_strings():
 def test_struct_sizes_match_header():
     assert ctypes.sizeof(N.dv_rcrs) == 8 * 6 + 4 * 2 + 8
     assert ctypes.sizeof(N.dv_build_stats) == 8 * 5 + 4 * 2 + 8 * 2 + 4 * 4
-    assert ctypes.sizeof(N.dv_tuning) == 4 * 12
+    assert ctypes.sizeof(N.dv_tuning) == 4 * 13
     assert ctypes.sizeof(N.dv_kernel_time) == 8 * 4
     assert ctypes.sizeof(N.dv_server_opts) == 8 + 4 * 4
     assert ctypes.sizeof(N.dv_request_info) == 4 * 4 + 8 * 4

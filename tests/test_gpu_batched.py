@@ -74,3 +74,21 @@ def test_graph_replays_match_eager(ctx):
         ctx.set_tuning(graphs=0)
         assert_identical(ctx.create_distance_index(da, 2, 5), orc.dindex(a, 2, 5), "eager between")
         ctx.set_tuning(graphs=1)
+
+
+def test_batched_size_classes_autotuned(ctx):
+    """With the autotuner on, components are built per size class (below 2^20
+    rows, below 2^24, larger), each class one block-diagonal build with its
+    own tuned launch shape; every component still equals the oracle."""
+    comps = [dv.generate_graph(2, backbone_bp=30000, seed=41),
+             dv.generate_graph(2, backbone_bp=1_200_000, seed=42),
+             dv.generate_graph(1, backbone_bp=8000, seed=43),
+             dv.generate_graph(2, backbone_bp=1_100_000, seed=44, size_bytes=4)]
+    assert sum(a.nrows >= 1 << 20 for a in comps) == 2
+    ctx.set_tuning(autotune=1)
+    print(f"batched size classes {[a.nrows for a in comps]}", flush=True)
+    for rep in range(2):
+        outs, st = ctx.create_distance_index_batched(comps, 5, 20, stats=True)
+        assert st["builds"] == 2
+        for c, (a, d) in enumerate(zip(comps, outs)):
+            assert_identical(d, orc.dindex(a, 5, 20), f"component {c} rep {rep}")

@@ -10,7 +10,7 @@ import pytest
 
 import diverg_amd as dv
 from oracle import oracle as orc
-from tests.rcrs_util import assert_identical, from_dense
+from tests.rcrs_util import assert_identical, from_dense, to_dense
 
 pytestmark = pytest.mark.gpu
 
@@ -212,3 +212,65 @@ def test_hub_row_repeated_builds(ctx, layout):
         assert_identical(d, want, f"hub build {build}")
         if layout == 0 and build > 0:
             assert st["layout"] == 1 and st["retried"] == 0
+
+
+def far_noncanonical(n, rng, far=3000):
+    """Rows of two or three ranges in random order, one of them far from the
+    row (outside the reference accumulator's 2048-bit band around it)."""
+    rows = []
+    for i in range(n):
+        r = [[min(i + 1, n - 1), min(i + 2, n - 1)]]
+        if rng.random() < 0.3:
+            c = int((i + rng.choice([-far, far])) % n)
+            r.append([c, min(c + int(rng.integers(0, 3)), n - 1)])
+        if rng.random() < 0.2:
+            c = int(rng.integers(0, n))
+            r.append([c, c])
+        rng.shuffle(r)
+        rows.append(r)
+    en = np.array([x for r in rows for pair in r for x in pair], np.uint32)
+    rm = np.concatenate([[0], np.cumsum([2 * len(r) for r in rows])]).astype(np.uint64)
+    return dv.RangeCRS(n, en, rm)
+
+
+def test_spgemm_noncanonical_far_ranges(ctx):
+    """A . B with non-canonical rows of B reaching outside the reference
+    accumulator's band: the reference keeps only what falls in its growing
+    window (range_sparse.hpp:1192-1242), and so does the product here."""
+    rng = np.random.default_rng(17)
+    b = far_noncanonical(9000, rng)
+    a = far_noncanonical(9000, rng)
+    want = orc.spgemm(a, b)
+    assert_identical(ctx.range_spgemm(a, b), want, "A.B non-canonical")
+    # the window matters here: the product with B's rows as sets differs
+    union = orc.spgemm(a, from_dense(to_dense(b)))
+    assert not (len(union.entries) == len(want.entries)
+                and np.array_equal(union.entries, want.entries))
+    for k in (1, 2, 5):
+        assert_identical(ctx.range_power(b, k), orc.power(b, k), f"B^{k} non-canonical")
+
+
+@pytest.mark.parametrize("layout", [0, 1, 2])
+def test_distance_index_noncanonical_far_ranges(ctx, layout):
+    """Distance indexes of an input whose non-canonical rows reach outside the
+    accumulator band, in every layout, first and repeated builds."""
+    rng = np.random.default_rng(23)
+    a = far_noncanonical(7000, rng)
+    ctx.set_tuning(layout=layout)
+    for dlo, dup in [(3, 9), (0, 6)]:
+        want = orc.dindex(a, dlo, dup)
+        print(f"non-canonical far ranges [{dlo},{dup}] layout {layout}", flush=True)
+        for build in range(2):
+            assert_identical(ctx.create_distance_index(a, dlo, dup), want,
+                             f"D[{dlo},{dup}] build {build}")
+
+
+def test_batched_with_noncanonical_component(ctx):
+    """A batched call with one non-canonical component: that component is
+    built on its own, the others in the assembly; all equal the oracle."""
+    rng = np.random.default_rng(29)
+    comps = [dv.generate_graph(1, backbone_bp=20000, seed=3), far_noncanonical(5000, rng),
+             dv.generate_graph(2, backbone_bp=30000, seed=4)]
+    outs = ctx.create_distance_index_batched(comps, 4, 12)
+    for c, (a, d) in enumerate(zip(comps, outs)):
+        assert_identical(d, orc.dindex(a, 4, 12), f"component {c}")
