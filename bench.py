@@ -16,14 +16,14 @@ Further legs (fields of the same JSON line; --legs selects them):
                 index (pairs drawn as in the serving workload, batches of
                 65536 resident in HBM): queries/s, per-batch p50/p99 latency,
                 the CPU lookup's rate on the same pairs, answers bit-exact
-  serving       (not in the default legs: dv_server_start faulted in its
-                first GPU run and stays off the GPU until a cause is shown)
-                the chromosome index (and the 5e6 bp one) resident in HBM, a
+  serving       the chromosome index (and the 5e6 bp one) resident in HBM, a
                 Poisson stream of locus-pair queries (mean 2e6/s offered;
                 sources uniform, half the targets 0-1000 bp downstream on the
                 backbone) with component builds of 1e5-2e7 bp interleaved:
                 queries/s, p50/p99 latency, build completion times, answers
-                checked against the CPU lookup
+                checked against the CPU lookup, builds in the mix against
+                plain builds; then the same queries all at once (saturation)
+                through the server and through plain dv_query_batch calls
   batched       24 components (log-normal lengths, median 5e6 bp) built in
                 one planned build over their block-diagonal assembly vs 24
                 plain builds; bit-exact per component
@@ -35,7 +35,10 @@ The JSON line also carries:
                 product per launch over its mean launch time (HIP events on the
                 library stream), against 8 TB/s
   cpu_baseline  the CPU oracle (the reference's OpenMP algorithm restated in
-                C) building the same index on the host, timed once
+                C) building the same index on the host: one thread per
+                physical core of the affinity mask within the job's CPU share,
+                OMP_PROC_BIND=close, OMP_PLACES=cores, 1 warm-up and the median
+                of 3, plus one 1-thread build
   bit_exact     the GPU index equals the oracle's (rowmap, entries, nnz)
 """
 import argparse
@@ -135,9 +138,13 @@ def main():
     ap.add_argument("--dlo", type=int, default=300)
     ap.add_argument("--dup", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="plain,chromosome,queries,batched",
-                    help="comma-separated legs to run (plain is always run); the serving leg "
-                         "is off by default: dv_server_start faulted in its first GPU run")
+    ap.add_argument("--legs", default="plain,chromosome,queries,serving,batched",
+                    help="comma-separated legs to run (plain is always run)")
+    ap.add_argument("--full-check", action="store_true",
+                    help="compare the whole chromosome index and every batched component with "
+                         "the oracle (minutes of CPU time); by default rows of the chromosome "
+                         "index are checked against the oracle's walk-length BFS and the "
+                         "batched components below 1e6 loci against the oracle")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-rate", type=float, default=2.0e6)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -333,14 +340,14 @@ def main():
     }
     legs = set(args.legs.split(","))
     if "chromosome" in legs or "serving" in legs or "queries" in legs:
-        out["chromosome"], chrom = leg_chromosome(ctx)
+        out["chromosome"], chrom = leg_chromosome(ctx, args.full_check)
         if "queries" in legs:
             out["queries"] = leg_queries(ctx, chrom)
         if "serving" in legs:
             out["serving"] = leg_serving(chrom, result_host, args)
         del chrom
     if "batched" in legs:
-        out["batched"] = leg_batched(ctx)
+        out["batched"] = leg_batched(ctx, args.full_check)
     ctx.close()
     print(json.dumps(out), flush=True)
 
@@ -349,7 +356,41 @@ def log(msg):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def leg_chromosome(ctx):
+def oracle_dindex_logged(a, dlo, dup, what):
+    """orc.dindex(a, dlo, dup) composed from the oracle's own operations in
+    the same order (dindex.hpp:46-189: rI + rA, rA^dlo and (rA + rI)^(dup -
+    dlo) by range_power_inplace, range_sparse.hpp:2376-2431, then one
+    product), with a line after each product, for builds that run minutes."""
+    from oracle import oracle as orc
+    t0 = time.time()
+    done = [0]
+
+    def mul(x, y):
+        z = orc.spgemm(x, y)
+        done[0] += 1
+        log(f"{what}: oracle product {done[0]} done ({time.time() - t0:.0f}s)")
+        return z
+
+    def power(p, k):
+        c = orc.identity(p.ncols)
+        while True:
+            if k & 1:
+                c = mul(c, p)
+            k >>= 1
+            if k == 0:
+                return c
+            p = mul(p, p)
+
+    rai = orc.spadd(orc.identity(a.nrows), a)
+    if dlo == 0:
+        return power(rai, dup)
+    rad = power(a, dlo)
+    raid = power(rai, dup - dlo)
+    del rai
+    return mul(rad, raid)
+
+
+def leg_chromosome(ctx, full_check=False):
     """5e7 bp graph with SVs, uint64 rowmap, [200, 1000]."""
     import diverg_amd as dv
     from oracle import oracle as orc
@@ -363,16 +404,17 @@ def leg_chromosome(ctx):
     dh = d.to_host()
     d.free()
     del da
-    # the whole index against the oracle, and sampled rows against the
-    # oracle's independent walk-length BFS
+    # sampled rows against the oracle's independent walk-length BFS; with
+    # full_check the whole index against the oracle
     orc.set_threads(cpu_threads())
-    log("chromosome: oracle build of the whole index")
-    t = time.perf_counter()
-    e = orc.dindex(a, 200, 1000)
-    oracle_s = time.perf_counter() - t
-    full_ok = bool(dh.nnz == e.nnz and np.array_equal(dh.rowmap, e.rowmap)
-                   and np.array_equal(dh.entries, e.entries))
-    del e
+    full_ok, oracle_s = None, None
+    if full_check:
+        t = time.perf_counter()
+        e = oracle_dindex_logged(a, 200, 1000, "chromosome")
+        oracle_s = time.perf_counter() - t
+        full_ok = bool(dh.nnz == e.nnz and np.array_equal(dh.rowmap, e.rowmap)
+                       and np.array_equal(dh.entries, e.entries))
+        del e
     rng = np.random.default_rng(0xD1E5)
     rows = np.sort(rng.choice(a.nrows, 1500, replace=False)).astype(np.uint32)
     w = orc.walk_rows(a, rows, 200, 1000)
@@ -390,12 +432,12 @@ def leg_chromosome(ctx):
         "index_ranges": len(dh.entries) // 2,
         "index_nnz": dh.nnz,
         "bit_exact_vs_oracle": full_ok,
-        "oracle_s": round(oracle_s, 2),
+        "oracle_s": round(oracle_s, 2) if oracle_s else None,
         "rows_checked_vs_walk_bfs": len(rows),
         "rows_bit_exact": bool(ok),
     }
-    log(f"chromosome build {res['build_s']}s, index bit-exact {full_ok} (oracle "
-        f"{oracle_s:.1f}s), BFS rows bit-exact {ok}")
+    log(f"chromosome build {res['build_s']}s, whole index vs oracle {full_ok}, "
+        f"BFS rows bit-exact {ok}")
     return res, (a, vpos, dh)
 
 
@@ -530,13 +572,40 @@ def leg_serving(chrom, idx2, args):
         res = srv.replay(qidx, qi, qj, arrival, builds)
         built = [srv.get_index(int(i)) for i in res["build_index"]]
     ok = True
+    exps = []
     for k in range(2):
         s = qidx == k
         exp, rc = orc.query(indexes[k], qi[s], qj[s])
+        exps.append(exp)
         ok = ok and rc == 0 and np.array_equal(res["answers"][s], exp)
     lat = res["latency_us"]
     done = arrival + lat
     span = (done.max() - arrival.min()) / 1e6
+    # saturation: every query at once, through the server (continuous
+    # batching, batches overlapping on the device) and through plain
+    # dv_query_batch calls of the same batch size, one index at a time
+    with dv.Server() as srv:
+        for x in indexes:
+            srv.add_index(x)
+        sat = srv.replay(qidx, qi, qj, np.zeros(n), ())
+    sat_ok = all(np.array_equal(sat["answers"][qidx == k], exps[k]) for k in range(2))
+    sat_qps = n / (sat["elapsed_us"] / 1e6)
+    with dv.Context(0) as c3:
+        dix = [c3.to_device(x) for x in indexes]
+        plain_ans = np.zeros(n, np.uint8)
+        batch = 65536
+        order = [np.flatnonzero(qidx == k) for k in range(2)]
+        hip_sync()
+        t = time.perf_counter()
+        for k in range(2):
+            sel = order[k]
+            for b0 in range(0, len(sel), batch):
+                part = sel[b0:b0 + batch]
+                plain_ans[part] = c3.query(dix[k], qi[part], qj[part])
+        plain_s = time.perf_counter() - t
+        for x in dix:
+            x.free()
+    plain_ok = all(np.array_equal(plain_ans[qidx == k], exps[k]) for k in range(2))
     # builds in the mix against plain builds of the same components
     bok = True
     with dv.Context(0) as c2:
@@ -557,13 +626,23 @@ def leg_serving(chrom, idx2, args):
         "build_completion_s": [round(x / 1e6, 4) for x in res["build_done_us"].tolist()],
         "answers_bit_exact_vs_cpu_lookup": bool(ok),
         "builds_in_mix_equal_alone": bool(bok),
+        "saturation": {
+            "workload": f"the same {n} queries all arriving at once, host arrays in and out",
+            "served_queries_per_s": round(sat_qps, 1),
+            "served_p50_us": round(float(np.percentile(sat["latency_us"], 50)), 2),
+            "served_p99_us": round(float(np.percentile(sat["latency_us"], 99)), 2),
+            "plain_batch_calls_queries_per_s": round(n / plain_s, 1),
+            "served_over_plain": round(sat_qps * plain_s / n, 3),
+            "bit_exact": bool(sat_ok and plain_ok),
+        },
     }
     log(f"serving: {out['queries_per_s']:.0f} q/s, p50 {out['p50_us']} us, "
-        f"p99 {out['p99_us']} us, exact {ok}, builds {out['build_completion_s']}")
+        f"p99 {out['p99_us']} us, exact {ok}, builds {out['build_completion_s']}; "
+        f"saturated {sat_qps:.3g} q/s served vs {n / plain_s:.3g} q/s plain calls")
     return out
 
 
-def leg_batched(ctx):
+def leg_batched(ctx, full_check=False):
     """24 components in one planned build vs 24 plain builds."""
     import diverg_amd as dv
     t = time.time()
@@ -606,16 +685,18 @@ def leg_batched(ctx):
     from oracle import oracle as orc
     orc.set_threads(cpu_threads())
     exact = oracle_exact = True
+    checked = 0
     t = time.perf_counter()
     for c, (comp, o, to, p) in enumerate(zip(comps, outs, touts, plain)):
         got = o.to_host()
         exact = exact and dv.identical(got, p.to_host()) and dv.identical(got, to.to_host())
-        oracle_exact = oracle_exact and dv.identical(got, orc.dindex(comp, 300, 600))
+        if full_check or comp.nrows < 1_000_000:
+            oracle_exact = oracle_exact and dv.identical(got, orc.dindex(comp, 300, 600))
+            checked += 1
+            log(f"batched: component {c} ({comp.nrows} loci) checked against the oracle")
         o.free()
         to.free()
         p.free()
-        if c % 6 == 5:
-            log(f"batched: {c + 1}/24 components checked against the oracle")
     oracle_s = time.perf_counter() - t
     res = {
         "workload": "24 components, log-normal lengths (median 5e6 bp, sigma 1, clipped "
@@ -633,6 +714,7 @@ def leg_batched(ctx):
         "peak_hbm_bytes": st["peak_device_bytes"],
         "bit_exact_per_component": bool(exact),
         "bit_exact_vs_oracle_per_component": bool(oracle_exact),
+        "components_checked_vs_oracle": checked,
         "oracle_s": round(oracle_s, 2),
     }
     log(f"batched {batched_s:.3f}s (per size class, tuned: {tuned_s:.3f}s in "

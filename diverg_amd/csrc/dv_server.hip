@@ -7,6 +7,10 @@
 // completes batches as their events fire; one build thread admits and runs
 // builds.  Queries run on their own stream (their own hardware queue), builds
 // on another.  All streams keep the default priority.
+//
+// Queries are queued as segments (a request's queries, or the replayed
+// queries that fell due together), so packing a batch and handing answers
+// back are contiguous copies, not work per query.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,6 +42,7 @@ struct Request {
   uint64_t n = 0;
   double submit_us = 0, start_us = 0, done_us = 0;
   std::vector<uint8_t> answers;
+  std::vector<uint32_t> qi, qj;  // the request's pairs (a private copy)
   std::atomic<uint64_t> remaining{0};
   // build
   std::vector<uint8_t> rowmap;
@@ -47,22 +52,29 @@ struct Request {
   uint64_t est_bytes = 0;
 };
 
-struct QItem {
-  uint32_t idx, i, j;
-  Request *req;       // ticketed request, or nullptr for replayed queries
-  uint64_t pos;       // position in req->answers
-  uint8_t *ans_out;   // replay: answer slot
-  double *lat_out;    // replay: latency slot
-  double arrive_us;
+// Queries [first, first + count) of a request or of a replayed stream.
+struct Seg {
+  Request *req;              // ticketed request, or nullptr for replayed queries
+  uint32_t idx;              // the index of every query (requests)
+  const uint32_t *qidx;      // replay: per-query index ids
+  const uint32_t *qi, *qj;   // the pairs (request copies or the replay arrays)
+  uint64_t first, count;
+  uint8_t *ans;              // replay: answers array
+  double *lat;               // replay: latency array
+  const double *arrival;     // replay: arrival times, microseconds after t0_us
+  double t0_us;
 };
 
+// One batch in flight: pinned host staging (index ids, rows, columns:
+// max_batch words each; answers) and their device copies.
 struct Slot {
   hipEvent_t ev = nullptr;
-  uint32_t *h_in = nullptr;  // packed (index, i, j)
+  uint32_t *h_in = nullptr;
   uint8_t *h_out = nullptr;
-  void *d_in = nullptr;
+  uint32_t *d_in = nullptr;
   uint8_t *d_out = nullptr;
-  std::vector<QItem> items;
+  std::vector<std::pair<Seg, uint64_t>> items;  // pieces and their batch offsets
+  uint64_t n = 0;
   bool busy = false;
 };
 
@@ -96,7 +108,7 @@ struct dv_server {
 
   // queries
   std::mutex q_mu;
-  std::deque<QItem> pending;
+  std::deque<Seg> pending;
   std::vector<Slot> slots;
   std::atomic<uint64_t> replay_done{0};
 
@@ -150,42 +162,70 @@ struct dv_server {
 
   void complete_slot(Slot &s) {
     const double t = now_us();
-    for (size_t k = 0; k < s.items.size(); ++k) {
-      const QItem &it = s.items[k];
-      const uint8_t a = s.h_out[k];
-      if (it.req) {
-        it.req->answers[it.pos] = a;
-        if (it.req->remaining.fetch_sub(1) == 1) finish(it.req, DV_OK);
+    uint64_t replayed = 0;
+    for (const auto &piece : s.items) {
+      const Seg &g = piece.first;
+      const uint8_t *src = s.h_out + piece.second;
+      if (g.req) {
+        std::memcpy(g.req->answers.data() + g.first, src, g.count);
+        if (g.req->remaining.fetch_sub(g.count) == g.count) finish(g.req, DV_OK);
       } else {
-        *it.ans_out = a;
-        *it.lat_out = t - it.arrive_us;
-        replay_done.fetch_add(1, std::memory_order_release);
+        std::memcpy(g.ans + g.first, src, g.count);
+        for (uint64_t q = 0; q < g.count; ++q)
+          g.lat[g.first + q] = t - (g.t0_us + g.arrival[g.first + q]);
+        replayed += g.count;
       }
     }
+    if (replayed) replay_done.fetch_add(replayed, std::memory_order_release);
     s.items.clear();
+    s.n = 0;
     s.busy = false;
   }
 
   bool launch_slot(Slot &s) {
+    const uint64_t cap = opts.max_batch;
     {
       std::lock_guard<std::mutex> lk(q_mu);
       if (pending.empty()) return false;
-      const size_t take = std::min<size_t>(pending.size(), opts.max_batch);
-      s.items.assign(pending.begin(), pending.begin() + take);
-      pending.erase(pending.begin(), pending.begin() + take);
+      uint64_t n = 0;
+      while (n < cap && !pending.empty()) {
+        Seg &g = pending.front();
+        const uint64_t take = std::min<uint64_t>(g.count, cap - n);
+        Seg piece = g;
+        piece.count = take;
+        s.items.emplace_back(piece, n);
+        n += take;
+        if (take == g.count) {
+          pending.pop_front();
+        } else {
+          g.first += take;
+          g.count -= take;
+        }
+      }
+      s.n = n;
     }
     const double t = now_us();
-    const uint32_t n = (uint32_t)s.items.size();
-    for (uint32_t k = 0; k < n; ++k) {
-      const QItem &it = s.items[k];
-      s.h_in[3 * k] = it.idx;
-      s.h_in[3 * k + 1] = it.i;
-      s.h_in[3 * k + 2] = it.j;
-      if (it.req && it.req->start_us == 0) it.req->start_us = t;
+    uint32_t *h_idx = s.h_in, *h_i = s.h_in + cap, *h_j = s.h_in + 2 * cap;
+    for (const auto &piece : s.items) {
+      const Seg &g = piece.first;
+      const uint64_t o = piece.second;
+      if (g.qidx) std::memcpy(h_idx + o, g.qidx + g.first, 4 * g.count);
+      else std::fill(h_idx + o, h_idx + o + g.count, g.idx);
+      std::memcpy(h_i + o, g.qi + g.first, 4 * g.count);
+      std::memcpy(h_j + o, g.qj + g.first, 4 * g.count);
+      if (g.req && g.req->start_us == 0) g.req->start_us = t;
     }
+    const uint32_t n = (uint32_t)s.n;
     const uint32_t nt = ntable.load(std::memory_order_acquire);
-    (void)hipMemcpyAsync(s.d_in, s.h_in, 12ull * n, hipMemcpyHostToDevice, qstream);
-    dv::query_multi(qstream, d_table, nt, s.d_in, n, s.d_out);
+    // one copy when the batch is full, else the three used prefixes
+    if (s.n == cap) {
+      (void)hipMemcpyAsync(s.d_in, s.h_in, 12ull * cap, hipMemcpyHostToDevice, qstream);
+    } else {
+      for (int k = 0; k < 3; ++k)
+        (void)hipMemcpyAsync(s.d_in + k * cap, s.h_in + k * cap, 4ull * n, hipMemcpyHostToDevice,
+                             qstream);
+    }
+    dv::query_multi(qstream, d_table, nt, s.d_in, s.d_in + cap, s.d_in + 2 * cap, n, s.d_out);
     (void)hipMemcpyAsync(s.h_out, s.d_out, n, hipMemcpyDeviceToHost, qstream);
     (void)hipEventRecord(s.ev, qstream);
     s.busy = true;
@@ -205,7 +245,7 @@ struct dv_server {
             any_busy = true;
           } else {  // the batch failed: answer 2 (no answer) and move on
             (void)hipGetLastError();
-            std::memset(s.h_out, 2, s.items.size());
+            std::memset(s.h_out, 2, s.n);
             complete_slot(s);
             did = true;
           }
@@ -274,7 +314,11 @@ struct dv_server {
     return it == reqs.end() ? nullptr : it->second.get();
   }
 
-  int submit_build(const dv_rcrs *a, uint32_t dlo, uint32_t dup, uint64_t *ticket) {
+  // copy: take a private copy of `a` (the caller's buffers may go away when
+  // the call returns); without it the caller keeps them alive until the
+  // build has finished (the replay load generator does)
+  int submit_build(const dv_rcrs *a, uint32_t dlo, uint32_t dup, uint64_t *ticket,
+                   bool copy = true) {
     if (!a || !ticket || !(dlo <= dup && dup != 0) || a->nrows != a->ncols ||
         (a->size_bytes != 4 && a->size_bytes != 8))
       return DV_ERR_INVALID;
@@ -286,23 +330,25 @@ struct dv_server {
     r->est_bytes = 256ull * (a->nrows + 1) + (uint64_t)a->size_bytes * (a->nrows + 1) +
                    4 * a->n_entries;
     if (r->est_bytes > opts.hbm_budget_bytes) return DV_ERR_BUDGET;
-    // private host copy, so the caller's buffers may go away
-    const uint64_t rmb = (uint64_t)a->size_bytes * (a->nrows + 1);
-    r->rowmap.resize(rmb);
-    r->entries.resize(a->n_entries);
-    const hipMemcpyKind kind = a->location == DV_DEVICE ? hipMemcpyDeviceToHost
-                                                        : hipMemcpyHostToHost;
-    if (hipMemcpy(r->rowmap.data(), a->rowmap, rmb, kind) != hipSuccess ||
-        (a->n_entries && hipMemcpy(r->entries.data(), a->entries, 4 * a->n_entries, kind) !=
-                             hipSuccess)) {
-      (void)hipGetLastError();
-      return DV_ERR_DEVICE;
-    }
     r->a = *a;
-    r->a.rowmap = r->rowmap.data();
-    r->a.entries = r->entries.data();
-    r->a.location = DV_HOST;
     r->a.owner = nullptr;
+    if (copy) {
+      // private host copy, so the caller's buffers may go away
+      const uint64_t rmb = (uint64_t)a->size_bytes * (a->nrows + 1);
+      r->rowmap.resize(rmb);
+      r->entries.resize(a->n_entries);
+      const hipMemcpyKind kind = a->location == DV_DEVICE ? hipMemcpyDeviceToHost
+                                                          : hipMemcpyHostToHost;
+      if (hipMemcpy(r->rowmap.data(), a->rowmap, rmb, kind) != hipSuccess ||
+          (a->n_entries && hipMemcpy(r->entries.data(), a->entries, 4 * a->n_entries, kind) !=
+                               hipSuccess)) {
+        (void)hipGetLastError();
+        return DV_ERR_DEVICE;
+      }
+      r->a.rowmap = r->rowmap.data();
+      r->a.entries = r->entries.data();
+      r->a.location = DV_HOST;
+    }
     Request *raw = r.get();
     *ticket = add_request(std::move(r));
     {
@@ -350,7 +396,7 @@ int dv_server_start(int device, const dv_server_opts *opts, dv_server **out) {
              hipSuccess &&
          hipHostMalloc((void **)&sl.h_out, s->opts.max_batch, hipHostMallocDefault) ==
              hipSuccess &&
-         hipMalloc(&sl.d_in, 12ull * s->opts.max_batch) == hipSuccess &&
+         hipMalloc((void **)&sl.d_in, 12ull * s->opts.max_batch) == hipSuccess &&
          hipMalloc((void **)&sl.d_out, s->opts.max_batch) == hipSuccess;
   }
   if (!ok) {
@@ -414,6 +460,8 @@ int dv_server_submit_queries(dv_server *s, uint32_t index_id, const uint32_t *qi
   r->n = n;
   r->submit_us = now_us();
   r->answers.assign(n, 0);
+  r->qi.assign(qi, qi + n);
+  r->qj.assign(qj, qj + n);
   r->remaining = n;
   Request *raw = r.get();
   *ticket = s->add_request(std::move(r));
@@ -422,8 +470,8 @@ int dv_server_submit_queries(dv_server *s, uint32_t index_id, const uint32_t *qi
     return DV_OK;
   }
   std::lock_guard<std::mutex> lk(s->q_mu);
-  for (uint64_t k = 0; k < n; ++k)
-    s->pending.push_back(QItem{index_id, qi[k], qj[k], raw, k, nullptr, nullptr, raw->submit_us});
+  s->pending.push_back(Seg{raw, index_id, nullptr, raw->qi.data(), raw->qj.data(), 0, n, nullptr,
+                           nullptr, nullptr, 0});
   return DV_OK;
 }
 
@@ -513,15 +561,17 @@ int dv_server_replay(dv_server *s, uint64_t n, const uint32_t *qidx, const uint3
     const double t = now_us() - t0;
     while (bpos < nbuilds && build_us[border[bpos]] <= t) {
       const uint32_t b = border[bpos++];
-      build_rc[b] = s->submit_build(&builds[b], build_dlo[b], build_dup[b], &tickets[b]);
+      // the caller's build inputs outlive the replay: no copy on this thread
+      build_rc[b] = s->submit_build(&builds[b], build_dlo[b], build_dup[b], &tickets[b], false);
     }
     if (qpos < n && arrival_us[qpos] <= t) {
+      // every query due by now, as one segment
+      const uint64_t end =
+          (uint64_t)(std::upper_bound(arrival_us + qpos, arrival_us + n, t) - arrival_us);
       std::lock_guard<std::mutex> lk(s->q_mu);
-      while (qpos < n && arrival_us[qpos] <= t) {
-        s->pending.push_back(QItem{qidx[qpos], qi[qpos], qj[qpos], nullptr, 0, &answers[qpos],
-                                   &latency_us[qpos], t0 + arrival_us[qpos]});
-        ++qpos;
-      }
+      s->pending.push_back(Seg{nullptr, 0, qidx, qi, qj, qpos, end - qpos, answers, latency_us,
+                               arrival_us, t0});
+      qpos = end;
     }
   }
   while (s->replay_done.load(std::memory_order_acquire) - done0 < n) std::this_thread::yield();

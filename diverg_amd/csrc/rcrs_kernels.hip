@@ -1448,10 +1448,12 @@ __global__ __launch_bounds__(BLOCK) void k_slice_lengths(Mat<S> A, Mat<S> B, uin
 // table of resident indexes (each with its own rowmap width).
 __global__ __launch_bounds__(BLOCK) void k_query_multi(const IndexDesc *__restrict__ table,
                                                        uint32_t ntable,
-                                                       const uint3 *__restrict__ q, uint32_t n,
+                                                       const uint32_t *__restrict__ qidx,
+                                                       const uint32_t *__restrict__ qi,
+                                                       const uint32_t *__restrict__ qj, uint32_t n,
                                                        uint8_t *__restrict__ out) {
   for (uint32_t t = blockIdx.x * BLOCK + threadIdx.x; t < n; t += gridDim.x * BLOCK) {
-    const uint3 v = q[t];  // x: index id, y: i, z: j
+    const uint3 v = make_uint3(qidx[t], qi[t], qj[t]);  // x: index id, y: i, z: j
     if (v.x >= ntable) {
       out[t] = 2;
       continue;
@@ -2973,11 +2975,11 @@ void slice_nnz(hipStream_t s, const uint32_t *entries, uint64_t n_entries,
                      s, reinterpret_cast<const uint2 *>(entries), n_entries / 2, out);
 }
 
-void query_multi(hipStream_t s, const IndexDesc *table, uint32_t ntable, const void *q, uint32_t n,
-                 uint8_t *out) {
+void query_multi(hipStream_t s, const IndexDesc *table, uint32_t ntable, const uint32_t *qidx,
+                 const uint32_t *qi, const uint32_t *qj, uint32_t n, uint8_t *out) {
   if (!n) return;
   hipLaunchKernelGGL(k_query_multi, dim3(grid_for(n, BLOCK, 1024)), dim3(BLOCK), 0, s, table,
-                     ntable, static_cast<const uint3 *>(q), n, out);
+                     ntable, qidx, qi, qj, n, out);
 }
 
 void validate_matrix(hipStream_t s, uint32_t sb, DView m, uint64_t n_entries, uint32_t *flag) {

@@ -146,10 +146,10 @@ struct IndexDesc {
   uint32_t sb;
   uint32_t pad;
 };
-// out[t] = D_q(i, j) for the packed queries q[t] = (index id, i, j) (12
-// bytes each); 2 for an unknown index or (i, j) outside it.
-void query_multi(hipStream_t s, const IndexDesc *table, uint32_t ntable, const void *q, uint32_t n,
-                 uint8_t *out);
+// out[t] = D_q(i, j) for query t = (qidx[t], qi[t], qj[t]) (index id, row,
+// column); 2 for an unknown index or (i, j) outside it.
+void query_multi(hipStream_t s, const IndexDesc *table, uint32_t ntable, const uint32_t *qidx,
+                 const uint32_t *qi, const uint32_t *qj, uint32_t n, uint8_t *out);
 
 // Sets *flag when rowmap[0] != 0, rowmap[n] != n_entries, a row length is
 // odd or negative, or a range is not s <= e < ncols.

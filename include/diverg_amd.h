@@ -339,7 +339,9 @@ int dv_server_get_index(dv_server *srv, uint32_t index_id, dv_rcrs *out);
  * the call returns when every query and build has finished.  answers[q] gets
  * the answer, latency_us[q] the time from arrival to the answer being on the
  * host; build_done_us[b] the completion time of build b and build_index[b]
- * the index it produced.  *elapsed_us is the replay's wall time. */
+ * the index it produced.  *elapsed_us is the replay's wall time.  The build
+ * inputs are read in place (no copy on the submitting thread), so they must
+ * stay valid until the call returns. */
 int dv_server_replay(dv_server *srv, uint64_t n, const uint32_t *qidx, const uint32_t *qi,
                      const uint32_t *qj, const double *arrival_us, uint8_t *answers,
                      double *latency_us, uint32_t nbuilds, const dv_rcrs *builds,
