@@ -1871,88 +1871,6 @@ struct Chains {
   const uint2 *__restrict__ chunk;
 };
 constexpr uint32_t CHAIN_ROWS = 64;
-constexpr int CHUNK_BATCH = 4;
-
-// Fold B rows [j0, j1] of one chunk: v0 = slot j0, t0 = the last row of the
-// chain starting at j0, v1 = slot j1 (all loaded by the caller).  A chain
-// covering [j0, j1] folds as one interval; otherwise the rows are walked
-// chain by chain.
-__device__ __forceinline__ void fold_chain_part(const Slots &B, const uint8_t *__restrict__ seg,
-                                                uint32_t j0, uint32_t j1, uint2 v0, uint32_t t0,
-                                                uint2 v1, uint32_t &cl, uint32_t &ch, Runs &r,
-                                                bool &ok, unsigned long long &gathered) {
-  if (j1 > j0 && t0 >= j1) {
-    fold(v0.x, v1.y, cl, ch, r, ok);
-    return;
-  }
-  uint32_t j = j0, t = t0;
-  uint2 v = v0;
-  for (;;) {
-    if (t > j) {  // rows j .. b are one chain: [lo(j), hi(b)]
-      const uint32_t b = min(t, j1);
-      const uint32_t h = b == j1 ? v1.y : B.slot[b].y;
-      fold(v.x, h, cl, ch, r, ok);
-      j = b + 1;
-    } else {  // row j alone: a range, a marked row or an empty row
-      if (v.x <= v.y) fold(v.x, v.y, cl, ch, r, ok);
-      else if (v.x == SLOT_MARK) fold_side(B.side, v.y, cl, ch, r, ok, gathered);
-      j = j + 1;
-    }
-    if (!ok || j > j1) break;
-    gathered += 1;
-    v = j == j1 ? v1 : B.slot[j];
-    t = (j & ~(CHAIN_ROWS - 1)) | seg[j];
-  }
-}
-
-// Fold B rows [s, e] through the chains: the head (s to the end of its
-// chunk), whole chunks by their chunk summaries, the tail (the start of e's
-// chunk to e); the loads of the head, the tail and the first CHUNK_BATCH
-// whole chunks are issued together.
-__device__ __forceinline__ void fold_chains(const Slots &B, const Chains &C, uint32_t s,
-                                            uint32_t e, uint32_t &cl, uint32_t &ch, Runs &r,
-                                            bool &ok, unsigned long long &gathered) {
-  const uint32_t c0 = s / CHAIN_ROWS, c1 = e / CHAIN_ROWS;
-  const uint32_t e0 = c0 == c1 ? e : c0 * CHAIN_ROWS + (CHAIN_ROWS - 1);
-  const uint32_t s1 = c1 * CHAIN_ROWS;
-  const uint2 v0 = B.slot[s], v1 = B.slot[e0];
-  const uint32_t t0 = c0 * CHAIN_ROWS + C.seg[s];
-  uint2 w0 = make_uint2(1u, 0u), w1 = w0;
-  uint32_t u0 = 0;
-  uint2 m[CHUNK_BATCH];
-  if (c1 != c0) {
-    w0 = B.slot[s1];
-    u0 = s1 + C.seg[s1];
-    w1 = B.slot[e];
-#pragma unroll
-    for (int q = 0; q < CHUNK_BATCH; ++q)
-      m[q] = c0 + 1 + q < c1 ? C.chunk[c0 + 1 + q] : make_uint2(1u, 0u);
-  }
-  gathered += 3;
-  fold_chain_part(B, C.seg, s, e0, v0, t0, v1, cl, ch, r, ok, gathered);
-  if (c1 == c0) return;
-  for (uint32_t c = c0 + 1; ok && c < c1; c += CHUNK_BATCH) {
-    if (c != c0 + 1) {
-#pragma unroll
-      for (int q = 0; q < CHUNK_BATCH; ++q)
-        m[q] = c + q < c1 ? C.chunk[c + q] : make_uint2(1u, 0u);
-    }
-#pragma unroll
-    for (int q = 0; q < CHUNK_BATCH; ++q) {
-      if (!ok || c + q >= c1) continue;
-      gathered += 1;
-      if (m[q].x <= m[q].y) {
-        fold(m[q].x, m[q].y, cl, ch, r, ok);
-      } else {  // not one chain: its chains one by one
-        const uint32_t b = (c + q) * CHAIN_ROWS;
-        fold_chain_part(B, C.seg, b, b + CHAIN_ROWS - 1, B.slot[b], b + C.seg[b],
-                        B.slot[b + CHAIN_ROWS - 1], cl, ch, r, ok, gathered);
-      }
-    }
-  }
-  gathered += 3;
-  if (ok) fold_chain_part(B, C.seg, s1, e, w0, u0, w1, cl, ch, r, ok, gathered);
-}
 
 // The chains of the 64 rows of this wave (one chunk): `one` when the lane's
 // row is the single range [lo, hi].  Every lane of the wave calls it.
@@ -2011,7 +1929,7 @@ __global__ __launch_bounds__(BLOCK) void k_slot_chains(const uint2 *__restrict__
       suf, pre2, suf2, summ_min, summ_shift, plus_i, parts, BC, c_seg, c_chunk, dlist,      \
       dcount, dcap
 
-enum SlotMode { SLOT_PLAIN = 0, SLOT_SUMM = 1, SLOT_CHAIN = 2 };
+enum SlotMode { SLOT_PLAIN = 0, SLOT_SUMM = 1 };
 
 // NB: slice loads per round trip (LANE_BATCH, or 2 / 4 through
 // fold_slots_tight for products of narrow slices) in SLOT_PLAIN mode.
@@ -2089,9 +2007,7 @@ __device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
       if (len > lane_max) ok = false;
       if (!ok) continue;
       if (plus_i) fold(s, e, cl, ch, r, ok);  // C = A (B + I)
-      if constexpr (MODE == SLOT_CHAIN) {
-        fold_chains(B, BC, s, e, cl, ch, r, ok, slice_acc);
-      } else if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min &&
+      if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min &&
           (e >> summ_shift) - (s >> summ_shift) < 64) {
         fold_summarised(
             pre, suf, pre2, suf2, s, e, summ_shift,
