@@ -145,6 +145,9 @@ SIGNATURES = [
     ("dv_init", C.c_int, [C.c_int, P(VP)]),
     ("dv_finalize", C.c_int, [VP]),
     ("dv_synchronize", C.c_int, [VP]),
+    ("dv_range_spadd_noalloc_max_rows", C.c_uint64, []),
+    ("dv_range_spadd_noalloc", C.c_int,
+     [VP, P(dv_rcrs), P(dv_rcrs), VP, U32P, C.c_uint64, P(C.c_uint64), P(C.c_uint64), VP]),
     ("dv_last_error", C.c_char_p, [VP]),
     ("dv_stream", VP, [VP]),
     ("dv_get_tuning", C.c_int, [VP, P(dv_tuning)]),

@@ -12,7 +12,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdiverg_amd.so")
 
 SOURCES = ["rcrs_kernels.hip", "dv_api.hip", "dv_ops.hip", "dv_plan.hip", "dv_slots.hip", "dv_batch.hip",
-           "dv_server.hip", "dv_graph.hip", "vgraph_gen.cpp", "vgraph_gfa.cpp", "dv_io.cpp"]
+           "dv_server.hip", "dv_graph.hip", "dv_noalloc.hip", "vgraph_gen.cpp", "vgraph_gfa.cpp",
+           "dv_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

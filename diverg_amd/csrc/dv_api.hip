@@ -355,6 +355,9 @@ int dv_init(int device, dv_context **out) {
     return DV_ERR_DEVICE;
   }
   c->device = device;
+  // the scratch array of dv_range_spadd_noalloc on this device (a failure
+  // here is reported by that entry point, which retries it)
+  (void)dv::noalloc_setup(device);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     c->num_cu = (uint32_t)prop.multiProcessorCount;

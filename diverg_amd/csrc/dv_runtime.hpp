@@ -284,6 +284,10 @@ struct Ctx {
   uint32_t stats_classes = 1;  // builds a batched build ran (one per size class)
 };
 
+// Resolve the allocation-free entry points' scratch array on a device
+// (dv_noalloc.hip; dv_init calls it).
+int noalloc_setup(int device);
+
 }  // namespace dv
 
 // The C ABI's opaque context is the library context.

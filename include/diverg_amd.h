@@ -46,7 +46,8 @@ typedef enum dv_status {
                            or an entry count that does not fit a 32-bit rowmap */
   DV_ERR_NOMEM = 3,     /* host or device allocation failed */
   DV_ERR_DEVICE = 4,    /* HIP runtime error */
-  DV_ERR_BUDGET = 5,    /* serving: request exceeds the HBM budget */
+  DV_ERR_BUDGET = 5,    /* serving: request exceeds the HBM budget; allocation-free
+                           entry points: input beyond their scratch */
   DV_ERR_RANGE = 6,     /* query (i, j) outside the matrix (crs_matrix.hpp:1222-1243) */
   DV_ERR_BUSY = 7,      /* serving: no free slot; retry */
   DV_ERR_NOT_FOUND = 8, /* serving: unknown ticket or index id */
@@ -281,6 +282,26 @@ int dv_merge_distance_index(dv_context *ctx, const dv_rcrs *d1, const dv_rcrs *d
 int dv_save(dv_context *ctx, const dv_rcrs *m, const char *path);
 int dv_load(dv_context *ctx, const char *path, uint32_t size_bytes, uint32_t location,
             dv_rcrs *out);
+
+/* --- allocation-free SpAdd -------------------------------------------------
+ * No counterpart in the reference: a variant of dv_range_spadd for callers
+ * that cannot allocate device memory (code under stream capture, serving
+ * loops, harnesses that forbid it).  C = A + B as dv_range_spadd computes it
+ * (range_sparse.hpp:112-428), with A and B device resident (same shape and
+ * size type) and C written to caller memory: c_rowmap (nrows + 1 words of
+ * the size type) and c_entries (c_capacity words; len(A) + len(B) always
+ * suffices).  *c_n_entries receives C's entry count (also when it exceeds
+ * c_capacity: DV_ERR_OVERFLOW, C not written), *c_nnz its nnz (wrapped to 32
+ * bits for a 32-bit rowmap).  stream: a hipStream_t, or NULL for the
+ * context's; the call returns when C is complete.  Its scratch is one
+ * library-owned device array of DV_NOALLOC_SCRATCH_BYTES per GPU (held until
+ * the process ends): at most dv_range_spadd_noalloc_max_rows() rows, more is
+ * DV_ERR_BUDGET.  Calls that overlap on one GPU share that array and race. */
+#define DV_NOALLOC_SCRATCH_BYTES (32ull << 20)
+uint64_t dv_range_spadd_noalloc_max_rows(void);
+int dv_range_spadd_noalloc(dv_context *ctx, const dv_rcrs *a, const dv_rcrs *b, void *c_rowmap,
+                           uint32_t *c_entries, uint64_t c_capacity, uint64_t *c_n_entries,
+                           uint64_t *c_nnz, void *stream);
 
 /* --- serving --------------------------------------------------------------
  * A serving loop over HBM-resident indexes (no counterpart in the reference,
