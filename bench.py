@@ -29,11 +29,13 @@ Further legs (fields of the same JSON line; --legs selects them):
                 plain builds; bit-exact per component
 
 The JSON line also carries:
-  roofline      the SpGEMM lane-per-row kernel (the dominant kernel:
-                k_slot_lane for slot-form products, k_spgemm_lane for range-CRS
-                ones): algorithmic bytes bytes(X) + bytes(Y) + bytes(C) of its
-                product per launch over its mean launch time (HIP events on the
-                library stream), against 8 TB/s
+  roofline      the SpGEMM lane tier, the dominant work of a build: per
+                product, its lane-per-row launches (slot form: k_slot_fast
+                then k_slot_list for products read through chains, else
+                k_slot_lane or k_slot_lane_narrow; range CRS: k_spgemm_lane),
+                timed together with HIP events on the library stream;
+                algorithmic bytes bytes(X) + bytes(Y) + bytes(C) of the
+                product over that time, against 8 TB/s
   cpu_baseline  the CPU oracle (the reference's OpenMP algorithm restated in
                 C) building the same index on the host: one thread per
                 physical core of the affinity mask within the job's CPU share,
@@ -56,6 +58,19 @@ import numpy as np  # noqa: E402
 METRIC = ("index build s + achieved HBM GB/s; batched queries/s with p50/p99, "
           "bit-exact")
 HBM_PEAK_GBS = 8000.0
+
+
+# The process's CPU mask at start.  With OMP_PROC_BIND set, the OpenMP runtime
+# binds the thread that loads it to the first place (one core) when the
+# oracle library loads; threads started later from that thread (the serving
+# loop, its build thread) would inherit that one core.
+START_CPUS = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+
+
+def unbind_main_thread():
+    """Give the calling thread the process's starting CPU mask again."""
+    if START_CPUS is not None:
+        os.sched_setaffinity(0, START_CPUS)
 
 
 def hip_sync():
@@ -218,7 +233,8 @@ def main():
     build_ms = float(np.median([s["build_ms"] for s in stats]))
     value = a.nrows * args.steps / elapsed
 
-    lane_kernel = "k_slot_lane" if stats[-1]["layout"] == 0 else "k_spgemm_lane"
+    # one "launch" is one product's lane tier (two kernels in chain form)
+    lane_kernel = "slot_lane_tier" if stats[-1]["layout"] == 0 else "k_spgemm_lane"
     cnt = prof["spgemm_lane"]
     mean_ms = cnt["total_ms"] / max(1, cnt["launches"])
     alg_per_launch = cnt["alg_bytes"] / max(1, cnt["launches"])
@@ -228,6 +244,8 @@ def main():
     roofline = {
         "bound": "hbm",
         "kernel": lane_kernel,
+        "kernels": (["k_slot_fast", "k_slot_list", "k_slot_lane", "k_slot_lane_narrow"]
+                    if lane_kernel == "slot_lane_tier" else ["k_spgemm_lane"]),
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -301,6 +319,7 @@ def main():
         print(f"# cpu oracle: median {cpu_s:.2f}s on {th} threads, 1 thread {one_s:.2f}s, "
               f"bit_exact={bit_exact}, A^{args.dlo} bit_exact={power_exact}",
               file=sys.stderr, flush=True)
+        unbind_main_thread()
 
     out = {
         "metric": METRIC,
@@ -567,6 +586,9 @@ def leg_serving(chrom, idx2, args):
         comp = dv.generate_graph(2, backbone_bp=int(lens[b]), seed=0xD1E55000 + b, size_bytes=8)
         builds.append((comp, 200, 1000, float((b + 0.5) * args.serve_seconds * 1e6 / nb)))
     log(f"serving: {n} queries over {args.serve_seconds}s, builds of {lens.tolist()} bp")
+    # the server's threads start from this one: all of the process's CPUs
+    unbind_main_thread()
+    host_cpus = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
     with dv.Server() as srv:
         ids = [srv.add_index(x) for x in indexes]
         res = srv.replay(qidx, qi, qj, arrival, builds)
@@ -622,6 +644,7 @@ def leg_serving(chrom, idx2, args):
         "p50_us": round(float(np.percentile(lat, 50)), 2),
         "p99_us": round(float(np.percentile(lat, 99)), 2),
         "max_us": round(float(lat.max()), 2),
+        "host_cpus_of_server_threads": host_cpus,
         "builds_bp": lens.tolist(),
         "build_completion_s": [round(x / 1e6, 4) for x in res["build_done_us"].tolist()],
         "answers_bit_exact_vs_cpu_lookup": bool(ok),
