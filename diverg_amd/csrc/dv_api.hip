@@ -410,7 +410,6 @@ int dv_finalize(dv_context *c) {
   if (c->h_plan_ctr) (void)hipHostFree(c->h_plan_ctr);
   if (c->d_plan_ctr) (void)hipFree(c->d_plan_ctr);
   if (c->slot_parts) (void)hipFree(c->slot_parts);
-  if (c->slot_defer) (void)hipFree(c->slot_defer);
   if (c->h_flag) (void)hipHostFree(c->h_flag);
   if (c->d_ctr) (void)hipFree(c->d_ctr);
   if (c->d_flag) (void)hipFree(c->d_flag);

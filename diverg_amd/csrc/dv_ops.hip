@@ -31,8 +31,8 @@ int Ctx::identity_dev(uint64_t n, uint32_t sb, DevMat &out) {
 bool Ctx::op_scratch(uint64_t n, OpScratch &s) {
   const uint64_t n1 = n ? n : 1;
   const uint32_t KS = tuning.stage_entries;
-  // the lane-tier grid is at most 8 blocks of 256 threads per CU (16 for
-  // the two launches of the chain form)
+  // the lane-tier grid is at most the resident blocks of 256 threads, at
+  // most 8 per CU; parts for twice that
   const uint32_t parts_cap = 16 * std::max<uint32_t>(1, num_cu);
   void *buf[8];
   const uint64_t sz[8] = {4 * n1, 4ull * KS * n1, 4 * n1, 4 * n1, 4 * n1, 8 * scan_words(n),

@@ -156,9 +156,6 @@ struct Ctx {
   // operation, and the predicted side-array entries per row
   unsigned long long *slot_parts = nullptr;
   uint64_t slot_parts_words = 0;
-  // rows the chain form defers to its list pass (per-block lists, counts)
-  uint32_t *slot_defer = nullptr;
-  uint64_t slot_defer_words = 0;
   double slot_side_factor = 1.0;
   // build shapes (rows, entries, nnz, size type, d1, d2) whose slot-form
   // products had many rows of several ranges: built in range CRS from then on
@@ -268,7 +265,6 @@ struct Ctx {
   // dv_slots.hip
   int dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d);
   bool slot_parts_reserve(uint64_t words);
-  bool slot_defer_reserve(uint64_t words);
   int autotune_dindex(const DevMat &a, uint32_t dlo, uint32_t dup, dv_tuning &best);
 
   // dv_graph.hip

@@ -21,19 +21,6 @@ namespace {
 SlotView view_of(const PlanSlot &s) { return SlotView{s.rowmap, s.entries}; }
 }  // namespace
 
-bool Ctx::slot_defer_reserve(uint64_t words) {
-  if (words <= slot_defer_words) return true;
-  if (slot_defer) (void)hipFree(slot_defer);
-  slot_defer = nullptr;
-  slot_defer_words = 0;
-  if (hipMalloc((void **)&slot_defer, 4 * words) != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
-  }
-  slot_defer_words = words;
-  return true;
-}
-
 bool Ctx::slot_parts_reserve(uint64_t words) {
   if (words <= slot_parts_words) return true;
   if (slot_parts) (void)hipFree(slot_parts);
@@ -76,10 +63,6 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   if (!slot_parts_reserve(parts_words + ctr_words))
     return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
   unsigned long long *side_ctr = slot_parts + parts_words;
-  // deferred-row lists of the chain form (per block: its share of the rows)
-  const uint64_t defer_list_words = nmax + (uint64_t)per_op * 256 + 256;
-  if (!slot_defer_reserve(defer_list_words + per_op))
-    return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
 
   // buffers: slot array (8 bytes per row) in `rowmap`, side array in `entries`
   auto release_all = [&]() {
@@ -209,17 +192,16 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     const uint32_t batch = A.span <= 2 ? 2u : A.span <= 4 ? 4u : 8u;
     const uint64_t resident =
         (uint64_t)num_cu * slot_lane_blocks_per_cu(summ_rows != 0, batch, chains);
-    // the chain form's two launches write parts for twice the grid
-    const uint32_t grid = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>({(n + 255) / 256, resident, chains ? per_op / 2 : per_op}));
+    const uint32_t grid =
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(n + 255) / 256, resident, per_op}));
     unsigned long long *op_side = side_ctr + (uint64_t)SLOT_SIDE_PARTS * i;
     slots_lane(stream, view_of(A), view_of(B), n, B.nrows, tuning.lane_slice_max,
                tuning.small_slice_max, grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big,
                ctr, d_status, summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows,
                o.plus_i ? 1u : 0u, slot_parts + 3ull * per_op * i, batch,
                chains ? slot_chains(B.rowmap, B.nrows) : ChainView{nullptr, nullptr},
-               ChainView{nullptr, nullptr}, slot_defer, slot_defer + defer_list_words);
-    if (!used.all) used.n[i] = n ? (chains ? 2 * grid : grid) : 0u;
+               ChainView{nullptr, nullptr});
+    if (!used.all) used.n[i] = n ? grid : 0u;
     if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
     const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);
     slots_groups(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.grid, s.mid, &ctr->n_mid,

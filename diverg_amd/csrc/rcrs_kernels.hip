@@ -274,12 +274,13 @@ __device__ __forceinline__ void fold(uint32_t vs, uint32_t ve, uint32_t &cl, uin
 // every range touching the running interval as it grows (a banded slice), is
 // done without branches; a lane with any other range redoes the batch from
 // the same state through fold(), so the result is fold()'s either way.
-__device__ __forceinline__ void fold_batch(const uint2 (&v)[LANE_BATCH], uint32_t &cl,
-                                           uint32_t &ch, Runs &r, bool &ok) {
+template <int N>
+__device__ __forceinline__ void fold_batch(const uint2 (&v)[N], uint32_t &cl, uint32_t &ch,
+                                           Runs &r, bool &ok) {
   uint32_t ncl = cl, nch = ch;
   bool all = true;
 #pragma unroll
-  for (int k = 0; k < LANE_BATCH; ++k) {
+  for (int k = 0; k < N; ++k) {
     const bool valid = v[k].x <= v[k].y;
     // an empty running interval takes the range as it is
     const bool touch = ncl > nch || (v[k].x <= nch + 1u && v[k].y + 1u >= ncl);
@@ -292,7 +293,7 @@ __device__ __forceinline__ void fold_batch(const uint2 (&v)[LANE_BATCH], uint32_
     ch = nch;
   } else {
 #pragma unroll
-    for (int k = 0; k < LANE_BATCH; ++k) fold(v[k].x, v[k].y, cl, ch, r, ok);
+    for (int k = 0; k < N; ++k) fold(v[k].x, v[k].y, cl, ch, r, ok);
   }
 }
 
@@ -1913,7 +1914,7 @@ __global__ __launch_bounds__(BLOCK) void k_slot_chains(const uint2 *__restrict__
 // the group lists as in k_spgemm_lane.  A two-stage pipeline loads the A
 // slot of the row after next while this row folds.  B slices are read slot
 // by slot (NB at a time), through interval summaries (SUMM) or through B's
-// chains (CHAIN); with c_seg the kernel writes C's chains.
+// chains (k_slot_fast); with c_seg the kernel writes C's chains.
 #define DV_SLOT_LANE_PARAMS                                                                \
   Slots A, Slots B, uint64_t nrows, uint32_t lane_max, uint32_t mid_max,                   \
       uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side, SideAlloc sa,                \
@@ -1922,159 +1923,167 @@ __global__ __launch_bounds__(BLOCK) void k_slot_chains(const uint2 *__restrict__
       const uint2 *__restrict__ pre, const uint2 *__restrict__ suf,                        \
       const uint4 *__restrict__ pre2, const uint4 *__restrict__ suf2, uint32_t summ_min,   \
       uint32_t summ_shift, uint32_t plus_i, unsigned long long *__restrict__ parts,        \
-      Chains BC, uint8_t *__restrict__ c_seg, uint2 *__restrict__ c_chunk,                 \
-      uint32_t *__restrict__ dlist, uint32_t *__restrict__ dcount, uint32_t dcap
+      Chains BC, uint8_t *__restrict__ c_seg, uint2 *__restrict__ c_chunk
 #define DV_SLOT_LANE_ARGS                                                                  \
   A, B, nrows, lane_max, mid_max, c_slot, c_side, sa, mid_list, big_list, ctr, status, pre, \
-      suf, pre2, suf2, summ_min, summ_shift, plus_i, parts, BC, c_seg, c_chunk, dlist,      \
-      dcount, dcap
+      suf, pre2, suf2, summ_min, summ_shift, plus_i, parts, BC, c_seg, c_chunk
 
 enum SlotMode { SLOT_PLAIN = 0, SLOT_SUMM = 1 };
 
-// NB: slice loads per round trip (LANE_BATCH, or 2 / 4 through
-// fold_slots_tight for products of narrow slices) in SLOT_PLAIN mode.
-// LIST: the rows are those k_slot_fast deferred, dcount[b] of them at
-// dlist[b * dcap ..] for block b (the grid must be k_slot_fast's); the
-// parts go to parts[3 (gridDim.x + b) ..].
-template <int MODE, int NB, bool LIST = false>
-__device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
+// The row folded into runs r and running interval [cl, ch] (ok false: more
+// than LANE_SLOTS runs, or a slice past lane_max): written, or handed to the
+// group lists.  Every lane of the wave calls it; `live` marks the lanes with
+// a row.
+template <bool CHAIN_OUT>
+__device__ __forceinline__ void slot_row_emit(DV_SLOT_LANE_PARAMS, bool live, uint64_t row,
+                                              uint64_t len, bool ok, Runs &r, uint32_t cl,
+                                              uint32_t ch, unsigned long long &nnz_acc,
+                                              unsigned long long &ranges_acc) {
+  int cls = 0;
+  uint32_t nruns = 0;
+  // live slots form a prefix: with slot 0 empty the row is [cl, ch] alone
+  const bool single = ok && r.lo[0] > r.hi[0];
+  if (live && single) {
+    if (cl <= ch) {
+      r.lo[0] = cl;
+      r.hi[0] = ch;
+      nnz_acc += (unsigned long long)(ch - cl) + 1;
+      nruns = 1;
+    }
+    ranges_acc += nruns;
+  } else if (live) {
+    if (ok && cl <= ch && !insert(r, cl, ch)) {
+      compact(r);
+      ok = insert(r, cl, ch);
+    }
+    if (!ok) {
+      cls = len <= mid_max ? 1 : 2;
+    } else {
+      compact(r);
+#pragma unroll
+      for (int t = 0; t < LANE_SLOTS; ++t)
+        if (r.lo[t] <= r.hi[t]) {
+          nnz_acc += (unsigned long long)(r.hi[t] - r.lo[t]) + 1;
+          ++nruns;
+        }
+      ranges_acc += nruns;
+    }
+  }
+  const uint32_t need_side = live && ok && nruns > 1 ? nruns + 1 : 0u;
+  const uint32_t part = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) & (SLOT_SIDE_PARTS - 1);
+  const unsigned long long loc = wave_alloc(need_side, &sa.ctr[part]);
+  if (live && ok) {
+    if (need_side && loc + need_side > sa.pcap) {
+      atomicOr(status, 1u);  // side partition full: the build falls back
+    } else {
+      slot_store_runs(r, nruns, row, c_slot, c_side, part * sa.pcap + loc);
+    }
+  }
+  wave_append(cls == 1, (uint32_t)row, mid_list, &ctr->n_mid);
+  wave_append(cls == 2, (uint32_t)row, big_list, &ctr->n_big);
+  if (CHAIN_OUT && c_seg)
+    chain_write(live && ok && nruns == 1, r.lo[0], r.hi[0], live, row, c_seg, c_chunk);
+}
+
+// One row of C = A.B (or A (B + I)) by one lane, its A slot `a` given: the
+// B slices are read slot by slot (NB at a time: LANE_BATCH, or 2 / 4 through
+// fold_slots_tight for products of narrow slices), or through interval
+// summaries (SUMM); a row of one or a few runs is written, a row of more
+// runs or a long slice goes to the group lists.  Every lane of the wave
+// calls it (wave-wide reservations); `live` marks the lanes with a row.
+// With CHAIN_OUT the row's chain data is written too.
+template <int MODE, int NB, bool CHAIN_OUT>
+__device__ __forceinline__ void slot_lane_row(DV_SLOT_LANE_PARAMS, bool live, uint64_t row,
+                                              uint2 a, unsigned long long &nnz_acc,
+                                              unsigned long long &slice_acc,
+                                              unsigned long long &ranges_acc) {
   constexpr bool SUMM = MODE == SLOT_SUMM;
+  uint64_t len = 0;
+  bool ok = true;
+  Runs r;
+#pragma unroll
+  for (int t = 0; t < LANE_SLOTS; ++t) {
+    r.lo[t] = 0xFFFFFFFFu;
+    r.hi[t] = 0u;
+  }
+  uint32_t cl = 0xFFFFFFFFu, ch = 0u;
+  // A ranges: the slot itself, or the side entries of a marked row
+  // (the reads of a marked row's ranges are waited on where they are
+  // made, so the common row does not wait for the caller's prefetches)
+  uint32_t na = 0;
+  if (live) {
+    if (a.x <= a.y) {
+      na = 1;
+    } else if (a.x == SLOT_MARK) {
+      na = A.side[a.y].x;
+      need(na);
+    }
+  }
+  for (uint32_t q = 0; q < na; ++q) {
+    uint2 ar = a;
+    if (a.x > a.y) {
+      ar = A.side[a.y + 1 + q];
+      need(ar.x);
+      need(ar.y);
+    }
+    const uint32_t s = ar.x, e = ar.y;
+    len += (uint64_t)(e - s) + 1;
+    if (len > lane_max) ok = false;
+    if (!ok) continue;
+    if (plus_i) fold(s, e, cl, ch, r, ok);  // C = A (B + I)
+    if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min &&
+        (e >> summ_shift) - (s >> summ_shift) < 64) {
+      fold_summarised(
+          pre, suf, pre2, suf2, s, e, summ_shift,
+          [&](uint32_t r0, uint32_t r1) { fold_slots(B, r0, r1 - 1, cl, ch, r, ok, slice_acc); },
+          cl, ch, r, ok, slice_acc);
+    } else if constexpr (NB < LANE_BATCH) {
+      fold_slots_tight<NB>(B, s, e, cl, ch, r, ok, slice_acc);
+    } else {
+      fold_slots(B, s, e, cl, ch, r, ok, slice_acc);
+    }
+  }
+  slot_row_emit<CHAIN_OUT>(DV_SLOT_LANE_ARGS, live, row, len, ok, r, cl, ch, nnz_acc,
+                           ranges_acc);
+}
+
+// Lane tier over the rows of A, one lane per row, persistent grid-stride
+// loop; a two-stage pipeline loads the A slot of the row after next while
+// this row folds.  Per-block (nnz, slice, ranges) sums go to
+// parts[3 blockIdx.x ..].
+template <int MODE, int NB>
+__device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
   if (*status) return;
   unsigned long long nnz_acc = 0, slice_acc = 0, ranges_acc = 0;
-  const uint64_t stride = LIST ? BLOCK : (uint64_t)gridDim.x * BLOCK;
-  const uint64_t limit = LIST ? dcount[blockIdx.x] : nrows;
-  const uint32_t *lrows = LIST ? dlist + (uint64_t)blockIdx.x * dcap : nullptr;
-  auto row_of = [&](uint64_t i) -> uint64_t { return LIST ? (uint64_t)lrows[i] : i; };
-  const uint64_t first = LIST ? 0 : (uint64_t)blockIdx.x * BLOCK;
-  uint64_t idx = first + threadIdx.x, row = 0, c_row = LIST ? 0 : idx, n_row = LIST ? 0 : idx + stride;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
   uint2 c_a = make_uint2(SLOT_EMPTY, 0u), n_a = make_uint2(SLOT_EMPTY, 0u);
-  if (idx < limit) {
-    c_row = row_of(idx);
-    c_a = A.slot[c_row];
-  }
-  if (idx + stride < limit) {
-    n_row = row_of(idx + stride);
-    n_a = A.slot[n_row];
-  }
+  if (row < nrows) c_a = A.slot[row];
+  if (row + stride < nrows) n_a = A.slot[row + stride];
   need(c_a.x);
   need(c_a.y);
   need(n_a.x);
   need(n_a.y);
-  for (uint64_t base = first; base < limit; base += stride) {
-    idx = base + threadIdx.x;
-    row = c_row;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows; base += stride) {
+    row = base + threadIdx.x;
     const uint2 a = c_a;
     c_a = n_a;
-    c_row = n_row;
     n_a = make_uint2(SLOT_EMPTY, 0u);
-    n_row = LIST ? 0 : idx + 2 * stride;
-    if (idx + 2 * stride < limit) {
-      n_row = row_of(idx + 2 * stride);
-      n_a = A.slot[n_row];
-    }
-    int cls = 0;
-    const bool live = idx < limit;
-    uint64_t len = 0;
-    bool ok = true;
-    Runs r;
-#pragma unroll
-    for (int t = 0; t < LANE_SLOTS; ++t) {
-      r.lo[t] = 0xFFFFFFFFu;
-      r.hi[t] = 0u;
-    }
-    uint32_t cl = 0xFFFFFFFFu, ch = 0u;
-    // A ranges: the slot itself, or the side entries of a marked row
-    // (the reads of a marked row's ranges are waited on where they are
-    // made, so the common row does not wait for the prefetch above)
-    uint32_t na = 0;
-    if (live) {
-      if (a.x <= a.y) {
-        na = 1;
-      } else if (a.x == SLOT_MARK) {
-        na = A.side[a.y].x;
-        need(na);
-      }
-    }
-    for (uint32_t q = 0; q < na; ++q) {
-      uint2 ar = a;
-      if (a.x > a.y) {
-        ar = A.side[a.y + 1 + q];
-        need(ar.x);
-        need(ar.y);
-      }
-      const uint32_t s = ar.x, e = ar.y;
-      len += (uint64_t)(e - s) + 1;
-      if (len > lane_max) ok = false;
-      if (!ok) continue;
-      if (plus_i) fold(s, e, cl, ch, r, ok);  // C = A (B + I)
-      if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min &&
-          (e >> summ_shift) - (s >> summ_shift) < 64) {
-        fold_summarised(
-            pre, suf, pre2, suf2, s, e, summ_shift,
-            [&](uint32_t r0, uint32_t r1) { fold_slots(B, r0, r1 - 1, cl, ch, r, ok, slice_acc); },
-            cl, ch, r, ok, slice_acc);
-      } else if constexpr (NB < LANE_BATCH) {
-        fold_slots_tight<NB>(B, s, e, cl, ch, r, ok, slice_acc);
-      } else {
-        fold_slots(B, s, e, cl, ch, r, ok, slice_acc);
-      }
-    }
+    if (row + 2 * stride < nrows) n_a = A.slot[row + 2 * stride];
+    slot_lane_row<MODE, NB, true>(DV_SLOT_LANE_ARGS, row < nrows, row, a, nnz_acc, slice_acc,
+                                  ranges_acc);
     need(c_a.x);
     need(c_a.y);
     need(n_a.x);
     need(n_a.y);
-    uint32_t nruns = 0;
-    // live slots form a prefix: with slot 0 empty the row is [cl, ch] alone
-    const bool single = ok && r.lo[0] > r.hi[0];
-    if (live && single) {
-      if (cl <= ch) {
-        r.lo[0] = cl;
-        r.hi[0] = ch;
-        nnz_acc += (unsigned long long)(ch - cl) + 1;
-        nruns = 1;
-      }
-      ranges_acc += nruns;
-    } else if (live) {
-      if (ok && cl <= ch && !insert(r, cl, ch)) {
-        compact(r);
-        ok = insert(r, cl, ch);
-      }
-      if (!ok) {
-        cls = len <= mid_max ? 1 : 2;
-      } else {
-        compact(r);
-#pragma unroll
-        for (int t = 0; t < LANE_SLOTS; ++t)
-          if (r.lo[t] <= r.hi[t]) {
-            nnz_acc += (unsigned long long)(r.hi[t] - r.lo[t]) + 1;
-            ++nruns;
-          }
-        ranges_acc += nruns;
-      }
-    }
-    const uint32_t need_side = live && ok && nruns > 1 ? nruns + 1 : 0u;
-    const uint32_t part = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) & (SLOT_SIDE_PARTS - 1);
-    const unsigned long long loc = wave_alloc(need_side, &sa.ctr[part]);
-    if (live && ok) {
-      if (need_side && loc + need_side > sa.pcap) {
-        atomicOr(status, 1u);  // side partition full: the build falls back
-      } else {
-        slot_store_runs(r, nruns, row, c_slot, c_side, part * sa.pcap + loc);
-      }
-    }
-    wave_append(cls == 1, (uint32_t)row, mid_list, &ctr->n_mid);
-    wave_append(cls == 2, (uint32_t)row, big_list, &ctr->n_big);
-    if (!LIST && c_seg)
-      chain_write(live && ok && nruns == 1, r.lo[0], r.hi[0], live, row, c_seg, c_chunk);
   }
   nnz_acc = block_sum_u64(nnz_acc);
   slice_acc = block_sum_u64(slice_acc);
   ranges_acc = block_sum_u64(ranges_acc);
   if (threadIdx.x == 0) {
-    const uint64_t b = LIST ? gridDim.x + blockIdx.x : blockIdx.x;
-    parts[3 * b] = nnz_acc;
-    parts[3 * b + 1] = slice_acc;
-    parts[3 * b + 2] = ranges_acc;
+    parts[3 * blockIdx.x] = nnz_acc;
+    parts[3 * blockIdx.x + 1] = slice_acc;
+    parts[3 * blockIdx.x + 2] = ranges_acc;
   }
 }
 
@@ -2092,23 +2101,29 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
 }
 
 
-// Phase 1 of a product over B's chains: every row of the block's share whose
-// A row is one range [s, e] covering at most FAST_CHUNKS + 1 chunks of B and
-// whose slice is one chain per part (the head s .. end of s's chunk, whole
-// chunks, the tail start of e's chunk .. e), the parts touching, is one
-// interval of C, found from a fixed set of loads issued together and a few
-// min/max operations, with no branch that differs between lanes.  The
-// block's other rows go to its list, dlist[b * dcap ..] (dcount[b] of them;
-// dcap covers the block's share), for the same grid's list pass
-// (slot_lane_body<.., LIST>).
+// A product over B's chains: every row whose A row is one range [s, e]
+// covering at most FAST_CHUNKS + 1 chunks of B and whose slice is one chain
+// per part (the head s .. end of s's chunk, whole chunks, the tail start of
+// e's chunk .. e), the parts touching, is one interval of C, found from a
+// fixed set of loads issued together and a few min/max operations, with no
+// branch that differs between lanes.  The wave queues its other rows (with
+// their A slots) in LDS and, whenever 64 are queued, folds them slot by slot
+// one per lane (slot_lane_row), so their round trips overlap the other
+// waves' rows; what is left is folded after the loop.  Rows near an indel
+// or a multi-range row defer together (1-3% of the rows of a power of a
+// banded graph).
 constexpr uint32_t FAST_CHUNKS = 4;
+constexpr uint32_t DEFER_RING = 128;  // per wave: < 64 left + one row's worth of 64
 
 __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
-  __shared__ uint32_t qn;
-  if (*status) return;
-  if (threadIdx.x == 0) qn = 0;
+  __shared__ uint4 ring[BLOCK / 64][DEFER_RING];
+  // the whole block returns or none of it (the barriers of the sums)
+  __shared__ unsigned int st0;
+  if (threadIdx.x == 0) st0 = *status;
   __syncthreads();
-  uint32_t *lrows = dlist + (uint64_t)blockIdx.x * dcap;
+  if (st0) return;
+  uint4 *q = ring[threadIdx.x >> 6];
+  uint32_t q_head = 0, q_tail = 0;  // wave-uniform
   unsigned long long nnz_acc = 0, slice_acc = 0, ranges_acc = 0;
   const uint32_t lane = __lane_id();
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -2116,6 +2131,17 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
   uint2 c_a = make_uint2(SLOT_EMPTY, 0u), n_a = make_uint2(SLOT_EMPTY, 0u);
   if (row < nrows) c_a = A.slot[row];
   if (row + stride < nrows) n_a = A.slot[row + stride];
+  // fold `count` (at most 64) queued rows from q_head, one per lane
+  auto fold_queued = [&](uint32_t count) {
+    __builtin_amdgcn_wave_barrier();
+    const bool on = lane < count;
+    const uint4 d = q[(q_head + lane) & (DEFER_RING - 1)];
+    slot_lane_row<SLOT_PLAIN, LANE_BATCH, false>(DV_SLOT_LANE_ARGS, on, on ? d.x : 0u,
+                                                 make_uint2(d.y, d.z), nnz_acc, slice_acc,
+                                                 ranges_acc);
+    q_head += count;
+    __builtin_amdgcn_wave_barrier();
+  };
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < nrows; base += stride) {
     row = base + threadIdx.x;
     const bool live = row < nrows;
@@ -2135,8 +2161,8 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
     const uint32_t t0 = c0 * CHAIN_ROWS + BC.seg[s], u0 = c1 * CHAIN_ROWS + BC.seg[s1];
     uint2 m[FAST_CHUNKS];
 #pragma unroll
-    for (int q = 0; q < (int)FAST_CHUNKS; ++q)
-      m[q] = BC.chunk[nmid ? c0 + 1 + min((uint32_t)q, nmid - 1) : c0];
+    for (int k = 0; k < (int)FAST_CHUNKS; ++k)
+      m[k] = BC.chunk[nmid ? c0 + 1 + min((uint32_t)k, nmid - 1) : c0];
     // a part is one chain: several rows chained to its end, or one single row
     const bool head_ok = e0 > s ? t0 >= e0 : v0.x <= v0.y;
     const bool tail_ok = !cross || (e > s1 ? u0 >= e : w0.x <= w0.y);
@@ -2150,10 +2176,10 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
     };
     if (plus_i) join(v0.x, v1.y);
 #pragma unroll
-    for (int q = 0; q < (int)FAST_CHUNKS; ++q)
-      if ((uint32_t)q < nmid) {
-        fast = fast && m[q].x <= m[q].y;
-        join(m[q].x, m[q].y);
+    for (int k = 0; k < (int)FAST_CHUNKS; ++k)
+      if ((uint32_t)k < nmid) {
+        fast = fast && m[k].x <= m[k].y;
+        join(m[k].x, m[k].y);
       }
     if (cross) join(w0.x, w1.y);
     const bool empty_a = live && a.x == SLOT_EMPTY;
@@ -2165,17 +2191,18 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
     } else if (empty_a) {
       c_slot[row] = make_uint2(SLOT_EMPTY, 0u);
     }
-    // the rest to the block's list (one LDS reservation per wave)
+    // the rest to the wave's queue, folded 64 at a time
     const bool defer = live && !fast && !empty_a;
     const unsigned long long dm = __ballot(defer);
     if (dm) {
-      const uint32_t leader = (uint32_t)__ffsll((long long)dm) - 1;
-      uint32_t qb = 0;
-      if (lane == leader) qb = atomicAdd(&qn, (uint32_t)__popcll(dm));
-      qb = (uint32_t)__shfl((int)qb, (int)leader, 64);
-      if (defer) lrows[qb + (uint32_t)__popcll(dm & ((1ull << lane) - 1))] = (uint32_t)row;
+      if (defer)
+        q[(q_tail + (uint32_t)__popcll(dm & ((1ull << lane) - 1))) & (DEFER_RING - 1)] =
+            make_uint4((uint32_t)row, a.x, a.y, 0u);
+      q_tail += (uint32_t)__popcll(dm);
+      if (q_tail - q_head >= 64) fold_queued(64);
     }
   }
+  if (q_tail != q_head) fold_queued(q_tail - q_head);
   nnz_acc = block_sum_u64(nnz_acc);
   slice_acc = block_sum_u64(slice_acc);
   ranges_acc = block_sum_u64(ranges_acc);
@@ -2183,14 +2210,7 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
     parts[3 * blockIdx.x] = nnz_acc;
     parts[3 * blockIdx.x + 1] = slice_acc;
     parts[3 * blockIdx.x + 2] = ranges_acc;
-    dcount[blockIdx.x] = qn;
   }
-}
-
-// Phase 2: the deferred rows, slot by slot (NB loads per round trip).
-template <int NB>
-__global__ __launch_bounds__(BLOCK) void k_slot_list(DV_SLOT_LANE_PARAMS) {
-  slot_lane_body<SLOT_PLAIN, NB, true>(DV_SLOT_LANE_ARGS);
 }
 
 // Group tiers on slots: one row of C by G lanes with an LDS bit window, as
@@ -3026,10 +3046,9 @@ void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t 
                 uint64_t side_cap, unsigned long long *side_ctr, uint32_t *mid_list,
                 uint32_t *big_list, OpCounters *ctr, unsigned int *status, const void *summ,
                 uint32_t summ_min, uint32_t summ_rows, uint32_t plus_i,
-                unsigned long long *parts, uint32_t batch, ChainView b_chains, ChainView c_chains,
-                uint32_t *dlist, uint32_t *dcount) {
+                unsigned long long *parts, uint32_t batch, ChainView b_chains,
+                ChainView c_chains) {
   if (!nrows) return;
-  const uint32_t dcap = (uint32_t)(((nrows + (uint64_t)grid * BLOCK - 1) / ((uint64_t)grid * BLOCK)) * BLOCK);
   const Chains bc{static_cast<const uint8_t *>(b_chains.seg),
                   static_cast<const uint2 *>(b_chains.chunk)};
   uint8_t *c_seg = static_cast<uint8_t *>(const_cast<void *>(c_chains.seg));
@@ -3044,17 +3063,9 @@ void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t 
   hipLaunchKernelGGL(K_, dim3(grid), dim3(BLOCK), 0, s, slots_of(a), slots_of(b), nrows,     \
                      lane_max, mid_max, static_cast<uint2 *>(c_slot),                        \
                      static_cast<uint2 *>(c_side), sa, mid_list, big_list, ctr, status, pre, \
-                     suf, pre2, suf2, summ_min, sh, plus_i, parts, bc, c_seg, c_chunk, dlist,  \
-                     dcount, dcap)
-  if (bc.seg) {
-    // phase 1, then the deferred rows on the same grid (no chains written)
-    DV_SLANE(k_slot_fast);
-    c_seg = nullptr;
-    c_chunk = nullptr;
-    if (batch <= 2) DV_SLANE(k_slot_list<2>);
-    else if (batch <= 4) DV_SLANE(k_slot_list<4>);
-    else DV_SLANE(k_slot_list<8>);
-  } else if (pre) DV_SLANE(k_slot_lane<true>);
+                     suf, pre2, suf2, summ_min, sh, plus_i, parts, bc, c_seg, c_chunk)
+  if (bc.seg) DV_SLANE(k_slot_fast);
+  else if (pre) DV_SLANE(k_slot_lane<true>);
   else if (batch <= 2) DV_SLANE(k_slot_lane_narrow<2>);
   else if (batch <= 4) DV_SLANE(k_slot_lane_narrow<4>);
   else DV_SLANE(k_slot_lane<false>);

@@ -224,17 +224,15 @@ void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
 // the widest A range (narrow products then run fewer instructions).  With
 // b_chains.seg set, B slices are read through B's chains (no summaries);
 // with c_chains.seg set, the kernel writes C's chains (the group tiers' rows
-// end chains).  The chain form runs two launches on `grid` blocks: the rows
-// whose slice is a few chains, then the others from per-block lists in dlist
-// (at least nrows + grid * 256 words) counted in dcount (grid words); parts
-// then needs 6 words per block.
+// end chains).  In the chain form the rows whose slice is not a few chains
+// are folded slot by slot within the same launch.
 void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t b_nrows,
                 uint32_t lane_max, uint32_t mid_max, uint32_t grid, void *c_slot, void *c_side,
                 uint64_t side_cap, unsigned long long *side_ctr, uint32_t *mid_list,
                 uint32_t *big_list, OpCounters *ctr, unsigned int *status, const void *summ,
                 uint32_t summ_min, uint32_t summ_rows, uint32_t plus_i,
                 unsigned long long *parts, uint32_t batch, ChainView b_chains,
-                ChainView c_chains, uint32_t *dlist, uint32_t *dcount);
+                ChainView c_chains);
 // Both group tiers of a product in one launch: rows1 (*n1 of them) by groups
 // of lanes1 lanes (LDS window lds1 words) on grid1 blocks, rows2 by 64-lane
 // groups (lds2) on grid2 blocks; grids multiples of 8.
@@ -263,6 +261,8 @@ void slots_counts(hipStream_t s, SlotView a, uint64_t n, uint32_t *cnt,
                   const unsigned int *status);
 void slots_emit(hipStream_t s, SlotView a, uint64_t n, uint32_t sb, const void *rowmap,
                 uint32_t *entries);
+// Resident blocks per CU of the lane kernel that slots_lane runs for these
+// arguments.
 uint32_t slot_lane_blocks_per_cu(bool summaries, uint32_t batch, bool chains);
 
 }  // namespace dv
