@@ -30,10 +30,10 @@ Further legs (fields of the same JSON line; --legs selects them):
 
 The JSON line also carries:
   roofline      the SpGEMM lane tier, the dominant work of a build: per
-                product, its lane-per-row launches (slot form: k_slot_fast
-                then k_slot_list for products read through chains, else
-                k_slot_lane or k_slot_lane_narrow; range CRS: k_spgemm_lane),
-                timed together with HIP events on the library stream;
+                product, its lane-per-row launch (slot form: k_slot_fast for
+                products read through chains, else k_slot_lane or
+                k_slot_lane_narrow; range CRS: k_spgemm_lane), timed with HIP
+                events on the library stream;
                 algorithmic bytes bytes(X) + bytes(Y) + bytes(C) of the
                 product over that time, against 8 TB/s
   cpu_baseline  the CPU oracle (the reference's OpenMP algorithm restated in
@@ -233,7 +233,8 @@ def main():
     build_ms = float(np.median([s["build_ms"] for s in stats]))
     value = a.nrows * args.steps / elapsed
 
-    # one "launch" is one product's lane tier (two kernels in chain form)
+    # one "launch" is one product's lane tier (a kernel that depends on the
+    # product's form)
     lane_kernel = "slot_lane_tier" if stats[-1]["layout"] == 0 else "k_spgemm_lane"
     cnt = prof["spgemm_lane"]
     mean_ms = cnt["total_ms"] / max(1, cnt["launches"])
@@ -244,7 +245,7 @@ def main():
     roofline = {
         "bound": "hbm",
         "kernel": lane_kernel,
-        "kernels": (["k_slot_fast", "k_slot_list", "k_slot_lane", "k_slot_lane_narrow"]
+        "kernels": (["k_slot_fast", "k_slot_lane", "k_slot_lane_narrow"]
                     if lane_kernel == "slot_lane_tier" else ["k_spgemm_lane"]),
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
