@@ -195,20 +195,23 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     const uint32_t grid =
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(n + 255) / 256, resident, per_op}));
     unsigned long long *op_side = side_ctr + (uint64_t)SLOT_SIDE_PARTS * i;
-    slots_lane(stream, view_of(A), view_of(B), n, B.nrows, tuning.lane_slice_max,
-               tuning.small_slice_max, grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big,
-               ctr, d_status, summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows,
-               o.plus_i ? 1u : 0u, slot_parts + 3ull * per_op * i, batch,
-               chains ? slot_chains(B.rowmap, B.nrows) : ChainView{nullptr, nullptr},
-               ChainView{nullptr, nullptr});
+    // the chains of C, for the products that read it through them: written
+    // by the lane kernel, or by a pass once the group tiers have run
+    const bool want_chains = c.chains && chain_reader[o.c];
+    const bool chains_done = slots_lane(
+        stream, view_of(A), view_of(B), n, B.nrows, tuning.lane_slice_max, tuning.small_slice_max,
+        grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big, ctr, d_status,
+        summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows, o.plus_i ? 1u : 0u,
+        slot_parts + 3ull * per_op * i, batch,
+        chains ? slot_chains(B.rowmap, B.nrows) : ChainView{nullptr, nullptr},
+        want_chains ? slot_chains(c.rowmap, c.nrows) : ChainView{nullptr, nullptr});
     if (!used.all) used.n[i] = n ? grid : 0u;
     if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
     const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);
     slots_groups(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.grid, s.mid, &ctr->n_mid,
                  gb.lds_words, gb.grid, s.big, &ctr->n_big, gm.rows_per_group, c.rowmap, c.entries,
                  c.cap, op_side, ctr, d_status, o.plus_i ? 1u : 0u);
-    // the chains of C, for the products that read it through them
-    if (c.chains && chain_reader[o.c])
+    if (want_chains && !chains_done)
       slots_chains(stream, c.rowmap, c.nrows, slot_chains(c.rowmap, c.nrows), d_status);
   }
   slots_reduce_parts(stream, slot_parts, per_op, used, side_ctr, (uint32_t)nops, d_plan_ctr);

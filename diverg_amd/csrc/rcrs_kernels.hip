@@ -3041,14 +3041,14 @@ void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
 #undef DV_SSUMM
 }
 
-void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t b_nrows,
+bool slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t b_nrows,
                 uint32_t lane_max, uint32_t mid_max, uint32_t grid, void *c_slot, void *c_side,
                 uint64_t side_cap, unsigned long long *side_ctr, uint32_t *mid_list,
                 uint32_t *big_list, OpCounters *ctr, unsigned int *status, const void *summ,
                 uint32_t summ_min, uint32_t summ_rows, uint32_t plus_i,
                 unsigned long long *parts, uint32_t batch, ChainView b_chains,
                 ChainView c_chains) {
-  if (!nrows) return;
+  if (!nrows) return true;
   const Chains bc{static_cast<const uint8_t *>(b_chains.seg),
                   static_cast<const uint2 *>(b_chains.chunk)};
   uint8_t *c_seg = static_cast<uint8_t *>(const_cast<void *>(c_chains.seg));
@@ -3064,12 +3064,19 @@ void slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t 
                      lane_max, mid_max, static_cast<uint2 *>(c_slot),                        \
                      static_cast<uint2 *>(c_side), sa, mid_list, big_list, ctr, status, pre, \
                      suf, pre2, suf2, summ_min, sh, plus_i, parts, bc, c_seg, c_chunk)
+  // the chain form leaves C's chains to slots_chains (its deferred rows
+  // are written after their chunks)
+  if (bc.seg) {
+    c_seg = nullptr;
+    c_chunk = nullptr;
+  }
   if (bc.seg) DV_SLANE(k_slot_fast);
   else if (pre) DV_SLANE(k_slot_lane<true>);
   else if (batch <= 2) DV_SLANE(k_slot_lane_narrow<2>);
   else if (batch <= 4) DV_SLANE(k_slot_lane_narrow<4>);
   else DV_SLANE(k_slot_lane<false>);
 #undef DV_SLANE
+  return c_seg != nullptr;
 }
 
 void slots_groups(hipStream_t s, SlotView a, SlotView b, uint32_t lanes1, uint32_t lds1,
