@@ -16,6 +16,10 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <array>
+#include <functional>
+#include <map>
+#include <mutex>
 
 #include "dv_runtime.hpp"
 
@@ -73,36 +77,168 @@ int Plan::spgemm_plus_i(int a, int b) {
   return c;
 }
 
-// C = P^k by binary exponentiation (range_sparse.hpp:2376-2431).
-int Plan::power(int p, uint32_t k) {
-  const uint64_t n = slots[p].ncols;
-  const uint32_t sb = slots[p].sb;
-  if (k == 0) return identity(n, sb);
-  int c = -1;
-  for (;;) {
-    if (k & 1) {
-      if (c >= 0)
-        c = spgemm(c, p);
-      else if (slots[p].kind == P_SPGEMM)
-        c = p;  // I.P == P for a canonical SpGEMM output
-      else
-        c = spgemm(identity(n, sb), p);
+// An addition chain for a power P^k: exponents 1 = e[0] < e[1] < ... <
+// e[L] = k, step i forming e[i] = e[a[i]] + e[b[i]], i.e. the product of the
+// powers e[a[i]] and e[b[i]] (a[i] <= b[i]).  Powers of one matrix commute,
+// and Boolean products and canonical range rows are unique, so every chain
+// gives the same arrays.
+struct PowerChain {
+  std::vector<uint32_t> e{1u};
+  std::vector<uint16_t> a{0}, b{0};
+  double cost = 0;
+};
+
+// The binary method's chain (range_sparse.hpp:2376-2431: squarings, and the
+// accumulated product times the square at each further set bit).
+static PowerChain binary_chain(uint32_t k) {
+  PowerChain c;
+  int acc = -1, sq = 0;  // indices into c.e
+  for (uint32_t r = k;;) {
+    if (r & 1) {
+      if (acc < 0) {
+        acc = sq;
+      } else {
+        c.e.push_back(c.e[acc] + c.e[sq]);
+        c.a.push_back((uint16_t)acc);
+        c.b.push_back((uint16_t)sq);
+        acc = (int)c.e.size() - 1;
+      }
     }
-    k >>= 1;
-    if (k == 0) break;
-    p = spgemm(p, p);
+    r >>= 1;
+    if (r == 0) break;
+    c.e.push_back(2 * c.e[sq]);
+    c.a.push_back((uint16_t)sq);
+    c.b.push_back((uint16_t)sq);
+    sq = (int)c.e.size() - 1;
   }
   return c;
 }
 
-// SpGEMMs that power(p, k) plans: floor(log2 k) squarings, one product per
-// further set bit, and I.P first when bit 0 is set and P is not an SpGEMM
-// output.
-static uint32_t power_spgemms(uint32_t k, bool p_is_product) {
+// Estimated time (microseconds at the 5e6-locus graph) of the product
+// P^x . P^y (x <= y) when P's ranges span `span1` rows: the slice width is
+// the span of P^x, w = x (span1 - 1) + 1, and the lane kernel it selects
+// costs about 52 us for narrow slices (w <= 4), 40 + 3.1 (w - 1) us through
+// chains (the rows near breaks, which it defers, grow with w), 125 us with
+// interval summaries (the summary pass included), 60 us slot by slot.
+double Plan::product_cost(uint32_t x, uint64_t span1) const {
+  const uint64_t w = (uint64_t)x * (span1 ? span1 - 1 : 0) + 1;
+  if (w <= 4) return 52.0;
+  if (summary_span >= 2 && w >= summary_span) return 125.0;
+  if (chain_span && w >= chain_span) return 40.0 + 3.1 * (double)(w - 1);
+  return 60.0;
+}
+
+// The cheapest star chain (each step adds an earlier exponent to the last)
+// for P^k under product_cost, no longer than the binary method's: a
+// depth-first search, largest step first, bounded by the cheapest step times
+// the steps still needed (the exponent at most doubles per step), limited to
+// 20 000 nodes; the binary chain when nothing cheaper is found.  Chains are
+// cached per (k, span, thresholds), so a build shape searches once.
+PowerChain Plan::power_chain(uint32_t k, uint64_t span1) const {
+  PowerChain best = binary_chain(k);
+  for (size_t i = 1; i < best.e.size(); ++i)
+    best.cost += product_cost(std::min(best.e[best.a[i]], best.e[best.b[i]]), span1);
+  if (k < 3 || schedule != 0) return best;
+  static std::mutex mu;
+  static std::map<std::array<uint64_t, 4>, PowerChain> cache;
+  const std::array<uint64_t, 4> key{k, span1, chain_span, summary_span};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  const size_t max_len = best.e.size() - 1;
+  double minc = 1e30;
+  for (uint32_t x : {1u, 2u, 3u, 4u}) minc = std::min(minc, product_cost(x, span1));
+  minc = std::min(minc, 52.0);
+  PowerChain cur;
+  uint64_t nodes = 0;
+  std::function<void()> dfs = [&]() {
+    const uint32_t last = cur.e.back();
+    if (last == k) {
+      if (cur.cost < best.cost - 1e-9) best = cur;
+      return;
+    }
+    if (++nodes > 20000) return;
+    uint32_t need = 0;
+    for (uint64_t v = last; v < k; v <<= 1) ++need;
+    if (cur.e.size() - 1 + need > max_len || cur.cost + need * minc >= best.cost - 1e-9) return;
+    // steps last + e[j], largest first (distinct sums only)
+    const uint16_t li = (uint16_t)(cur.e.size() - 1);
+    for (int j = (int)cur.e.size() - 1; j >= 0; --j) {
+      const uint32_t x = cur.e[j];
+      if ((uint64_t)last + x > k) continue;
+      if (j + 1 < (int)cur.e.size() && cur.e[j + 1] == x) continue;
+      const double c = product_cost(std::min(x, last), span1);
+      cur.e.push_back(last + x);
+      cur.a.push_back((uint16_t)j);
+      cur.b.push_back(li);
+      cur.cost += c;
+      dfs();
+      cur.cost -= c;
+      cur.e.pop_back();
+      cur.a.pop_back();
+      cur.b.pop_back();
+    }
+  };
+  dfs();
+  std::lock_guard<std::mutex> lk(mu);
+  cache.emplace(key, best);
+  return best;
+}
+
+// C = P^k along the cheapest addition chain; with schedule 1 by the
+// reference's binary exponentiation (range_sparse.hpp:2376-2431: I.P first
+// when bit 0 is set and P is not an SpGEMM output, then the accumulated
+// product times each further square).
+int Plan::power(int p, uint32_t k) {
+  const uint64_t n = slots[p].ncols;
+  const uint32_t sb = slots[p].sb;
+  if (k == 0) return identity(n, sb);
+  if (schedule != 0) {
+    int c = -1;
+    for (;;) {
+      if (k & 1) {
+        if (c >= 0)
+          c = spgemm(c, p);
+        else if (slots[p].kind == P_SPGEMM)
+          c = p;  // I.P == P for a canonical SpGEMM output
+        else
+          c = spgemm(identity(n, sb), p);
+      }
+      k >>= 1;
+      if (k == 0) break;
+      p = spgemm(p, p);
+    }
+    return c;
+  }
+  if (k == 1) {
+    // I.P == P for a canonical SpGEMM output
+    return slots[p].kind == P_SPGEMM ? p : spgemm(identity(n, sb), p);
+  }
+  const PowerChain ch = power_chain(k, slots[p].span);
+  std::vector<int> at(ch.e.size(), p);
+  for (size_t i = 1; i < ch.e.size(); ++i) {
+    // the narrower power on the left: its ranges select the slices
+    const int x = at[ch.a[i]], y = at[ch.b[i]];
+    at[i] = ch.e[ch.a[i]] <= ch.e[ch.b[i]] ? spgemm(x, y) : spgemm(y, x);
+  }
+  return at.back();
+}
+
+// SpGEMMs that power(p, k) plans for a P of this kind and span: the chain's
+// steps, I.P alone for k = 1 when P is not an SpGEMM output; with schedule
+// 1, floor(log2 k) squarings, one product per further set bit, and I.P
+// first when bit 0 is set and P is not an SpGEMM output.
+uint32_t Plan::power_spgemms(uint32_t k, bool p_is_product, uint64_t span) const {
   if (k == 0) return 0;
-  uint32_t sq = 0;
-  while ((k >> (sq + 1)) != 0) ++sq;
-  return sq + (uint32_t)__builtin_popcount(k) - 1 + ((k & 1) && !p_is_product ? 1 : 0);
+  if (schedule != 0) {
+    uint32_t sq = 0;
+    while ((k >> (sq + 1)) != 0) ++sq;
+    return sq + (uint32_t)__builtin_popcount(k) - 1 + ((k & 1) && !p_is_product ? 1 : 0);
+  }
+  if (k == 1) return p_is_product ? 0u : 1u;
+  return (uint32_t)power_chain(k, span).e.size() - 1;
 }
 
 // D = A^d1 . (A + I)^(d2 - d1) (dindex.hpp:46-189), or (A + I)^d2 when d1 = 0.
@@ -122,11 +258,16 @@ int Plan::dindex(int a, uint32_t dlo, uint32_t dup) {
   };
   if (dlo == 0) return power(rai(), dup);
   const uint32_t b = dup - dlo;
-  const uint32_t classic = power_spgemms(dlo, false) + power_spgemms(b, false) + 1;
+  // spans: A, A + I one more, M = A (A + I) twice A's
+  const uint64_t sa = slots[a].span;
+  const uint32_t classic =
+      power_spgemms(dlo, false, sa) + power_spgemms(b, false, sa + 1) + 1;
   const uint32_t m = std::min(dlo, b);
   const uint32_t rest = std::max(dlo, b) - m;
   const uint32_t paired =
-      m == 0 ? ~0u : 1 + power_spgemms(m, true) + (rest ? power_spgemms(rest, false) + 1 : 0);
+      m == 0 ? ~0u
+             : 1 + power_spgemms(m, true, 2 * sa) +
+                   (rest ? power_spgemms(rest, false, dlo > b ? sa : sa + 1) + 1 : 0);
   if (paired < classic && schedule != 1) {
     // M = A (A + I) straight from A: the product kernels add each A range
     // itself, so A + I is only planned when a rest factor needs it
@@ -523,6 +664,8 @@ int Ctx::dindex_planned(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) 
     return dindex_slots(a, dlo, dup, d);
   Plan p;
   p.schedule = tuning.schedule;
+  p.chain_span = tuning.chain_span;
+  p.summary_span = tuning.summary_span;
   p.ref_key = key;
   p.ref_key_set = true;
   p.out = p.dindex(p.external(a), dlo, dup);

@@ -106,7 +106,12 @@ struct Plan {
   int spgemm_plus_i(int a, int b);  // a (b + I)
   int power(int p, uint32_t k);
   int dindex(int a, uint32_t dlo, uint32_t dup);
+  double product_cost(uint32_t x, uint64_t span1) const;
+  struct PowerChain power_chain(uint32_t k, uint64_t span1) const;
+  uint32_t power_spgemms(uint32_t k, bool p_is_product, uint64_t span) const;
   uint32_t schedule = 0;  // 0: fewer SpGEMMs; 1: the reference's order
+  // the lane kernels' span thresholds (dv_tuning), for the cost of a product
+  uint32_t chain_span = 0, summary_span = 0;
   // index builds: the build shape, remembered when the input turns out not
   // to be canonical
   std::array<uint64_t, 6> ref_key{};

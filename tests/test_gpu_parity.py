@@ -196,7 +196,9 @@ def test_distance_index_schedules(ctx, schedule, dlo, dup, groups_only, layout):
     assert_identical(d, orc.dindex(a, dlo, dup), f"D[{dlo},{dup}] schedule {schedule}")
     assert st["layout"] == (0 if layout == 2 else 1)
     if schedule == 0 and (dlo, dup) == (300, 600):
-        assert st["n_spgemm"] == 12 and st["n_spadd"] == 0  # M = A (A + I), then M^300
+        # M = A (A + I), then M^300 along an addition chain no longer than
+        # the binary method's 11 products
+        assert st["n_spgemm"] <= 12 and st["n_spadd"] == 0
 
 
 @pytest.mark.parametrize("layout", [1, 2])
