@@ -117,14 +117,15 @@ static PowerChain binary_chain(uint32_t k) {
 // Estimated time (microseconds at the 5e6-locus graph) of the product
 // P^x . P^y (x <= y) when P's ranges span `span1` rows: the slice width is
 // the span of P^x, w = x (span1 - 1) + 1, and the lane kernel it selects
-// costs about 52 us for narrow slices (w <= 4), 40 + 3.1 (w - 1) us through
-// chains (the rows near breaks, which it defers, grow with w), 125 us with
-// interval summaries (the summary pass included), 60 us slot by slot.
+// costs about 52 us for narrow slices (w <= 4), 52 + 3.1 (w - 1) us through
+// chains (the rows near breaks, which it defers, grow with w; the chain
+// pass over P^y included), 125 us with interval summaries (the summary pass
+// included), 60 us slot by slot.
 double Plan::product_cost(uint32_t x, uint64_t span1) const {
   const uint64_t w = (uint64_t)x * (span1 ? span1 - 1 : 0) + 1;
   if (w <= 4) return 52.0;
   if (summary_span >= 2 && w >= summary_span) return 125.0;
-  if (chain_span && w >= chain_span) return 40.0 + 3.1 * (double)(w - 1);
+  if (chain_span && w >= chain_span) return 52.0 + 3.1 * (double)(w - 1);
   return 60.0;
 }
 
@@ -132,16 +133,17 @@ double Plan::product_cost(uint32_t x, uint64_t span1) const {
 // for P^k under product_cost, no longer than the binary method's: a
 // depth-first search, largest step first, bounded by the cheapest step times
 // the steps still needed (the exponent at most doubles per step), limited to
-// 20 000 nodes; the binary chain when nothing cheaper is found.  Chains are
-// cached per (k, span, thresholds), so a build shape searches once.
+// search_nodes nodes (about 12 ns each); the binary chain when nothing
+// cheaper is found.  Chains are cached per (k, span, thresholds, limit), so
+// a build shape searches once.
 PowerChain Plan::power_chain(uint32_t k, uint64_t span1) const {
   PowerChain best = binary_chain(k);
   for (size_t i = 1; i < best.e.size(); ++i)
     best.cost += product_cost(std::min(best.e[best.a[i]], best.e[best.b[i]]), span1);
   if (k < 3 || schedule != 0) return best;
   static std::mutex mu;
-  static std::map<std::array<uint64_t, 4>, PowerChain> cache;
-  const std::array<uint64_t, 4> key{k, span1, chain_span, summary_span};
+  static std::map<std::array<uint64_t, 5>, PowerChain> cache;
+  const std::array<uint64_t, 5> key{k, span1, chain_span, summary_span, search_nodes};
   {
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);
@@ -159,7 +161,7 @@ PowerChain Plan::power_chain(uint32_t k, uint64_t span1) const {
       if (cur.cost < best.cost - 1e-9) best = cur;
       return;
     }
-    if (++nodes > 20000) return;
+    if (++nodes > search_nodes) return;
     uint32_t need = 0;
     for (uint64_t v = last; v < k; v <<= 1) ++need;
     if (cur.e.size() - 1 + need > max_len || cur.cost + need * minc >= best.cost - 1e-9) return;
@@ -666,6 +668,7 @@ int Ctx::dindex_planned(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) 
   p.schedule = tuning.schedule;
   p.chain_span = tuning.chain_span;
   p.summary_span = tuning.summary_span;
+  p.search_nodes = Plan::search_nodes_for(a.nrows);
   p.ref_key = key;
   p.ref_key_set = true;
   p.out = p.dindex(p.external(a), dlo, dup);

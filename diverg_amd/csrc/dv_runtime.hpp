@@ -112,6 +112,11 @@ struct Plan {
   uint32_t schedule = 0;  // 0: fewer SpGEMMs; 1: the reference's order
   // the lane kernels' span thresholds (dv_tuning), for the cost of a product
   uint32_t chain_span = 0, summary_span = 0;
+  // power chain search: nodes at most (a larger build affords a longer one)
+  uint64_t search_nodes = 5000;
+  static uint64_t search_nodes_for(uint64_t nrows) {
+    return std::min<uint64_t>(20000, std::max<uint64_t>(5000, nrows / 20));
+  }
   // index builds: the build shape, remembered when the input turns out not
   // to be canonical
   std::array<uint64_t, 6> ref_key{};

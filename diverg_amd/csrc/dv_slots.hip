@@ -40,6 +40,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   p.schedule = tuning.schedule;
   p.chain_span = tuning.chain_span;
   p.summary_span = tuning.summary_span;
+  p.search_nodes = Plan::search_nodes_for(a.nrows);
   const int x = p.external(a);
   p.out = p.dindex(x, dlo, dup);
   auto exact = [&](DevMat &o) { return dindex_exact(a, dlo, dup, o); };
