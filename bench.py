@@ -31,9 +31,9 @@ Further legs (fields of the same JSON line; --legs selects them):
 The JSON line also carries:
   roofline      the SpGEMM lane tier, the dominant work of a build: per
                 product, its lane-per-row launch (slot form: k_slot_fast for
-                products read through chains, else k_slot_lane or
-                k_slot_lane_narrow; range CRS: k_spgemm_lane), timed with HIP
-                events on the library stream;
+                products read through chains or of narrow slices, else
+                k_slot_lane; range CRS: k_spgemm_lane), timed with HIP events
+                on the library stream;
                 algorithmic bytes bytes(X) + bytes(Y) + bytes(C) of the
                 product over that time, against 8 TB/s
   cpu_baseline  the CPU oracle (the reference's OpenMP algorithm restated in
@@ -245,7 +245,7 @@ def main():
     roofline = {
         "bound": "hbm",
         "kernel": lane_kernel,
-        "kernels": (["k_slot_fast", "k_slot_lane", "k_slot_lane_narrow"]
+        "kernels": (["k_slot_fast", "k_slot_lane"]
                     if lane_kernel == "slot_lane_tier" else ["k_spgemm_lane"]),
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,

@@ -193,14 +193,14 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     const uint64_t n = c.nrows;
     // slice loads per round trip: about the widest A range
     const uint32_t batch = A.span <= 2 ? 2u : A.span <= 4 ? 4u : 8u;
+    // the chains of C, for the products that read it through them: written
+    // by the lane kernel, or by a pass once the group tiers have run
+    const bool want_chains = c.chains && chain_reader[o.c];
     const uint64_t resident =
         (uint64_t)num_cu * slot_lane_blocks_per_cu(summ_rows != 0, batch, chains);
     const uint32_t grid =
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(n + 255) / 256, resident, per_op}));
     unsigned long long *op_side = side_ctr + (uint64_t)SLOT_SIDE_PARTS * i;
-    // the chains of C, for the products that read it through them: written
-    // by the lane kernel, or by a pass once the group tiers have run
-    const bool want_chains = c.chains && chain_reader[o.c];
     const bool chains_done = slots_lane(
         stream, view_of(A), view_of(B), n, B.nrows, tuning.lane_slice_max, tuning.small_slice_max,
         grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big, ctr, d_status,

@@ -2092,13 +2092,6 @@ __global__ __launch_bounds__(BLOCK) void k_slot_lane(DV_SLOT_LANE_PARAMS) {
   slot_lane_body<SUMM ? SLOT_SUMM : SLOT_PLAIN, LANE_BATCH>(DV_SLOT_LANE_ARGS);
 }
 
-// Narrow slices (2 or 4 slots a batch): six waves per SIMD measured faster
-// than five, even with the few bytes the register cap spills.
-template <int NB>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_slot_lane_narrow(
-    DV_SLOT_LANE_PARAMS) {
-  slot_lane_body<SLOT_PLAIN, NB>(DV_SLOT_LANE_ARGS);
-}
 
 
 // A product over B's chains: every row whose A row is one range [s, e]
@@ -2111,10 +2104,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
 // one per lane (slot_lane_row), so their round trips overlap the other
 // waves' rows; what is left is folded after the loop.  Rows near an indel
 // or a multi-range row defer together (1-3% of the rows of a power of a
-// banded graph).
+// banded graph).  Narrow form (NB = 2 or 4: the product's A ranges span at
+// most NB rows): a row whose A row is one range [s, e] with e - s < NB and
+// whose NB slots s .. (repeating e) are single ranges each touching the
+// running union is one interval, with the same queue for the others.
 constexpr uint32_t FAST_CHUNKS = 4;
 constexpr uint32_t DEFER_RING = 128;  // per wave: < 64 left + one row's worth of 64
 
+template <int NB>  // 0: chain form; 2, 4: narrow form
 __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
   __shared__ uint4 ring[BLOCK / 64][DEFER_RING];
   // the whole block returns or none of it (the barriers of the sums)
@@ -2136,9 +2133,9 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
     __builtin_amdgcn_wave_barrier();
     const bool on = lane < count;
     const uint4 d = q[(q_head + lane) & (DEFER_RING - 1)];
-    slot_lane_row<SLOT_PLAIN, LANE_BATCH, false>(DV_SLOT_LANE_ARGS, on, on ? d.x : 0u,
-                                                 make_uint2(d.y, d.z), nnz_acc, slice_acc,
-                                                 ranges_acc);
+    slot_lane_row<SLOT_PLAIN, NB ? NB : LANE_BATCH, false>(DV_SLOT_LANE_ARGS, on, on ? d.x : 0u,
+                                                            make_uint2(d.y, d.z), nnz_acc,
+                                                            slice_acc, ranges_acc);
     q_head += count;
     __builtin_amdgcn_wave_barrier();
   };
@@ -2152,42 +2149,65 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
     // lanes whose A row is not one range read row 0's data and discard it
     const bool one_a = live && a.x <= a.y;
     const uint32_t s = one_a ? a.x : 0u, e = one_a ? a.y : 0u;
-    const uint32_t c0 = s / CHAIN_ROWS, c1 = e / CHAIN_ROWS;
-    const bool cross = c1 != c0;
-    const uint32_t e0 = cross ? c0 * CHAIN_ROWS + (CHAIN_ROWS - 1) : e;
-    const uint32_t s1 = cross ? c1 * CHAIN_ROWS : s;
-    const uint32_t nmid = cross ? c1 - c0 - 1 : 0u;
-    const uint2 v0 = B.slot[s], v1 = B.slot[e0], w0 = B.slot[s1], w1 = B.slot[e];
-    const uint32_t t0 = c0 * CHAIN_ROWS + BC.seg[s], u0 = c1 * CHAIN_ROWS + BC.seg[s1];
-    uint2 m[FAST_CHUNKS];
+    bool fast;
+    uint32_t cl, ch, pieces;
+    if constexpr (NB == 0) {
+      const uint32_t c0 = s / CHAIN_ROWS, c1 = e / CHAIN_ROWS;
+      const bool cross = c1 != c0;
+      const uint32_t e0 = cross ? c0 * CHAIN_ROWS + (CHAIN_ROWS - 1) : e;
+      const uint32_t s1 = cross ? c1 * CHAIN_ROWS : s;
+      const uint32_t nmid = cross ? c1 - c0 - 1 : 0u;
+      const uint2 v0 = B.slot[s], v1 = B.slot[e0], w0 = B.slot[s1], w1 = B.slot[e];
+      const uint32_t t0 = c0 * CHAIN_ROWS + BC.seg[s], u0 = c1 * CHAIN_ROWS + BC.seg[s1];
+      uint2 m[FAST_CHUNKS];
 #pragma unroll
-    for (int k = 0; k < (int)FAST_CHUNKS; ++k)
-      m[k] = BC.chunk[nmid ? c0 + 1 + min((uint32_t)k, nmid - 1) : c0];
-    // a part is one chain: several rows chained to its end, or one single row
-    const bool head_ok = e0 > s ? t0 >= e0 : v0.x <= v0.y;
-    const bool tail_ok = !cross || (e > s1 ? u0 >= e : w0.x <= w0.y);
-    bool fast = one_a && nmid <= FAST_CHUNKS && head_ok && tail_ok;
-    // the union of the parts, in row order, each touching the running union
-    uint32_t cl = plus_i ? s : v0.x, ch = plus_i ? e : v1.y;
-    auto join = [&](uint32_t lo, uint32_t hi) {
-      fast = fast && lo <= ch + 1u && hi + 1u >= cl;
-      cl = min(cl, lo);
-      ch = max(ch, hi);
-    };
-    if (plus_i) join(v0.x, v1.y);
+      for (int k = 0; k < (int)FAST_CHUNKS; ++k)
+        m[k] = BC.chunk[nmid ? c0 + 1 + min((uint32_t)k, nmid - 1) : c0];
+      // a part is one chain: several rows chained to its end, or one single row
+      const bool head_ok = e0 > s ? t0 >= e0 : v0.x <= v0.y;
+      const bool tail_ok = !cross || (e > s1 ? u0 >= e : w0.x <= w0.y);
+      fast = one_a && nmid <= FAST_CHUNKS && head_ok && tail_ok;
+      // the union of the parts, in row order, each touching the running union
+      cl = plus_i ? s : v0.x;
+      ch = plus_i ? e : v1.y;
+      auto join = [&](uint32_t lo, uint32_t hi) {
+        fast = fast && lo <= ch + 1u && hi + 1u >= cl;
+        cl = min(cl, lo);
+        ch = max(ch, hi);
+      };
+      if (plus_i) join(v0.x, v1.y);
 #pragma unroll
-    for (int k = 0; k < (int)FAST_CHUNKS; ++k)
-      if ((uint32_t)k < nmid) {
-        fast = fast && m[k].x <= m[k].y;
-        join(m[k].x, m[k].y);
+      for (int k = 0; k < (int)FAST_CHUNKS; ++k)
+        if ((uint32_t)k < nmid) {
+          fast = fast && m[k].x <= m[k].y;
+          join(m[k].x, m[k].y);
+        }
+      if (cross) join(w0.x, w1.y);
+      pieces = 3 + nmid + (cross ? 3 : 0);
+    } else {
+      // the NB slots from s, repeating e past it (a repeated range joins
+      // unchanged), each one range touching the running union
+      const uint32_t len = e - s;
+      uint2 v[NB];
+#pragma unroll
+      for (int k = 0; k < NB; ++k) v[k] = B.slot[s + min((uint32_t)k, len)];
+      fast = one_a && len < (uint32_t)NB;
+      cl = plus_i ? s : v[0].x;
+      ch = plus_i ? e : v[0].y;
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        fast = fast && v[k].x <= v[k].y && v[k].x <= ch + 1u && v[k].y + 1u >= cl;
+        cl = min(cl, v[k].x);
+        ch = max(ch, v[k].y);
       }
-    if (cross) join(w0.x, w1.y);
+      pieces = len + 1;
+    }
     const bool empty_a = live && a.x == SLOT_EMPTY;
     if (fast) {
       c_slot[row] = make_uint2(cl, ch);
       nnz_acc += (unsigned long long)(ch - cl) + 1;
       ranges_acc += 1;
-      slice_acc += 3 + nmid + (cross ? 3 : 0);
+      slice_acc += pieces;
     } else if (empty_a) {
       c_slot[row] = make_uint2(SLOT_EMPTY, 0u);
     }
@@ -3064,16 +3084,17 @@ bool slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t 
                      lane_max, mid_max, static_cast<uint2 *>(c_slot),                        \
                      static_cast<uint2 *>(c_side), sa, mid_list, big_list, ctr, status, pre, \
                      suf, pre2, suf2, summ_min, sh, plus_i, parts, bc, c_seg, c_chunk)
-  // the chain form leaves C's chains to slots_chains (its deferred rows
-  // are written after their chunks)
-  if (bc.seg) {
+  // the one-launch forms (chains, narrow slices) leave C's chains to
+  // slots_chains: their deferred rows are written after their chunks
+  const bool narrow = !bc.seg && !pre && batch <= 4;
+  if (bc.seg || narrow) {
     c_seg = nullptr;
     c_chunk = nullptr;
   }
-  if (bc.seg) DV_SLANE(k_slot_fast);
+  if (bc.seg) DV_SLANE(k_slot_fast<0>);
   else if (pre) DV_SLANE(k_slot_lane<true>);
-  else if (batch <= 2) DV_SLANE(k_slot_lane_narrow<2>);
-  else if (batch <= 4) DV_SLANE(k_slot_lane_narrow<4>);
+  else if (batch <= 2) DV_SLANE(k_slot_fast<2>);
+  else if (batch <= 4) DV_SLANE(k_slot_fast<4>);
   else DV_SLANE(k_slot_lane<false>);
 #undef DV_SLANE
   return c_seg != nullptr;
@@ -3134,10 +3155,10 @@ uint32_t slot_lane_blocks_per_cu(bool summ, uint32_t batch, bool chains) {
   if (!cache[k].load(std::memory_order_relaxed)) {
     int nb = 0;
     const hipError_t e =
-        k == 4   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_fast, BLOCK, 0)
+        k == 4   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_fast<0>, BLOCK, 0)
         : k == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<true>, BLOCK, 0)
-        : k == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane_narrow<2>, BLOCK, 0)
-        : k == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane_narrow<4>, BLOCK, 0)
+        : k == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_fast<2>, BLOCK, 0)
+        : k == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_fast<4>, BLOCK, 0)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slot_lane<false>, BLOCK, 0);
     if (e != hipSuccess || nb < 1) {
       (void)hipGetLastError();

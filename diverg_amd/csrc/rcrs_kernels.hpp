@@ -224,11 +224,11 @@ void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
 // the widest A range (narrow products then run fewer instructions).  With
 // b_chains.seg set, B slices are read through B's chains (no summaries);
 // with c_chains.seg set, the kernel writes C's chains (the group tiers' rows
-// end chains).  In the chain form the rows whose slice is not a few chains
-// are folded slot by slot within the same launch.  Returns whether C's
-// chains were written (false when c_chains is null, and in the chain form,
-// whose deferred rows are written after their chunks: then slots_chains
-// makes them).
+// end chains).  In the chain and narrow forms the rows the branch-free join
+// cannot take are folded slot by slot within the same launch.  Returns
+// whether C's chains were written (false when c_chains is null, and in the
+// chain and narrow forms, whose deferred rows are written after their
+// chunks: then slots_chains makes them).
 bool slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t b_nrows,
                 uint32_t lane_max, uint32_t mid_max, uint32_t grid, void *c_slot, void *c_side,
                 uint64_t side_cap, unsigned long long *side_ctr, uint32_t *mid_list,
