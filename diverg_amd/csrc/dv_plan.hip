@@ -117,13 +117,13 @@ static PowerChain binary_chain(uint32_t k) {
 // Estimated time (microseconds at the 5e6-locus graph) of the product
 // P^x . P^y (x <= y) when P's ranges span `span1` rows: the slice width is
 // the span of P^x, w = x (span1 - 1) + 1, and the lane kernel it selects
-// costs about 52 us for narrow slices (w <= 4), 52 + 3.1 (w - 1) us through
+// costs about 36 us for narrow slices (w <= 4), 52 + 3.1 (w - 1) us through
 // chains (the rows near breaks, which it defers, grow with w; the chain
 // pass over P^y included), 125 us with interval summaries (the summary pass
 // included), 60 us slot by slot.
 double Plan::product_cost(uint32_t x, uint64_t span1) const {
   const uint64_t w = (uint64_t)x * (span1 ? span1 - 1 : 0) + 1;
-  if (w <= 4) return 52.0;
+  if (w <= 4) return 36.0;
   if (summary_span >= 2 && w >= summary_span) return 125.0;
   if (chain_span && w >= chain_span) return 52.0 + 3.1 * (double)(w - 1);
   return 60.0;
@@ -152,7 +152,7 @@ PowerChain Plan::power_chain(uint32_t k, uint64_t span1) const {
   const size_t max_len = best.e.size() - 1;
   double minc = 1e30;
   for (uint32_t x : {1u, 2u, 3u, 4u}) minc = std::min(minc, product_cost(x, span1));
-  minc = std::min(minc, 52.0);
+  minc = std::min(minc, 36.0);
   PowerChain cur;
   uint64_t nodes = 0;
   std::function<void()> dfs = [&]() {
