@@ -2163,26 +2163,31 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
 #pragma unroll
       for (int k = 0; k < (int)FAST_CHUNKS; ++k)
         m[k] = BC.chunk[nmid ? c0 + 1 + min((uint32_t)k, nmid - 1) : c0];
-      // a part is one chain: several rows chained to its end, or one single row
-      const bool head_ok = e0 > s ? t0 >= e0 : v0.x <= v0.y;
-      const bool tail_ok = !cross || (e > s1 ? u0 >= e : w0.x <= w0.y);
-      fast = one_a && nmid <= FAST_CHUNKS && head_ok && tail_ok;
-      // the union of the parts, in row order, each touching the running union
+      // a part is one chain: several rows chained to its end, or one single
+      // row.  (Bitwise tests and selects throughout: with short-circuit
+      // tests the compiler sinks the load of seg[s1] into a branch, a
+      // second round trip for every wave with a crossing slice.)
+      const bool head_ok = ((e0 > s) & (t0 >= e0)) | ((e0 <= s) & (v0.x <= v0.y));
+      const bool tail_ok = !cross | ((e > s1) & (u0 >= e)) | ((e <= s1) & (w0.x <= w0.y));
+      fast = one_a & (nmid <= FAST_CHUNKS) & head_ok & tail_ok;
+      // the union of the parts, in row order, each touching the running
+      // union; a part that is not there joins the head's interval again,
+      // which changes nothing
       cl = plus_i ? s : v0.x;
       ch = plus_i ? e : v1.y;
       auto join = [&](uint32_t lo, uint32_t hi) {
-        fast = fast && lo <= ch + 1u && hi + 1u >= cl;
+        fast = fast & (lo <= ch + 1u) & (hi + 1u >= cl);
         cl = min(cl, lo);
         ch = max(ch, hi);
       };
       if (plus_i) join(v0.x, v1.y);
 #pragma unroll
-      for (int k = 0; k < (int)FAST_CHUNKS; ++k)
-        if ((uint32_t)k < nmid) {
-          fast = fast && m[k].x <= m[k].y;
-          join(m[k].x, m[k].y);
-        }
-      if (cross) join(w0.x, w1.y);
+      for (int k = 0; k < (int)FAST_CHUNKS; ++k) {
+        const bool there = (uint32_t)k < nmid;
+        fast = fast & (!there | (m[k].x <= m[k].y));
+        join(there ? m[k].x : v0.x, there ? m[k].y : v1.y);
+      }
+      join(cross ? w0.x : v0.x, cross ? w1.y : v1.y);
       pieces = 3 + nmid + (cross ? 3 : 0);
     } else {
       // the NB slots from s, repeating e past it (a repeated range joins
@@ -2191,12 +2196,12 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
       uint2 v[NB];
 #pragma unroll
       for (int k = 0; k < NB; ++k) v[k] = B.slot[s + min((uint32_t)k, len)];
-      fast = one_a && len < (uint32_t)NB;
+      fast = one_a & (len < (uint32_t)NB);
       cl = plus_i ? s : v[0].x;
       ch = plus_i ? e : v[0].y;
 #pragma unroll
       for (int k = 0; k < NB; ++k) {
-        fast = fast && v[k].x <= v[k].y && v[k].x <= ch + 1u && v[k].y + 1u >= cl;
+        fast = fast & (v[k].x <= v[k].y) & (v[k].x <= ch + 1u) & (v[k].y + 1u >= cl);
         cl = min(cl, v[k].x);
         ch = max(ch, v[k].y);
       }
