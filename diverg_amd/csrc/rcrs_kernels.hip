@@ -357,13 +357,17 @@ __device__ __forceinline__ void cswap_iv(uint32_t &la, uint32_t &ha, uint32_t &l
   ha = h;
 }
 
+// The union of two partial unions.  The merge of the general case is
+// computed with selects (a wave's lanes take different cases in it, and a
+// branch per case would run them one after another).
 __device__ __forceinline__ Ivl2 ivl_join(const Ivl2 &a, const Ivl2 &b) {
   if (ivl_empty(a)) return b;
   if (ivl_empty(b)) return a;
   const Ivl2 u{1u, 0u, SUMM_EMPTY, 0u};
-  if (ivl_unknown(a) || ivl_unknown(b)) return u;
+  if (ivl_unknown(a) | ivl_unknown(b)) return u;
   // the common case: two single intervals that touch
-  if (a.lo1 == SUMM_EMPTY && b.lo1 == SUMM_EMPTY && b.lo0 <= a.hi0 + 1u && b.hi0 + 1u >= a.lo0)
+  if ((a.lo1 == SUMM_EMPTY) & (b.lo1 == SUMM_EMPTY) & (b.lo0 <= a.hi0 + 1u) &
+      (b.hi0 + 1u >= a.lo0))
     return Ivl2{min(a.lo0, b.lo0), max(a.hi0, b.hi0), SUMM_EMPTY, 0u};
   uint32_t L0 = a.lo0, H0 = a.hi0, L1 = a.lo1, H1 = a.hi1, L2 = b.lo0, H2 = b.hi0, L3 = b.lo1,
            H3 = b.hi1;
@@ -372,35 +376,29 @@ __device__ __forceinline__ Ivl2 ivl_join(const Ivl2 &a, const Ivl2 &b) {
   cswap_iv(L0, H0, L2, H2);
   cswap_iv(L1, H1, L3, H3);
   cswap_iv(L1, H1, L2, H2);
+  // sweep the pieces in order, merging each into the last output piece it
+  // touches; a third output piece makes the union unknown
   uint32_t l0 = SUMM_EMPTY, h0 = 0u, l1 = SUMM_EMPTY, h1 = 0u;
-  int k = 0;
+  uint32_t k = 0;  // output pieces so far
   bool over = false;
   auto put = [&](uint32_t L, uint32_t H) {
-    if (L > H) return;  // unused piece
-    if (k == 0) {
-      l0 = L;
-      h0 = H;
-      k = 1;
-    } else if (k == 1) {
-      if (L <= h0 + 1u) {
-        h0 = max(h0, H);
-      } else {
-        l1 = L;
-        h1 = H;
-        k = 2;
-      }
-    } else if (L <= h1 + 1u) {
-      h1 = max(h1, H);
-    } else {
-      over = true;
-    }
+    const bool valid = L <= H;  // an unused piece is skipped
+    const bool t0 = L <= h0 + 1u, t1 = L <= h1 + 1u;
+    const bool first = valid & (k == 0);
+    const bool grow0 = valid & (k == 1) & t0, second = valid & (k == 1) & !t0;
+    const bool grow1 = valid & (k == 2) & t1;
+    over = over | (valid & (k == 2) & !t1);
+    l0 = first ? L : l0;
+    h0 = first ? H : grow0 ? max(h0, H) : h0;
+    l1 = second ? L : l1;
+    h1 = second ? H : grow1 ? max(h1, H) : h1;
+    k = first ? 1u : second ? 2u : k;
   };
   put(L0, H0);
   put(L1, H1);
   put(L2, H2);
   put(L3, H3);
-  if (over) return u;
-  return Ivl2{l0, h0, l1, h1};
+  return over ? u : Ivl2{l0, h0, l1, h1};
 }
 
 // A stored summary is 8 bytes: (lo, hi) for one interval, (EMPTY, 0) for
@@ -449,7 +447,7 @@ __device__ __forceinline__ Ivl2 ivl_shift(const Ivl2 &v) {
 // Single intervals [lo, hi]: one step of the scans of the fast path, which
 // holds while every join is of two touching intervals (ok).
 __device__ __forceinline__ bool iv_touch(uint32_t alo, uint32_t ahi, uint32_t blo, uint32_t bhi) {
-  return blo <= ahi + 1u && bhi + 1u >= alo;
+  return (blo <= ahi + 1u) & (bhi + 1u >= alo);
 }
 
 template <int W, int D>
@@ -458,16 +456,13 @@ __device__ __forceinline__ void iv_scan_step(uint32_t &flo, uint32_t &fhi, uint3
   if constexpr (D < W) {
     const uint32_t olo = lane_shift<W, D, true>(flo), ohi = lane_shift<W, D, true>(fhi);
     const uint32_t ulo = lane_shift<W, D, false>(glo), uhi = lane_shift<W, D, false>(ghi);
-    if (pos >= D) {
-      ok = ok && iv_touch(olo, ohi, flo, fhi);
-      flo = min(flo, olo);
-      fhi = max(fhi, ohi);
-    }
-    if (pos + D < W) {
-      ok = ok && iv_touch(glo, ghi, ulo, uhi);
-      glo = min(glo, ulo);
-      ghi = max(ghi, uhi);
-    }
+    // (selects and bitwise tests: no branch)
+    const bool up = pos >= D, down = pos + D < W;
+    ok = ok & (!up | iv_touch(olo, ohi, flo, fhi)) & (!down | iv_touch(glo, ghi, ulo, uhi));
+    flo = up ? min(flo, olo) : flo;
+    fhi = up ? max(fhi, ohi) : fhi;
+    glo = down ? min(glo, ulo) : glo;
+    ghi = down ? max(ghi, uhi) : ghi;
   }
 }
 
@@ -511,7 +506,7 @@ __device__ __forceinline__ void summary_lane(const uint2 (&x)[SUMM_ROWS_PER_LANE
     phi[0] = x[0].y;
 #pragma unroll
     for (int k = 1; k < K; ++k) {
-      ok = ok && iv_touch(plo[k - 1], phi[k - 1], x[k].x, x[k].y);
+      ok = ok & iv_touch(plo[k - 1], phi[k - 1], x[k].x, x[k].y);
       plo[k] = min(plo[k - 1], x[k].x);
       phi[k] = max(phi[k - 1], x[k].y);
     }
@@ -519,7 +514,7 @@ __device__ __forceinline__ void summary_lane(const uint2 (&x)[SUMM_ROWS_PER_LANE
     shi[K - 1] = x[K - 1].y;
 #pragma unroll
     for (int k = K - 2; k >= 0; --k) {
-      ok = ok && iv_touch(x[k].x, x[k].y, slo[k + 1], shi[k + 1]);
+      ok = ok & iv_touch(x[k].x, x[k].y, slo[k + 1], shi[k + 1]);
       slo[k] = min(slo[k + 1], x[k].x);
       shi[k] = max(shi[k + 1], x[k].y);
     }
@@ -532,18 +527,15 @@ __device__ __forceinline__ void summary_lane(const uint2 (&x)[SUMM_ROWS_PER_LANE
     iv_scan_step<LANES, 32>(flo, fhi, glo, ghi, ok, pos);
     const uint32_t blo = lane_shift<LANES, 1, true>(flo), bhi = lane_shift<LANES, 1, true>(fhi);
     const uint32_t alo = lane_shift<LANES, 1, false>(glo), ahi = lane_shift<LANES, 1, false>(ghi);
+    const bool first = pos == 0, last = pos == LANES - 1;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      if (pos > 0) {
-        ok = ok && iv_touch(blo, bhi, plo[k], phi[k]);
-        plo[k] = min(plo[k], blo);
-        phi[k] = max(phi[k], bhi);
-      }
-      if (pos < LANES - 1) {
-        ok = ok && iv_touch(slo[k], shi[k], alo, ahi);
-        slo[k] = min(slo[k], alo);
-        shi[k] = max(shi[k], ahi);
-      }
+      ok = ok & (first | iv_touch(blo, bhi, plo[k], phi[k])) &
+           (last | iv_touch(slo[k], shi[k], alo, ahi));
+      plo[k] = first ? plo[k] : min(plo[k], blo);
+      phi[k] = first ? phi[k] : max(phi[k], bhi);
+      slo[k] = last ? slo[k] : min(slo[k], alo);
+      shi[k] = last ? shi[k] : max(shi[k], ahi);
     }
     if (__all(ok)) {  // (every row is then below nrows: rows past it are empty)
 #pragma unroll
@@ -588,6 +580,27 @@ __device__ __forceinline__ void summary_lane(const uint2 (&x)[SUMM_ROWS_PER_LANE
   }
 }
 
+// The first range of each of K consecutive rows with bounds rb[0 .. K]
+// (empty: (empty, 0)), all loaded together under one branch; an empty
+// row's load repeats a non-empty row's and is dropped.
+template <int K, typename S>
+__device__ __forceinline__ void load_first_ranges(const uint32_t *__restrict__ en,
+                                                  const S (&rb)[K + 1], uint2 (&x)[K],
+                                                  uint32_t empty) {
+  S any_at = rb[K];
+#pragma unroll
+  for (int k = K - 1; k >= 0; --k) any_at = rb[k + 1] > rb[k] ? rb[k] : any_at;
+#pragma unroll
+  for (int k = 0; k < K; ++k) x[k] = make_uint2(empty, 0u);
+  if (rb[K] > rb[0]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] = ld_range(en, rb[k + 1] > rb[k] ? rb[k] : any_at);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (rb[k + 1] <= rb[k]) x[k] = make_uint2(empty, 0u);
+  }
+}
+
 template <typename S, int LANES>
 __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t nrows,
                                                             uint2 *__restrict__ pre,
@@ -606,15 +619,12 @@ __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t n
     // are read one by one
     S rb[K + 1];
 #pragma unroll
-    for (int k = 0; k <= K; ++k)
-      rb[k] = r0 + k <= nrows ? B.rm[r0 + k] : (k ? rb[k - 1] : (S)0);
+    for (int k = 0; k <= K; ++k) rb[k] = B.rm[min(r0 + k, nrows)];
     uint2 x[K];
+    load_first_ranges<K>(B.en, rb, x, SUMM_EMPTY);
     bool single = true;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      x[k] = rb[k] < rb[k + 1] ? ld_range(B.en, rb[k]) : make_uint2(SUMM_EMPTY, 0u);
-      single = single && rb[k + 1] - rb[k] == 2;
-    }
+    for (int k = 0; k < K; ++k) single = single && rb[k + 1] - rb[k] == 2;
     summary_lane<LANES>(
         x, single,
         [&](int k, Ivl2 &v) {
@@ -2444,9 +2454,13 @@ __global__ __launch_bounds__(BLOCK) void k_slot_summary(Slots B, uint64_t nrows,
     const uint64_t r0 = base + (uint64_t)threadIdx.x * K;
     uint2 sl[K], x[K];
     bool single = true;
+    // every load is made (rows past the end read the last row and are then
+    // emptied): a load under a branch waits at the branch's end
+#pragma unroll
+    for (int k = 0; k < K; ++k) sl[k] = B.slot[min(r0 + k, nrows - 1)];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      sl[k] = r0 + k < nrows ? B.slot[r0 + k] : make_uint2(SLOT_EMPTY, 0u);
+      sl[k] = r0 + k < nrows ? sl[k] : make_uint2(SLOT_EMPTY, 0u);
       const bool one = sl[k].x <= sl[k].y;
       x[k] = one ? sl[k] : make_uint2(SUMM_EMPTY, 0u);
       single = single && one;
@@ -2480,15 +2494,18 @@ __global__ __launch_bounds__(BLOCK) void k_rcrs_to_slots(Mat<S> A, uint64_t nrow
   for (uint64_t b0 = (uint64_t)blockIdx.x * BLOCK * K; b0 < nrows;
        b0 += (uint64_t)gridDim.x * BLOCK * K) {
     const uint64_t r0 = b0 + (uint64_t)threadIdx.x * K;
+    // the rows' bounds (past the end: the last bound, so those rows are
+    // empty), then their first ranges, each set loaded at once (a load under
+    // a branch would wait at the branch's end)
     S rb[K + 1];
 #pragma unroll
-    for (int k = 0; k <= K; ++k) rb[k] = r0 + k <= nrows ? A.rm[r0 + k] : (k ? rb[k - 1] : (S)0);
+    for (int k = 0; k <= K; ++k) rb[k] = A.rm[min(r0 + k, nrows)];
     uint2 x[K];
+    load_first_ranges<K>(A.en, rb, x, SLOT_EMPTY);
     uint64_t need64 = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const uint64_t nr = (uint64_t)(rb[k + 1] - rb[k]) / 2;
-      x[k] = nr ? ld_range(A.en, rb[k]) : make_uint2(SLOT_EMPTY, 0u);
       need64 += nr > 1 ? nr + 1 : 0;
     }
     // (more than a side partition holds: flagged below without a reservation)
