@@ -165,7 +165,8 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     PlanSlot &c = p.slots[o.c];
     OpCounters *ctr = d_plan_ctr + i;
     // timing: two events per SpAdd / SpGEMM, around the kernel it is
-    // priced on (every timed marker holds the queue for a few microseconds)
+    // priced on (a product's lane launch records its own; every event
+    // recorded as a packet of its own holds the queue for a few microseconds)
     if (o.kind == P_IDENT) {
       slots_identity(stream, c.nrows, c.rowmap);
       continue;
@@ -189,7 +190,6 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
                              : 0u;
     if (B.nrows > s.rows) summ_rows = 0;
     if (summ_rows) slots_summary(stream, view_of(B), B.nrows, s.summ, d_status, summ_rows);
-    if (timing) (void)hipEventRecord(plan_ev[2 * i], stream);
     const uint64_t n = c.nrows;
     // slice loads per round trip: about the widest A range
     const uint32_t batch = A.span <= 2 ? 2u : A.span <= 4 ? 4u : 8u;
@@ -207,9 +207,13 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
         summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows, o.plus_i ? 1u : 0u,
         slot_parts + 3ull * per_op * i, batch,
         chains ? slot_chains(B.rowmap, B.nrows) : ChainView{nullptr, nullptr},
-        want_chains ? slot_chains(c.rowmap, c.nrows) : ChainView{nullptr, nullptr});
+        want_chains ? slot_chains(c.rowmap, c.nrows) : ChainView{nullptr, nullptr},
+        timing && n ? plan_ev[2 * i] : nullptr, timing && n ? plan_ev[2 * i + 1] : nullptr);
+    if (timing && !n) {
+      (void)hipEventRecord(plan_ev[2 * i], stream);
+      (void)hipEventRecord(plan_ev[2 * i + 1], stream);
+    }
     if (!used.all) used.n[i] = n ? grid : 0u;
-    if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
     const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);
     slots_groups(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.grid, s.mid, &ctr->n_mid,
                  gb.lds_words, gb.grid, s.big, &ctr->n_big, gm.rows_per_group, c.rowmap, c.entries,

@@ -32,6 +32,7 @@
 // later copy coalesce across rows); after the row-pointer scan they are
 // copied into place and only rows with longer output are recomputed, in
 // direct-write mode.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,6 +40,11 @@
 #include <cstdint>
 
 #include "rcrs_kernels.hpp"
+
+// Tests on bools are written with & and | where both operands are cheap:
+// the short-circuit forms let the compiler branch, and a load it sinks into
+// such a branch costs a round trip of its own.
+#pragma clang diagnostic ignored "-Wbitwise-instead-of-logical"
 
 namespace dv {
 namespace {
@@ -3089,7 +3095,7 @@ bool slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t 
                 uint32_t *big_list, OpCounters *ctr, unsigned int *status, const void *summ,
                 uint32_t summ_min, uint32_t summ_rows, uint32_t plus_i,
                 unsigned long long *parts, uint32_t batch, ChainView b_chains,
-                ChainView c_chains) {
+                ChainView c_chains, hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (!nrows) return true;
   const Chains bc{static_cast<const uint8_t *>(b_chains.seg),
                   static_cast<const uint2 *>(b_chains.chunk)};
@@ -3101,11 +3107,24 @@ bool slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t 
   const uint4 *pre2 = pre ? reinterpret_cast<const uint4 *>(pre + 2 * b_nrows) : nullptr;
   const uint4 *suf2 = pre ? pre2 + b_nrows : nullptr;
   const uint32_t sh = (uint32_t)__builtin_ctz(summ_rows ? summ_rows : 1);
-#define DV_SLANE(K_)                                                                       \
-  hipLaunchKernelGGL(K_, dim3(grid), dim3(BLOCK), 0, s, slots_of(a), slots_of(b), nrows,     \
-                     lane_max, mid_max, static_cast<uint2 *>(c_slot),                        \
-                     static_cast<uint2 *>(c_side), sa, mid_list, big_list, ctr, status, pre, \
-                     suf, pre2, suf2, summ_min, sh, plus_i, parts, bc, c_seg, c_chunk)
+  // with events, the launch records them itself as it starts and ends (the
+  // kernel's own time; an event recorded as a packet of its own holds the
+  // queue for a few microseconds more)
+#define DV_SLANE(K_)                                                                         \
+  do {                                                                                       \
+    if (ev_start)                                                                            \
+      hipExtLaunchKernelGGL(K_, dim3(grid), dim3(BLOCK), 0, s, ev_start, ev_stop, 0u,        \
+                            slots_of(a), slots_of(b), nrows, lane_max, mid_max,              \
+                            static_cast<uint2 *>(c_slot), static_cast<uint2 *>(c_side), sa,  \
+                            mid_list, big_list, ctr, status, pre, suf, pre2, suf2, summ_min, \
+                            sh, plus_i, parts, bc, c_seg, c_chunk);                          \
+    else                                                                                     \
+      hipLaunchKernelGGL(K_, dim3(grid), dim3(BLOCK), 0, s, slots_of(a), slots_of(b), nrows, \
+                         lane_max, mid_max, static_cast<uint2 *>(c_slot),                    \
+                         static_cast<uint2 *>(c_side), sa, mid_list, big_list, ctr, status,  \
+                         pre, suf, pre2, suf2, summ_min, sh, plus_i, parts, bc, c_seg,       \
+                         c_chunk);                                                           \
+  } while (0)
   // the one-launch forms (chains, narrow slices) leave C's chains to
   // slots_chains: their deferred rows are written after their chunks
   const bool narrow = !bc.seg && !pre && batch <= 4;

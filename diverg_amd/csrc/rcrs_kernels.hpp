@@ -228,14 +228,16 @@ void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
 // cannot take are folded slot by slot within the same launch.  Returns
 // whether C's chains were written (false when c_chains is null, and in the
 // chain and narrow forms, whose deferred rows are written after their
-// chunks: then slots_chains makes them).
+// chunks: then slots_chains makes them).  With ev_start / ev_stop the lane
+// kernel's launch records them as it starts and ends.
 bool slots_lane(hipStream_t s, SlotView a, SlotView b, uint64_t nrows, uint64_t b_nrows,
                 uint32_t lane_max, uint32_t mid_max, uint32_t grid, void *c_slot, void *c_side,
                 uint64_t side_cap, unsigned long long *side_ctr, uint32_t *mid_list,
                 uint32_t *big_list, OpCounters *ctr, unsigned int *status, const void *summ,
                 uint32_t summ_min, uint32_t summ_rows, uint32_t plus_i,
                 unsigned long long *parts, uint32_t batch, ChainView b_chains,
-                ChainView c_chains);
+                ChainView c_chains, hipEvent_t ev_start = nullptr,
+                hipEvent_t ev_stop = nullptr);
 // Both group tiers of a product in one launch: rows1 (*n1 of them) by groups
 // of lanes1 lanes (LDS window lds1 words) on grid1 blocks, rows2 by 64-lane
 // groups (lds2) on grid2 blocks; grids multiples of 8.
