@@ -484,22 +484,21 @@ __device__ __forceinline__ void ivl_scan_step(Ivl2 &f, Ivl2 &g, int pos) {
   }
 }
 
-// Each lane takes SUMM_ROWS_PER_LANE consecutive rows: their unions are
+// Each lane takes K (SUMM_ROWS_PER_LANE unless given) consecutive rows: their unions are
 // joined serially into the lane's own prefixes and suffixes, the lane totals
 // are scanned across the block's LANES lanes (both directions), and each
 // row's summary is the join of the lanes before (after) it with its own
 // prefix (suffix).  Joins of summaries in a different order may give
 // "unknown" where another order would not, never a different interval.
-// Summaries of one lane's SUMM_ROWS_PER_LANE rows r0 .. r0+K-1: x[k] is row
+// Summaries of one lane's K rows r0 .. r0+K-1: x[k] is row
 // k's first range ((EMPTY, 0) for an empty row), `single` says whether every
 // row is exactly one range, and more(k, v) joins row k's further ranges
 // into v (only called on the general path).
-template <int LANES, typename More>
-__device__ __forceinline__ void summary_lane(const uint2 (&x)[SUMM_ROWS_PER_LANE], bool single,
+template <int LANES, int K = SUMM_ROWS_PER_LANE, typename More>
+__device__ __forceinline__ void summary_lane(const uint2 (&x)[K], bool single,
                                              More more, uint64_t r0, uint64_t nrows, int pos,
                                              uint2 *__restrict__ pre, uint2 *__restrict__ suf,
                                              uint4 *__restrict__ pre2, uint4 *__restrict__ suf2) {
-  constexpr int K = SUMM_ROWS_PER_LANE;
   const Ivl2 none{SUMM_EMPTY, 0u, SUMM_EMPTY, 0u};
   // Fast path: every row one range and every join below one of two
   // touching intervals, so each partial union is a single interval and
@@ -2445,7 +2444,7 @@ __global__ __launch_bounds__(BLOCK) void k_slot_groups(
 }
 
 // Interval summaries of a slot matrix (as k_interval_summary).
-template <int LANES>
+template <int LANES, int K>
 __global__ __launch_bounds__(BLOCK) void k_slot_summary(Slots B, uint64_t nrows,
                                                         uint2 *__restrict__ pre,
                                                         uint2 *__restrict__ suf,
@@ -2453,7 +2452,6 @@ __global__ __launch_bounds__(BLOCK) void k_slot_summary(Slots B, uint64_t nrows,
                                                         uint4 *__restrict__ suf2,
                                                         const unsigned int *__restrict__ status) {
   if (*status) return;
-  constexpr int K = SUMM_ROWS_PER_LANE;
   const int pos = threadIdx.x & (LANES - 1);
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK * K; base < nrows;
        base += (uint64_t)gridDim.x * BLOCK * K) {
@@ -2471,7 +2469,7 @@ __global__ __launch_bounds__(BLOCK) void k_slot_summary(Slots B, uint64_t nrows,
       x[k] = one ? sl[k] : make_uint2(SUMM_EMPTY, 0u);
       single = single && one;
     }
-    summary_lane<LANES>(
+    summary_lane<LANES, K>(
         x, single,
         [&](int k, Ivl2 &v) {
           if (sl[k].x != SLOT_MARK) return;
@@ -3076,15 +3074,17 @@ void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
   if (!nrows) return;
   uint2 *pre = static_cast<uint2 *>(summ), *suf = pre + nrows;
   uint4 *pre2 = reinterpret_cast<uint4 *>(pre + 2 * nrows), *suf2 = pre2 + nrows;
-  const uint32_t grid = grid_for(nrows, BLOCK * SUMM_ROWS_PER_LANE);
-#define DV_SSUMM(L_)                                                                         \
-  hipLaunchKernelGGL((k_slot_summary<L_>), dim3(grid), dim3(BLOCK), 0, s, slots_of(b), nrows, \
-                     pre, suf, pre2, suf2, status)
-  switch (block_rows / SUMM_ROWS_PER_LANE) {
-    case 8: DV_SSUMM(8); break;
-    case 16: DV_SSUMM(16); break;
-    case 32: DV_SSUMM(32); break;
-    default: DV_SSUMM(64); break;
+  // 32-row blocks take two rows per lane (16 lanes, within one DPP row): a
+  // wave then covers 128 rows, and fewer waves meet a row of several ranges,
+  // which sends the whole wave down the general path
+#define DV_SSUMM(L_, K_)                                                                   \
+  hipLaunchKernelGGL((k_slot_summary<L_, K_>), dim3(grid_for(nrows, BLOCK * K_)), dim3(BLOCK), \
+                     0, s, slots_of(b), nrows, pre, suf, pre2, suf2, status)
+  switch (block_rows) {
+    case 32: DV_SSUMM(16, 2); break;
+    case 64: DV_SSUMM(16, 4); break;
+    case 128: DV_SSUMM(32, 4); break;
+    default: DV_SSUMM(64, 4); break;
   }
 #undef DV_SSUMM
 }
