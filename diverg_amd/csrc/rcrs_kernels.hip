@@ -310,8 +310,8 @@ __device__ __forceinline__ void fold_batch(const uint2 (&v)[N], uint32_t &cl, ui
 template <typename S, bool FAST = true>
 __device__ __forceinline__ void fold_segment(const uint32_t *__restrict__ en, S p0, S p1,
                                              uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
-                                             unsigned long long &gathered) {
-  gathered += (p1 - p0) >> 1;
+                                             uint32_t &gathered) {
+  gathered += (uint32_t)((p1 - p0) >> 1);
   for (S p = p0; ok && p < p1;) {
     const S rem = p1 - p;  // no offset past p1 is formed (it could wrap)
     uint2 v[LANE_BATCH];
@@ -658,7 +658,7 @@ __device__ __forceinline__ void fold_summarised(const uint2 *__restrict__ pre,
                                                 const uint4 *__restrict__ suf2, uint32_t s,
                                                 uint32_t e, uint32_t sh, Rows rows,
                                                 uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
-                                                unsigned long long &gathered) {
+                                                uint32_t &gathered) {
   // row indices are column indices of A, at most 0xFFFFFFFD: 32-bit math
   const uint32_t W = 1u << sh;  // rows per summary block
   const uint32_t h1 = (s + W - 1) >> sh << sh;
@@ -723,7 +723,8 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
     const uint4 *__restrict__ suf2, uint32_t summ_min, uint32_t summ_shift, uint32_t plus_i,
     unsigned long long *__restrict__ parts) {
   if (*status) return;  // an earlier step overflowed: operands are not valid
-  unsigned long long nnz_acc = 0, slice_acc = 0;
+  unsigned long long nnz_acc = 0;
+  uint32_t slice_acc = 0;  // per lane: slices of a few thousand rows at most
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   // Three-stage software pipeline over this lane's rows (one visit apart):
   // while the current row is folded, the next row's first A range's B row
@@ -884,10 +885,10 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
   // one atomic per block on the same two words would serialise at the end
   // of the kernel, when every block finishes at once
   nnz_acc = block_sum_u64(nnz_acc);
-  slice_acc = block_sum_u64(slice_acc);
+  const unsigned long long slice_sum = block_sum_u64(slice_acc);
   if (threadIdx.x == 0) {
     parts[2 * blockIdx.x] = nnz_acc;
-    parts[2 * blockIdx.x + 1] = slice_acc;
+    parts[2 * blockIdx.x + 1] = slice_sum;
   }
 }
 
@@ -1736,6 +1737,30 @@ __device__ __forceinline__ unsigned long long wave_alloc(uint32_t need,
   return (((unsigned long long)bhi << 32) | blo) + incl - need;
 }
 
+// wave_alloc for needs below 8 (a lane-tier row's side entries): the
+// exclusive prefix of the lanes' needs from three ballots of their bits
+// (no shuffle scan), one atomic per wave, the base read back by a scalar
+// broadcast.
+__device__ __forceinline__ unsigned long long wave_alloc_small(uint32_t need,
+                                                               unsigned long long *counter) {
+  const unsigned long long m = __ballot(need != 0);
+  if (m == 0) return 0;
+  const uint32_t lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1;
+  const unsigned long long b0 = __ballot(need & 1u), b1 = __ballot(need & 2u),
+                           b2 = __ballot(need & 4u);
+  const uint32_t before = (uint32_t)__popcll(b0 & below) + 2u * (uint32_t)__popcll(b1 & below) +
+                          4u * (uint32_t)__popcll(b2 & below);
+  const uint32_t total =
+      (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
+  const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)total);
+  const uint32_t blo = (uint32_t)__shfl((int)(uint32_t)base, (int)leader, 64);
+  const uint32_t bhi = (uint32_t)__shfl((int)(uint32_t)(base >> 32), (int)leader, 64);
+  return (((unsigned long long)bhi << 32) | blo) + before;
+}
+
 // Fold the ranges of a marked row (side entry `off`): the header and the
 // first SIDE_BATCH ranges in one batch, any further ones after.
 constexpr int SIDE_BATCH = 4;
@@ -1743,7 +1768,7 @@ static_assert(SLOT_SIDE_PAD >= SIDE_BATCH, "side padding covers a batch");
 
 __device__ __forceinline__ void fold_side(const uint2 *__restrict__ side, uint32_t off,
                                           uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
-                                          unsigned long long &gathered) {
+                                          uint32_t &gathered) {
   uint2 v[SIDE_BATCH];
   const uint2 h = side[off];
 #pragma unroll
@@ -1766,8 +1791,8 @@ __device__ __forceinline__ void fold_side(const uint2 *__restrict__ side, uint32
 // ranges are read after their batch.
 __device__ __forceinline__ void fold_slots(const Slots &B, uint32_t s, uint32_t e, uint32_t &cl,
                                            uint32_t &ch, Runs &r, bool &ok,
-                                           unsigned long long &gathered) {
-  gathered += (unsigned long long)(e - s) + 1;
+                                           uint32_t &gathered) {
+  gathered += (e - s) + 1u;
   for (uint32_t j = s; ok && j <= e;) {
     const uint32_t rem = e - j;  // rows after j in the slice (no index past e is formed)
     uint2 v[LANE_BATCH];
@@ -1810,8 +1835,8 @@ __device__ __forceinline__ void fold_slots(const Slots &B, uint32_t s, uint32_t 
 template <int NB>
 __device__ __forceinline__ void fold_slots_tight(const Slots &B, uint32_t s, uint32_t e,
                                                  uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
-                                                 unsigned long long &gathered) {
-  gathered += (unsigned long long)(e - s) + 1;
+                                                 uint32_t &gathered) {
+  gathered += (e - s) + 1u;
   for (uint32_t j = s; ok && j <= e;) {
     const uint32_t rem = e - j;  // rows after j in the slice (no index past e is formed)
     uint2 v[NB];
@@ -1953,7 +1978,7 @@ template <bool CHAIN_OUT>
 __device__ __forceinline__ void slot_row_emit(DV_SLOT_LANE_PARAMS, bool live, uint64_t row,
                                               uint64_t len, bool ok, Runs &r, uint32_t cl,
                                               uint32_t ch, unsigned long long &nnz_acc,
-                                              unsigned long long &ranges_acc) {
+                                              uint32_t &ranges_acc) {
   int cls = 0;
   uint32_t nruns = 0;
   // live slots form a prefix: with slot 0 empty the row is [cl, ch] alone
@@ -1986,7 +2011,8 @@ __device__ __forceinline__ void slot_row_emit(DV_SLOT_LANE_PARAMS, bool live, ui
   }
   const uint32_t need_side = live && ok && nruns > 1 ? nruns + 1 : 0u;
   const uint32_t part = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) & (SLOT_SIDE_PARTS - 1);
-  const unsigned long long loc = wave_alloc(need_side, &sa.ctr[part]);
+  static_assert(LANE_SLOTS + 1 < 8, "a lane row needs fewer than 8 side entries");
+  const unsigned long long loc = wave_alloc_small(need_side, &sa.ctr[part]);
   if (live && ok) {
     if (need_side && loc + need_side > sa.pcap) {
       atomicOr(status, 1u);  // side partition full: the build falls back
@@ -2010,8 +2036,8 @@ __device__ __forceinline__ void slot_row_emit(DV_SLOT_LANE_PARAMS, bool live, ui
 template <int MODE, int NB, bool CHAIN_OUT>
 __device__ __forceinline__ void slot_lane_row(DV_SLOT_LANE_PARAMS, bool live, uint64_t row,
                                               uint2 a, unsigned long long &nnz_acc,
-                                              unsigned long long &slice_acc,
-                                              unsigned long long &ranges_acc) {
+                                              uint32_t &slice_acc,
+                                              uint32_t &ranges_acc) {
   constexpr bool SUMM = MODE == SLOT_SUMM;
   uint64_t len = 0;
   bool ok = true;
@@ -2069,7 +2095,8 @@ __device__ __forceinline__ void slot_lane_row(DV_SLOT_LANE_PARAMS, bool live, ui
 template <int MODE, int NB>
 __device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
   if (*status) return;
-  unsigned long long nnz_acc = 0, slice_acc = 0, ranges_acc = 0;
+  unsigned long long nnz_acc = 0;
+  uint32_t slice_acc = 0, ranges_acc = 0;  // per lane (the block sums are 64-bit)
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
   uint2 c_a = make_uint2(SLOT_EMPTY, 0u), n_a = make_uint2(SLOT_EMPTY, 0u);
@@ -2093,12 +2120,12 @@ __device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
     need(n_a.y);
   }
   nnz_acc = block_sum_u64(nnz_acc);
-  slice_acc = block_sum_u64(slice_acc);
-  ranges_acc = block_sum_u64(ranges_acc);
+  const unsigned long long slice_sum = block_sum_u64(slice_acc);
+  const unsigned long long ranges_sum = block_sum_u64(ranges_acc);
   if (threadIdx.x == 0) {
     parts[3 * blockIdx.x] = nnz_acc;
-    parts[3 * blockIdx.x + 1] = slice_acc;
-    parts[3 * blockIdx.x + 2] = ranges_acc;
+    parts[3 * blockIdx.x + 1] = slice_sum;
+    parts[3 * blockIdx.x + 2] = ranges_sum;
   }
 }
 
@@ -2136,7 +2163,8 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
   if (st0) return;
   uint4 *q = ring[threadIdx.x >> 6];
   uint32_t q_head = 0, q_tail = 0;  // wave-uniform
-  unsigned long long nnz_acc = 0, slice_acc = 0, ranges_acc = 0;
+  unsigned long long nnz_acc = 0;
+  uint32_t slice_acc = 0, ranges_acc = 0;  // per lane (the block sums are 64-bit)
   const uint32_t lane = __lane_id();
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -2244,12 +2272,12 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
   }
   if (q_tail != q_head) fold_queued(q_tail - q_head);
   nnz_acc = block_sum_u64(nnz_acc);
-  slice_acc = block_sum_u64(slice_acc);
-  ranges_acc = block_sum_u64(ranges_acc);
+  const unsigned long long slice_sum = block_sum_u64(slice_acc);
+  const unsigned long long ranges_sum = block_sum_u64(ranges_acc);
   if (threadIdx.x == 0) {
     parts[3 * blockIdx.x] = nnz_acc;
-    parts[3 * blockIdx.x + 1] = slice_acc;
-    parts[3 * blockIdx.x + 2] = ranges_acc;
+    parts[3 * blockIdx.x + 1] = slice_sum;
+    parts[3 * blockIdx.x + 2] = ranges_sum;
   }
 }
 
@@ -2262,7 +2290,7 @@ __device__ __forceinline__ void slot_row(uint64_t row, const Slots &A, const Slo
                                          uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side,
                                          const SideAlloc &sa, OpCounters *ctr,
                                          unsigned int *status, unsigned long long &nnz_acc,
-                                         unsigned long long &ranges_acc, uint32_t plus_i) {
+                                         uint32_t &ranges_acc, uint32_t plus_i) {
   const uint2 a = A.slot[row];
   uint32_t na = 0, aoff = 0;  // A ranges: the slot itself, or side[aoff + 1 ..]
   if (a.x <= a.y) {
@@ -2405,7 +2433,8 @@ __device__ __forceinline__ void slot_group_rows(
   const uint32_t grp = threadIdx.x / G, lane = threadIdx.x % G;
   uint64_t *lds = lds_all + (size_t)grp * W;
   const uint64_t nl = *n_dev;
-  unsigned long long nnz_acc = 0, ranges_acc = 0;
+  unsigned long long nnz_acc = 0;
+  uint32_t ranges_acc = 0;  // per lane (the block sums are 64-bit)
   const uint64_t groups = (uint64_t)nb * GPB;
   const uint64_t gid = (uint64_t)xcd_tile(b, nb) * GPB + grp;
   for (uint64_t i0 = gid * RPG; i0 < nl; i0 += groups * RPG) {
@@ -2559,7 +2588,8 @@ __global__ __launch_bounds__(BLOCK) void k_slot_plus_identity(
     Slots A, uint64_t nrows, uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side,
     SideAlloc sa, unsigned int *__restrict__ status, unsigned long long *__restrict__ parts) {
   if (*status) return;
-  unsigned long long nnz_acc = 0, ranges_acc = 0;
+  unsigned long long nnz_acc = 0;
+  uint32_t ranges_acc = 0;  // per lane (the block sums are 64-bit)
   for (uint64_t b0 = (uint64_t)blockIdx.x * BLOCK; b0 < nrows; b0 += (uint64_t)gridDim.x * BLOCK) {
     const uint64_t row = b0 + threadIdx.x;
     const bool live = row < nrows;
@@ -2636,11 +2666,11 @@ __global__ __launch_bounds__(BLOCK) void k_slot_plus_identity(
   }
   // per-block partial sums (as the lane tier)
   nnz_acc = block_sum_u64(nnz_acc);
-  ranges_acc = block_sum_u64(ranges_acc);
+  const unsigned long long ranges_sum = block_sum_u64(ranges_acc);
   if (threadIdx.x == 0) {
     parts[3 * blockIdx.x] = nnz_acc;
     parts[3 * blockIdx.x + 1] = 0;
-    parts[3 * blockIdx.x + 2] = ranges_acc;
+    parts[3 * blockIdx.x + 2] = ranges_sum;
   }
 }
 
