@@ -1824,59 +1824,6 @@ __device__ __forceinline__ void fold_slots(const Slots &B, uint32_t s, uint32_t 
   }
 }
 
-// fold_slots for narrow slices, NB (2 or 4) loads per round trip, with
-// fewer instructions per slot: loads past e repeat slot e (a union takes a
-// repeated range unchanged, so no load is conditional) and one test per
-// slot checks that it is a range touching the running interval as it
-// grows.  A batch with any other slot (an empty or marked row, a gap) is
-// redone from the same state through fold(), its marked rows' side ranges
-// after it, so the result is fold()'s either way.  (For 8-slot batches
-// this form takes more registers than the lane kernel can spare.)
-template <int NB>
-__device__ __forceinline__ void fold_slots_tight(const Slots &B, uint32_t s, uint32_t e,
-                                                 uint32_t &cl, uint32_t &ch, Runs &r, bool &ok,
-                                                 uint32_t &gathered) {
-  gathered += (e - s) + 1u;
-  for (uint32_t j = s; ok && j <= e;) {
-    const uint32_t rem = e - j;  // rows after j in the slice (no index past e is formed)
-    uint2 v[NB];
-#pragma unroll
-    for (int k = 0; k < NB; ++k) v[k] = B.slot[j + min((uint32_t)k, rem)];
-    // an empty running interval starts as the first slot
-    const bool none = cl > ch;
-    uint32_t ncl = none ? v[0].x : cl, nch = none ? v[0].y : ch;
-    bool all = true;
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      all = all && v[k].x <= v[k].y && v[k].x <= nch + 1u && v[k].y + 1u >= ncl;
-      ncl = min(ncl, v[k].x);
-      nch = max(nch, v[k].y);
-    }
-    if (all) {
-      cl = ncl;
-      ch = nch;
-    } else {
-      uint32_t mk = 0, offs[NB];
-#pragma unroll
-      for (int k = 0; k < NB; ++k) {
-        if ((uint32_t)k <= rem) fold(v[k].x, v[k].y, cl, ch, r, ok);
-        mk |= (uint32_t)k <= rem && v[k].x == SLOT_MARK ? 1u << k : 0u;
-        offs[k] = v[k].y;
-      }
-      while (ok && mk) {
-        const uint32_t k = (uint32_t)__builtin_ctz(mk);
-        mk &= mk - 1;
-        uint32_t off = offs[0];
-#pragma unroll
-        for (int q = 1; q < NB; ++q) off = k == (uint32_t)q ? offs[q] : off;
-        fold_side(B.side, off, cl, ch, r, ok, gathered);
-      }
-    }
-    j = rem >= (uint32_t)NB ? j + NB : e + 1;
-    if (j == 0) break;  // (e + 1 wrapped: e was the last ordinal)
-  }
-}
-
 // Write a lane-tier row (slot) from its compacted runs: one range in place,
 // several in the side array (reserved by the caller: off, `need` entries).
 __device__ __forceinline__ void slot_store_runs(const Runs &r, uint32_t nruns, uint64_t row,
@@ -1953,8 +1900,9 @@ __global__ __launch_bounds__(BLOCK) void k_slot_chains(const uint2 *__restrict__
 // whose union has more than LANE_SLOTS runs or whose slices are long go to
 // the group lists as in k_spgemm_lane.  A two-stage pipeline loads the A
 // slot of the row after next while this row folds.  B slices are read slot
-// by slot (NB at a time), through interval summaries (SUMM) or through B's
-// chains (k_slot_fast); with c_seg the kernel writes C's chains.
+// by slot (LANE_BATCH at a time), through interval summaries (SUMM) or
+// through B's chains or narrow joins (k_slot_fast); with c_seg the kernel
+// writes C's chains.
 #define DV_SLOT_LANE_PARAMS                                                                \
   Slots A, Slots B, uint64_t nrows, uint32_t lane_max, uint32_t mid_max,                   \
       uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side, SideAlloc sa,                \
@@ -2027,13 +1975,12 @@ __device__ __forceinline__ void slot_row_emit(DV_SLOT_LANE_PARAMS, bool live, ui
 }
 
 // One row of C = A.B (or A (B + I)) by one lane, its A slot `a` given: the
-// B slices are read slot by slot (NB at a time: LANE_BATCH, or 2 / 4 through
-// fold_slots_tight for products of narrow slices), or through interval
+// B slices are read slot by slot (LANE_BATCH at a time), or through interval
 // summaries (SUMM); a row of one or a few runs is written, a row of more
 // runs or a long slice goes to the group lists.  Every lane of the wave
 // calls it (wave-wide reservations); `live` marks the lanes with a row.
 // With CHAIN_OUT the row's chain data is written too.
-template <int MODE, int NB, bool CHAIN_OUT>
+template <int MODE, bool CHAIN_OUT>
 __device__ __forceinline__ void slot_lane_row(DV_SLOT_LANE_PARAMS, bool live, uint64_t row,
                                               uint2 a, unsigned long long &nnz_acc,
                                               uint32_t &slice_acc,
@@ -2078,8 +2025,6 @@ __device__ __forceinline__ void slot_lane_row(DV_SLOT_LANE_PARAMS, bool live, ui
           pre, suf, pre2, suf2, s, e, summ_shift,
           [&](uint32_t r0, uint32_t r1) { fold_slots(B, r0, r1 - 1, cl, ch, r, ok, slice_acc); },
           cl, ch, r, ok, slice_acc);
-    } else if constexpr (NB < LANE_BATCH) {
-      fold_slots_tight<NB>(B, s, e, cl, ch, r, ok, slice_acc);
     } else {
       fold_slots(B, s, e, cl, ch, r, ok, slice_acc);
     }
@@ -2092,7 +2037,7 @@ __device__ __forceinline__ void slot_lane_row(DV_SLOT_LANE_PARAMS, bool live, ui
 // loop; a two-stage pipeline loads the A slot of the row after next while
 // this row folds.  Per-block (nnz, slice, ranges) sums go to
 // parts[3 blockIdx.x ..].
-template <int MODE, int NB>
+template <int MODE>
 __device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
   if (*status) return;
   unsigned long long nnz_acc = 0;
@@ -2112,7 +2057,7 @@ __device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
     c_a = n_a;
     n_a = make_uint2(SLOT_EMPTY, 0u);
     if (row + 2 * stride < nrows) n_a = A.slot[row + 2 * stride];
-    slot_lane_row<MODE, NB, true>(DV_SLOT_LANE_ARGS, row < nrows, row, a, nnz_acc, slice_acc,
+    slot_lane_row<MODE, true>(DV_SLOT_LANE_ARGS, row < nrows, row, a, nnz_acc, slice_acc,
                                   ranges_acc);
     need(c_a.x);
     need(c_a.y);
@@ -2131,7 +2076,7 @@ __device__ __forceinline__ void slot_lane_body(DV_SLOT_LANE_PARAMS) {
 
 template <bool SUMM>
 __global__ __launch_bounds__(BLOCK) void k_slot_lane(DV_SLOT_LANE_PARAMS) {
-  slot_lane_body<SUMM ? SLOT_SUMM : SLOT_PLAIN, LANE_BATCH>(DV_SLOT_LANE_ARGS);
+  slot_lane_body<SUMM ? SLOT_SUMM : SLOT_PLAIN>(DV_SLOT_LANE_ARGS);
 }
 
 
@@ -2171,12 +2116,14 @@ __global__ __launch_bounds__(BLOCK) void k_slot_fast(DV_SLOT_LANE_PARAMS) {
   uint2 c_a = make_uint2(SLOT_EMPTY, 0u), n_a = make_uint2(SLOT_EMPTY, 0u);
   if (row < nrows) c_a = A.slot[row];
   if (row + stride < nrows) n_a = A.slot[row + stride];
-  // fold `count` (at most 64) queued rows from q_head, one per lane
+  // fold `count` (at most 64) queued rows from q_head, one per lane (in the
+  // narrow form too with LANE_BATCH slots per round trip: the kernel then
+  // fits 80 VGPRs, six waves per SIMD)
   auto fold_queued = [&](uint32_t count) {
     __builtin_amdgcn_wave_barrier();
     const bool on = lane < count;
     const uint4 d = q[(q_head + lane) & (DEFER_RING - 1)];
-    slot_lane_row<SLOT_PLAIN, NB ? NB : LANE_BATCH, false>(DV_SLOT_LANE_ARGS, on, on ? d.x : 0u,
+    slot_lane_row<SLOT_PLAIN, false>(DV_SLOT_LANE_ARGS, on, on ? d.x : 0u,
                                                             make_uint2(d.y, d.z), nnz_acc,
                                                             slice_acc, ranges_acc);
     q_head += count;

@@ -220,8 +220,8 @@ void slots_summary(hipStream_t s, SlotView b, uint64_t nrows, void *summ,
 // Lane tier of C = A.B (A (B + I) with plus_i): rows it finishes are
 // written; others go to mid_list / big_list (counts in ctr->n_mid, n_big)
 // for slots_group.  parts receives 3 words per block (nnz, slice, ranges).
-// batch (without summaries): slice loads per round trip, 2, 4 or 8, about
-// the widest A range (narrow products then run fewer instructions).  With
+// batch (without summaries): 2, 4 or 8, about the widest A range; 2 and 4
+// select the narrow form, which loads that many B slots per row.  With
 // b_chains.seg set, B slices are read through B's chains (no summaries);
 // with c_chains.seg set, the kernel writes C's chains (the group tiers' rows
 // end chains).  In the chain and narrow forms the rows the branch-free join
