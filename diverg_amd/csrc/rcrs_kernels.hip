@@ -648,7 +648,7 @@ __global__ __launch_bounds__(BLOCK) void k_interval_summary(Mat<S> B, uint64_t n
 // The summaries are loaded SUMM_BATCH at a time before any is used, so a wide
 // range costs a few round trips, not one per block.  An unknown summary falls
 // back to that piece's ranges.
-constexpr int SUMM_BATCH = 8;
+constexpr int SUMM_BATCH = 6;  // the items of a 151-row slice over 32-row blocks
 
 // rows(r0, r1) folds B rows [r0, r1) from their ranges (the fallback).
 template <typename Rows>
@@ -664,9 +664,9 @@ __device__ __forceinline__ void fold_summarised(const uint2 *__restrict__ pre,
   const uint32_t h1 = (s + W - 1) >> sh << sh;
   const uint32_t t0 = (e + 1) >> sh << sh;
   const uint32_t head = s < h1 ? 1 : 0, tail = t0 <= e ? 1 : 0;
-  const uint32_t nitems = head + ((t0 - h1) >> sh) + tail;  // at most 64 (the caller's check)
-  uint64_t unknown = 0;  // pieces whose summary is unknown, read after the summaries
-  uint64_t two = 0;      // pieces whose summary has two pieces (in the side entries)
+  const uint32_t nitems = head + ((t0 - h1) >> sh) + tail;  // at most 32 (the caller's check)
+  uint32_t unknown = 0;  // pieces whose summary is unknown, read after the summaries
+  uint32_t two = 0;      // pieces whose summary has two pieces (in the side entries)
   for (uint32_t j0 = 0; ok && j0 < nitems; j0 += SUMM_BATCH) {
     uint2 sm[SUMM_BATCH];
 #pragma unroll
@@ -686,15 +686,15 @@ __device__ __forceinline__ void fold_summarised(const uint2 *__restrict__ pre,
         fold(sm[q].x, sm[q].y, cl, ch, r, ok);
         gathered += 1;
       } else if (sm[q].x == SUMM_TWO) {
-        two |= 1ull << (j0 + q);
+        two |= 1u << (j0 + q);
       } else if (sm[q].x != SUMM_EMPTY) {
-        unknown |= 1ull << (j0 + q);
+        unknown |= 1u << (j0 + q);
       }
     }
   }
   // two-piece summaries: one more read each
   while (ok && two) {
-    const uint32_t j = (uint32_t)__builtin_ctzll(two);
+    const uint32_t j = (uint32_t)__builtin_ctz(two);
     two &= two - 1;
     const uint4 t = j < head ? suf2[s] : (tail && j + 1 == nitems) ? pre2[e]
                                                                   : pre2[h1 + ((j - head) << sh) + W - 1];
@@ -704,7 +704,7 @@ __device__ __forceinline__ void fold_summarised(const uint2 *__restrict__ pre,
   }
   // unknown pieces: their own ranges (the union does not depend on the order)
   while (ok && unknown) {
-    const uint32_t j = (uint32_t)__builtin_ctzll(unknown);
+    const uint32_t j = (uint32_t)__builtin_ctz(unknown);
     unknown &= unknown - 1;
     const uint32_t r0 = j < head ? s : (tail && j + 1 == nitems) ? t0 : h1 + ((j - head) << sh);
     const uint32_t r1 = j < head ? h1 : (tail && j + 1 == nitems) ? e + 1 : r0 + W;
@@ -818,7 +818,7 @@ __global__ __launch_bounds__(BLOCK) void k_spgemm_lane(
         // summaries for a range that crosses a block boundary and touches at
         // most 64 blocks
         if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min &&
-            (e >> summ_shift) - (s >> summ_shift) < 64) {
+            (e >> summ_shift) - (s >> summ_shift) < 31) {
           fold_summarised(
               pre, suf, pre2, suf2, ar.x, ar.y, summ_shift,
               [&](uint32_t r0, uint32_t r1) {
@@ -2020,7 +2020,7 @@ __device__ __forceinline__ void slot_lane_row(DV_SLOT_LANE_PARAMS, bool live, ui
     if (!ok) continue;
     if (plus_i) fold(s, e, cl, ch, r, ok);  // C = A (B + I)
     if (SUMM && (s >> summ_shift) != (e >> summ_shift) && e - s + 1 >= summ_min &&
-        (e >> summ_shift) - (s >> summ_shift) < 64) {
+        (e >> summ_shift) - (s >> summ_shift) < 31) {
       fold_summarised(
           pre, suf, pre2, suf2, s, e, summ_shift,
           [&](uint32_t r0, uint32_t r1) { fold_slots(B, r0, r1 - 1, cl, ch, r, ok, slice_acc); },
