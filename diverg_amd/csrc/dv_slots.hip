@@ -223,9 +223,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   }
   slots_reduce_parts(stream, slot_parts, per_op, used, side_ctr, (uint32_t)nops, d_plan_ctr);
   const PlanSlot &O = p.slots[p.out];
-  slots_counts(stream, view_of(O), O.nrows, s.cnt, d_status);
-  scan_rowmap(stream, d.sb, s.cnt, O.nrows, d.rowmap, s.tmp, c_out, ~0ull, d_status, nullptr, 0,
-              nullptr);
+  slots_scan_rowmap(stream, d.sb, view_of(O), O.nrows, s.cnt, d.rowmap, s.tmp, c_out, d_status);
 
   // ---- one readback
   bool ok = hip(hipGetLastError(), "slot plan launch") &&

@@ -2657,15 +2657,35 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_parts(const unsigned long long
 
 // Slots -> rCRS: entry counts per row, then (after the row-pointer scan) the
 // entries at their row pointers.
-__global__ __launch_bounds__(BLOCK) void k_slot_counts(Slots A, uint64_t nrows,
-                                                       uint32_t *__restrict__ cnt,
-                                                       const unsigned int *__restrict__ status) {
-  if (*status) return;  // a product ran out of side entries: slots not valid
-  for (uint64_t row = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; row < nrows;
-       row += (uint64_t)gridDim.x * BLOCK) {
-    const uint2 a = A.slot[row];
-    cnt[row] = a.x <= a.y ? 2u : a.x == SLOT_MARK ? 2u * A.side[a.y].x : 0u;
+// The entry counts of a slot matrix's rows (2 per range) and the
+// row-pointer scan's tile sums in one pass: each block takes one scan tile
+// of rows, writes their counts and the tile's sum.  The slots are loaded together (rows past the end read the
+// last row and count nothing).
+__global__ __launch_bounds__(BLOCK) void k_slot_tile_counts(Slots A, uint64_t n,
+                                                            uint32_t *__restrict__ cnt,
+                                                            unsigned long long *__restrict__ tsum,
+                                                            const unsigned int *__restrict__ status) {
+  // the whole block returns or none of it (the barriers of the sum)
+  __shared__ unsigned int st0;
+  if (threadIdx.x == 0) st0 = *status;
+  __syncthreads();
+  if (st0) return;  // a product ran out of side entries: slots not valid
+  const uint64_t t0 = (uint64_t)blockIdx.x * SCAN_TILE;
+  uint2 a[SCAN_ITEMS];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k)
+    a[k] = A.slot[min(t0 + (uint64_t)k * BLOCK + threadIdx.x, n - 1)];
+  unsigned long long v = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const uint64_t i = t0 + (uint64_t)k * BLOCK + threadIdx.x;
+    if (i >= n) continue;
+    const uint32_t c = a[k].x <= a[k].y ? 2u : a[k].x == SLOT_MARK ? 2u * A.side[a[k].y].x : 0u;
+    cnt[i] = c;
+    v += c;
   }
+  v = block_sum_u64(v);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = v;
 }
 
 template <typename S>
@@ -2713,6 +2733,11 @@ void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *en
 
 uint64_t scan_words(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
+void scan_tiles_rows(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
+                     unsigned long long *tile_sums, OpCounters *ctr, uint64_t cap,
+                     unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
+                     uint32_t *c_entries);
+
 void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
                  unsigned long long *tile_sums, OpCounters *ctr, uint64_t cap,
                  unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
@@ -2724,6 +2749,17 @@ void scan_rowmap(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, vo
   const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   hipLaunchKernelGGL(k_tile_sums, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, cnt, n, tile_sums,
                      parts, parts ? nparts : 0u, ctr);
+  scan_tiles_rows(s, sb, cnt, n, rowmap, tile_sums, ctr, cap, status, stage, stage_entries,
+                  c_entries);
+}
+
+// The row-pointer scan after its tile sums: the spine (many tiles), then
+// the rows.
+void scan_tiles_rows(hipStream_t s, uint32_t sb, const uint32_t *cnt, uint64_t n, void *rowmap,
+                     unsigned long long *tile_sums, OpCounters *ctr, uint64_t cap,
+                     unsigned int *status, const uint32_t *stage, uint32_t stage_entries,
+                     uint32_t *c_entries) {
+  const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   const int prefixed = ntiles >= SPINE_MIN_TILES;
   if (prefixed)
     hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(SPINE_THREADS), 0, s, tile_sums, ntiles);
@@ -3149,11 +3185,17 @@ void slots_reduce_parts(hipStream_t s, const unsigned long long *parts, uint32_t
                      side_ctr, ctr);
 }
 
-void slots_counts(hipStream_t s, SlotView a, uint64_t n, uint32_t *cnt,
-                  const unsigned int *status) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_slot_counts, dim3(grid_for(n)), dim3(BLOCK), 0, s, slots_of(a), n, cnt,
-                     status);
+void slots_scan_rowmap(hipStream_t s, uint32_t sb, SlotView a, uint64_t n, uint32_t *cnt,
+                       void *rowmap, unsigned long long *tile_sums, OpCounters *ctr,
+                       unsigned int *status) {
+  if (n == 0) {
+    fill_u32(s, static_cast<uint32_t *>(rowmap), sb / 4, 0u);
+    return;
+  }
+  const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_slot_tile_counts, dim3((uint32_t)ntiles), dim3(BLOCK), 0, s, slots_of(a),
+                     n, cnt, tile_sums, status);
+  scan_tiles_rows(s, sb, cnt, n, rowmap, tile_sums, ctr, ~0ull, status, nullptr, 0, nullptr);
 }
 
 void slots_emit(hipStream_t s, SlotView a, uint64_t n, uint32_t sb, const void *rowmap,

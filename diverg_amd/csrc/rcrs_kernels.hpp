@@ -169,7 +169,6 @@ void window_pieces(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *cnt,
 // B slice length) -- the work measure behind the size classes.
 void slice_lengths(hipStream_t s, uint32_t sb, DView a, DView b, uint32_t *len);
 
-
 // --- Slot format (intermediate matrices of planned index builds) -----------
 // One uint2 per row: (lo, hi) for a row of one range, (0xFFFFFFFF, 0) for an
 // empty row, (0xFFFFFFFE, off) for a row of several ranges in the side array
@@ -261,9 +260,12 @@ struct PartBlocks {
 void slots_reduce_parts(hipStream_t s, const unsigned long long *parts, uint32_t per_op,
                         const PartBlocks &used, const unsigned long long *side_ctr,
                         uint32_t nops, OpCounters *ctr);
-// Slots -> rCRS: per-row entry counts, and the entries at their row pointers.
-void slots_counts(hipStream_t s, SlotView a, uint64_t n, uint32_t *cnt,
-                  const unsigned int *status);
+// Slots -> rCRS: the row pointers (2 entries per range; cnt receives the
+// per-row counts, tile_sums the scan's tile sums: scan_rowmap's buffers,
+// no capacity), then the entries at their row pointers.
+void slots_scan_rowmap(hipStream_t s, uint32_t sb, SlotView a, uint64_t n, uint32_t *cnt,
+                       void *rowmap, unsigned long long *tile_sums, OpCounters *ctr,
+                       unsigned int *status);
 void slots_emit(hipStream_t s, SlotView a, uint64_t n, uint32_t sb, const void *rowmap,
                 uint32_t *entries);
 // Resident blocks per CU of the lane kernel that slots_lane runs for these
