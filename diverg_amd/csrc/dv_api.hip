@@ -404,6 +404,7 @@ int dv_finalize(dv_context *c) {
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c->plan_ev) (void)hipEventDestroy(e);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->slot_graph_exec) (void)hipGraphExecDestroy(c->slot_graph_exec);
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);
   if (c->h_status) (void)hipHostFree(c->h_status);
   if (c->d_status) (void)hipFree(c->d_status);

@@ -176,6 +176,10 @@ struct Ctx {
   std::vector<hipEvent_t> plan_ev;
   // hipGraph of the last planned build (replayed while its fingerprint holds)
   hipGraphExec_t graph_exec = nullptr;
+  // hipGraph of the last slot-form build's products (replayed while every
+  // pointer, size and launch parameter they use, slot_graph_key, holds)
+  hipGraphExec_t slot_graph_exec = nullptr;
+  std::vector<uint64_t> slot_graph_key;
   uint64_t graph_key = 0;
   uint64_t graph_captures = 0;
   // autotuned launch shapes per build shape (rows, entries, nnz, size type,

@@ -144,84 +144,150 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   for (const PlanOp &o : p.ops)
     if (reads_chains(o)) chain_reader[o.b] = 1;
 
-  // ---- enqueue
+  // ---- enqueue: the products (with tuning.graphs, captured once into a
+  // hipGraph and replayed while everything they use is the same; timed
+  // builds run eagerly), then the row-pointer scan into the output
   const bool timing = prof && plan_events_reserve(nops);
-  fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
-           (uint32_t)((nops + 2) * sizeof(OpCounters) / 4), 0u);
-  fill_u32(stream, d_status, 1, 0u);
-  // per-block parts: each launched block writes its own, so only the side
-  // counters are cleared (all parts too when the operations are too many
-  // to pass their block counts)
   PartBlocks used{};
-  used.all = nops > (int)PART_BLOCKS_MAX_OPS ? 1u : 0u;
-  if (used.all)
-    fill_u32(stream, reinterpret_cast<uint32_t *>(slot_parts), 2 * (parts_words + ctr_words), 0u);
-  else
-    fill_u32(stream, reinterpret_cast<uint32_t *>(side_ctr), 2 * ctr_words, 0u);
-  slots_from_rcrs(stream, a.sb, view(a), in.rowmap, in.entries, in.cap,
-                  side_ctr + (uint64_t)SLOT_SIDE_PARTS * nops, d_status);
-  for (int i = 0; i < nops; ++i) {
-    const PlanOp &o = p.ops[i];
-    PlanSlot &c = p.slots[o.c];
-    OpCounters *ctr = d_plan_ctr + i;
-    // timing: two events per SpAdd / SpGEMM, around the kernel it is
-    // priced on (a product's lane launch records its own; every event
-    // recorded as a packet of its own holds the queue for a few microseconds)
-    if (o.kind == P_IDENT) {
-      slots_identity(stream, c.nrows, c.rowmap);
-      continue;
+  auto enqueue_products = [&]() {
+    fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
+             (uint32_t)((nops + 2) * sizeof(OpCounters) / 4), 0u);
+    fill_u32(stream, d_status, 1, 0u);
+    // per-block parts: each launched block writes its own, so only the side
+    // counters are cleared (all parts too when the operations are too many
+    // to pass their block counts)
+    used = PartBlocks{};
+    used.all = nops > (int)PART_BLOCKS_MAX_OPS ? 1u : 0u;
+    if (used.all)
+      fill_u32(stream, reinterpret_cast<uint32_t *>(slot_parts), 2 * (parts_words + ctr_words), 0u);
+    else
+      fill_u32(stream, reinterpret_cast<uint32_t *>(side_ctr), 2 * ctr_words, 0u);
+    slots_from_rcrs(stream, a.sb, view(a), in.rowmap, in.entries, in.cap,
+                    side_ctr + (uint64_t)SLOT_SIDE_PARTS * nops, d_status);
+    for (int i = 0; i < nops; ++i) {
+      const PlanOp &o = p.ops[i];
+      PlanSlot &c = p.slots[o.c];
+      OpCounters *ctr = d_plan_ctr + i;
+      // timing: two events per SpAdd / SpGEMM, around the kernel it is
+      // priced on (a product's lane launch records its own; every event
+      // recorded as a packet of its own holds the queue for a few microseconds)
+      if (o.kind == P_IDENT) {
+        slots_identity(stream, c.nrows, c.rowmap);
+        continue;
+      }
+      if (o.kind == P_SPADD) {
+        const PlanSlot &xs = p.slots[p.slots[o.a].kind == P_IDENT ? o.b : o.a];
+        if (timing) (void)hipEventRecord(plan_ev[2 * i], stream);
+        slots_plus_identity(stream, view_of(xs), c.nrows, c.rowmap, c.entries, c.cap,
+                            side_ctr + (uint64_t)SLOT_SIDE_PARTS * i, d_status,
+                            slot_parts + 3ull * per_op * i, per_op);
+        if (!used.all) used.n[i] = slots_plus_identity_blocks(c.nrows, per_op);
+        if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
+        continue;
+      }
+      const PlanSlot &A = p.slots[o.a], &B = p.slots[o.b];
+      // B slices through B's chains, or through interval summaries, or slot by slot
+      // (wide products, whose slices hold more chain ends, read summaries)
+      const bool chains = reads_chains(o) && B.chains;
+      uint32_t summ_rows = !chains && tuning.summary_span >= 2 && A.span >= tuning.summary_span
+                               ? summary_block_rows(A.span)
+                               : 0u;
+      if (B.nrows > s.rows) summ_rows = 0;
+      if (summ_rows) slots_summary(stream, view_of(B), B.nrows, s.summ, d_status, summ_rows);
+      const uint64_t n = c.nrows;
+      // slice loads per round trip: about the widest A range
+      const uint32_t batch = A.span <= 2 ? 2u : A.span <= 4 ? 4u : 8u;
+      // the chains of C, for the products that read it through them: written
+      // by the lane kernel, or by a pass once the group tiers have run
+      const bool want_chains = c.chains && chain_reader[o.c];
+      const uint64_t resident =
+          (uint64_t)num_cu * slot_lane_blocks_per_cu(summ_rows != 0, batch, chains);
+      const uint32_t grid =
+          (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(n + 255) / 256, resident, per_op}));
+      unsigned long long *op_side = side_ctr + (uint64_t)SLOT_SIDE_PARTS * i;
+      const bool chains_done = slots_lane(
+          stream, view_of(A), view_of(B), n, B.nrows, tuning.lane_slice_max, tuning.small_slice_max,
+          grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big, ctr, d_status,
+          summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows, o.plus_i ? 1u : 0u,
+          slot_parts + 3ull * per_op * i, batch,
+          chains ? slot_chains(B.rowmap, B.nrows) : ChainView{nullptr, nullptr},
+          want_chains ? slot_chains(c.rowmap, c.nrows) : ChainView{nullptr, nullptr},
+          timing && n ? plan_ev[2 * i] : nullptr, timing && n ? plan_ev[2 * i + 1] : nullptr);
+      if (timing && !n) {
+        (void)hipEventRecord(plan_ev[2 * i], stream);
+        (void)hipEventRecord(plan_ev[2 * i + 1], stream);
+      }
+      if (!used.all) used.n[i] = n ? grid : 0u;
+      const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);
+      slots_groups(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.grid, s.mid, &ctr->n_mid,
+                   gb.lds_words, gb.grid, s.big, &ctr->n_big, gm.rows_per_group, c.rowmap, c.entries,
+                   c.cap, op_side, ctr, d_status, o.plus_i ? 1u : 0u);
+      if (want_chains && !chains_done)
+        slots_chains(stream, c.rowmap, c.nrows, slot_chains(c.rowmap, c.nrows), d_status);
     }
-    if (o.kind == P_SPADD) {
-      const PlanSlot &xs = p.slots[p.slots[o.a].kind == P_IDENT ? o.b : o.a];
-      if (timing) (void)hipEventRecord(plan_ev[2 * i], stream);
-      slots_plus_identity(stream, view_of(xs), c.nrows, c.rowmap, c.entries, c.cap,
-                          side_ctr + (uint64_t)SLOT_SIDE_PARTS * i, d_status,
-                          slot_parts + 3ull * per_op * i, per_op);
-      if (!used.all) used.n[i] = slots_plus_identity_blocks(c.nrows, per_op);
-      if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
-      continue;
+    slots_reduce_parts(stream, slot_parts, per_op, used, side_ctr, (uint32_t)nops, d_plan_ctr);
+  };
+  if (tuning.graphs && !timing) {
+    // everything the captured launches read: operations, buffers, sizes,
+    // spans (they choose the kernels), scratch, counters, tuning
+    std::vector<uint64_t> key;
+    key.reserve(16 * p.slots.size() + 8 * nops + 64);
+    auto put = [&](uint64_t v) { key.push_back(v); };
+    put((uint64_t)nops);
+    for (const PlanOp &o : p.ops) {
+      put(o.kind);
+      put(o.plus_i);
+      put((uint64_t)(int64_t)o.a);
+      put((uint64_t)(int64_t)o.b);
+      put((uint64_t)(int64_t)o.c);
     }
-    const PlanSlot &A = p.slots[o.a], &B = p.slots[o.b];
-    // B slices through B's chains, or through interval summaries, or slot by slot
-    // (wide products, whose slices hold more chain ends, read summaries)
-    const bool chains = reads_chains(o) && B.chains;
-    uint32_t summ_rows = !chains && tuning.summary_span >= 2 && A.span >= tuning.summary_span
-                             ? summary_block_rows(A.span)
-                             : 0u;
-    if (B.nrows > s.rows) summ_rows = 0;
-    if (summ_rows) slots_summary(stream, view_of(B), B.nrows, s.summ, d_status, summ_rows);
-    const uint64_t n = c.nrows;
-    // slice loads per round trip: about the widest A range
-    const uint32_t batch = A.span <= 2 ? 2u : A.span <= 4 ? 4u : 8u;
-    // the chains of C, for the products that read it through them: written
-    // by the lane kernel, or by a pass once the group tiers have run
-    const bool want_chains = c.chains && chain_reader[o.c];
-    const uint64_t resident =
-        (uint64_t)num_cu * slot_lane_blocks_per_cu(summ_rows != 0, batch, chains);
-    const uint32_t grid =
-        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(n + 255) / 256, resident, per_op}));
-    unsigned long long *op_side = side_ctr + (uint64_t)SLOT_SIDE_PARTS * i;
-    const bool chains_done = slots_lane(
-        stream, view_of(A), view_of(B), n, B.nrows, tuning.lane_slice_max, tuning.small_slice_max,
-        grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big, ctr, d_status,
-        summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows, o.plus_i ? 1u : 0u,
-        slot_parts + 3ull * per_op * i, batch,
-        chains ? slot_chains(B.rowmap, B.nrows) : ChainView{nullptr, nullptr},
-        want_chains ? slot_chains(c.rowmap, c.nrows) : ChainView{nullptr, nullptr},
-        timing && n ? plan_ev[2 * i] : nullptr, timing && n ? plan_ev[2 * i + 1] : nullptr);
-    if (timing && !n) {
-      (void)hipEventRecord(plan_ev[2 * i], stream);
-      (void)hipEventRecord(plan_ev[2 * i + 1], stream);
+    for (const PlanSlot &sl : p.slots) {
+      put((uint64_t)sl.rowmap);
+      put((uint64_t)sl.entries);
+      put(sl.cap);
+      put(sl.nrows);
+      put(sl.ncols);
+      put(sl.sb);
+      put(sl.span);
+      put(sl.kind);
+      put(sl.chains);
     }
-    if (!used.all) used.n[i] = n ? grid : 0u;
-    const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);
-    slots_groups(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.grid, s.mid, &ctr->n_mid,
-                 gb.lds_words, gb.grid, s.big, &ctr->n_big, gm.rows_per_group, c.rowmap, c.entries,
-                 c.cap, op_side, ctr, d_status, o.plus_i ? 1u : 0u);
-    if (want_chains && !chains_done)
-      slots_chains(stream, c.rowmap, c.nrows, slot_chains(c.rowmap, c.nrows), d_status);
+    for (uint64_t v : {(uint64_t)a.rowmap, (uint64_t)a.entries, a.nrows, a.ncols, (uint64_t)a.sb,
+                       a.n_entries, (uint64_t)s.mid, (uint64_t)s.big, (uint64_t)s.summ, s.rows,
+                       (uint64_t)slot_parts, (uint64_t)side_ctr, (uint64_t)d_plan_ctr,
+                       (uint64_t)d_status, (uint64_t)per_op, (uint64_t)num_cu})
+      put(v);
+    const uint32_t *tw = reinterpret_cast<const uint32_t *>(&tuning);
+    for (size_t k = 0; k < sizeof(dv_tuning) / 4; ++k) put(tw[k]);
+    if (!slot_graph_exec || key != slot_graph_key) {
+      if (slot_graph_exec) (void)hipGraphExecDestroy(slot_graph_exec);
+      slot_graph_exec = nullptr;
+      slot_graph_key.clear();
+      hipGraph_t g = nullptr;
+      bool ok = hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+      if (ok) {
+        enqueue_products();
+        ok = hip(hipStreamEndCapture(stream, &g), "end capture");
+      }
+      ok = ok && hip(hipGraphInstantiate(&slot_graph_exec, g, nullptr, nullptr, 0), "instantiate");
+      if (g) (void)hipGraphDestroy(g);
+      if (!ok) {
+        slot_graph_exec = nullptr;
+        release_all();
+        release(d);
+        return DV_ERR_DEVICE;
+      }
+      slot_graph_key = std::move(key);
+      graph_captures += 1;
+    }
+    if (!hip(hipGraphLaunch(slot_graph_exec, stream), "graph launch")) {
+      release_all();
+      release(d);
+      return DV_ERR_DEVICE;
+    }
+  } else {
+    enqueue_products();
   }
-  slots_reduce_parts(stream, slot_parts, per_op, used, side_ctr, (uint32_t)nops, d_plan_ctr);
   const PlanSlot &O = p.slots[p.out];
   slots_scan_rowmap(stream, d.sb, view_of(O), O.nrows, s.cnt, d.rowmap, s.tmp, c_out, d_status);
 

@@ -259,6 +259,34 @@ def test_device_resident_chain(ctx, graph_1e5):
     assert_identical(d.to_host(), orc.dindex(graph_1e5, 3, 9), "D on device")
 
 
+@pytest.mark.parametrize("layout", [0, 2])
+def test_distance_index_graph_replays(ctx, graph_1e5, layout):
+    """Slot-form builds with graphs=1: the products are captured into a
+    hipGraph once per shape and replayed.  Several replays of one shape on a
+    resident input, with eager builds of another shape, host-input builds
+    (which capture again) and other device work between them; every result
+    against the oracle."""
+    ctx.set_tuning(graphs=1, layout=layout)
+    print(f"dindex graph replays layout={layout}", flush=True)
+    want = orc.dindex(graph_1e5, 150, 450)
+    small = dv.generate_graph(3, backbone_bp=20000, seed=3, snp_rate=0.02, indel_rate=0.005)
+    want_small = orc.dindex(small, 20, 90)
+    da = ctx.to_device(graph_1e5)
+    for rep in range(3):
+        # the first build of the shape captures, the next ones replay
+        for k in range(3):
+            d = ctx.create_distance_index(da, 150, 450, location="device")
+            assert_identical(d.to_host(), want, f"round {rep} build {k}")
+            d.free()
+        # between rounds: eager products and a build of another shape from a
+        # host input (its own capture)
+        sq = ctx.range_spgemm(da, da, location="device")
+        sq.free()
+        assert_identical(ctx.create_distance_index(small, 20, 90), want_small, f"other {rep}")
+    ctx.set_tuning(graphs=0)
+    assert_identical(ctx.create_distance_index(da, 150, 450), want, "eager after replays")
+
+
 def test_query_batch(ctx, graph_1e5):
     d = orc.dindex(graph_1e5, 150, 450)
     d = dv.RangeCRS(d.ncols, d.entries, d.rowmap, d.nnz)
