@@ -51,9 +51,12 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
       return plan_run(p, d, exact);
   if (p.out == x || nops == 0) return plan_run(p, d, exact);
 
-  // records: one per operation, then the input conversion and the output scan
-  if (!plan_ctr_reserve(nops + 2)) return fail("device allocation failed (plan counters)", DV_ERR_NOMEM);
+  // records: one per operation, then the input conversion and the output
+  // scan, then the build's status word (so one fill clears them all and one
+  // copy reads them back)
+  if (!plan_ctr_reserve(nops + 3)) return fail("device allocation failed (plan counters)", DV_ERR_NOMEM);
   OpCounters *c_out = d_plan_ctr + nops + 1;
+  unsigned int *st_dev = reinterpret_cast<unsigned int *>(d_plan_ctr + nops + 2);
   uint64_t nmax = a.nrows;
   for (const PlanOp &o : p.ops) nmax = std::max(nmax, p.slots[o.c].nrows);
   OpScratch s;
@@ -151,8 +154,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   PartBlocks used{};
   auto enqueue_products = [&]() {
     fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
-             (uint32_t)((nops + 2) * sizeof(OpCounters) / 4), 0u);
-    fill_u32(stream, d_status, 1, 0u);
+             (uint32_t)((nops + 3) * sizeof(OpCounters) / 4), 0u);
     // per-block parts: each launched block writes its own, so only the side
     // counters are cleared (all parts too when the operations are too many
     // to pass their block counts)
@@ -163,7 +165,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     else
       fill_u32(stream, reinterpret_cast<uint32_t *>(side_ctr), 2 * ctr_words, 0u);
     slots_from_rcrs(stream, a.sb, view(a), in.rowmap, in.entries, in.cap,
-                    side_ctr + (uint64_t)SLOT_SIDE_PARTS * nops, d_status);
+                    side_ctr + (uint64_t)SLOT_SIDE_PARTS * nops, st_dev);
     for (int i = 0; i < nops; ++i) {
       const PlanOp &o = p.ops[i];
       PlanSlot &c = p.slots[o.c];
@@ -179,7 +181,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
         const PlanSlot &xs = p.slots[p.slots[o.a].kind == P_IDENT ? o.b : o.a];
         if (timing) (void)hipEventRecord(plan_ev[2 * i], stream);
         slots_plus_identity(stream, view_of(xs), c.nrows, c.rowmap, c.entries, c.cap,
-                            side_ctr + (uint64_t)SLOT_SIDE_PARTS * i, d_status,
+                            side_ctr + (uint64_t)SLOT_SIDE_PARTS * i, st_dev,
                             slot_parts + 3ull * per_op * i, per_op);
         if (!used.all) used.n[i] = slots_plus_identity_blocks(c.nrows, per_op);
         if (timing) (void)hipEventRecord(plan_ev[2 * i + 1], stream);
@@ -193,7 +195,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
                                ? summary_block_rows(A.span)
                                : 0u;
       if (B.nrows > s.rows) summ_rows = 0;
-      if (summ_rows) slots_summary(stream, view_of(B), B.nrows, s.summ, d_status, summ_rows);
+      if (summ_rows) slots_summary(stream, view_of(B), B.nrows, s.summ, st_dev, summ_rows);
       const uint64_t n = c.nrows;
       // slice loads per round trip: about the widest A range
       const uint32_t batch = A.span <= 2 ? 2u : A.span <= 4 ? 4u : 8u;
@@ -207,7 +209,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
       unsigned long long *op_side = side_ctr + (uint64_t)SLOT_SIDE_PARTS * i;
       const bool chains_done = slots_lane(
           stream, view_of(A), view_of(B), n, B.nrows, tuning.lane_slice_max, tuning.small_slice_max,
-          grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big, ctr, d_status,
+          grid, c.rowmap, c.entries, c.cap, op_side, s.mid, s.big, ctr, st_dev,
           summ_rows ? s.summ : nullptr, tuning.summary_span, summ_rows, o.plus_i ? 1u : 0u,
           slot_parts + 3ull * per_op * i, batch,
           chains ? slot_chains(B.rowmap, B.nrows) : ChainView{nullptr, nullptr},
@@ -221,9 +223,9 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
       const GroupLaunch gm = group_launch(tuning.lanes_small, n), gb = group_launch(64, n);
       slots_groups(stream, view_of(A), view_of(B), gm.lanes, gm.lds_words, gm.grid, s.mid, &ctr->n_mid,
                    gb.lds_words, gb.grid, s.big, &ctr->n_big, gm.rows_per_group, c.rowmap, c.entries,
-                   c.cap, op_side, ctr, d_status, o.plus_i ? 1u : 0u);
+                   c.cap, op_side, ctr, st_dev, o.plus_i ? 1u : 0u);
       if (want_chains && !chains_done)
-        slots_chains(stream, c.rowmap, c.nrows, slot_chains(c.rowmap, c.nrows), d_status);
+        slots_chains(stream, c.rowmap, c.nrows, slot_chains(c.rowmap, c.nrows), st_dev);
     }
     slots_reduce_parts(stream, slot_parts, per_op, used, side_ctr, (uint32_t)nops, d_plan_ctr);
   };
@@ -255,7 +257,7 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     for (uint64_t v : {(uint64_t)a.rowmap, (uint64_t)a.entries, a.nrows, a.ncols, (uint64_t)a.sb,
                        a.n_entries, (uint64_t)s.mid, (uint64_t)s.big, (uint64_t)s.summ, s.rows,
                        (uint64_t)slot_parts, (uint64_t)side_ctr, (uint64_t)d_plan_ctr,
-                       (uint64_t)d_status, (uint64_t)per_op, (uint64_t)num_cu})
+                       (uint64_t)st_dev, (uint64_t)per_op, (uint64_t)num_cu})
       put(v);
     const uint32_t *tw = reinterpret_cast<const uint32_t *>(&tuning);
     for (size_t k = 0; k < sizeof(dv_tuning) / 4; ++k) put(tw[k]);
@@ -289,15 +291,13 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     enqueue_products();
   }
   const PlanSlot &O = p.slots[p.out];
-  slots_scan_rowmap(stream, d.sb, view_of(O), O.nrows, s.cnt, d.rowmap, s.tmp, c_out, d_status);
+  slots_scan_rowmap(stream, d.sb, view_of(O), O.nrows, s.cnt, d.rowmap, s.tmp, c_out, st_dev);
 
   // ---- one readback
   bool ok = hip(hipGetLastError(), "slot plan launch") &&
-            hip(hipMemcpyAsync(h_plan_ctr, d_plan_ctr, (nops + 2) * sizeof(OpCounters),
+            hip(hipMemcpyAsync(h_plan_ctr, d_plan_ctr, (nops + 3) * sizeof(OpCounters),
                                hipMemcpyDeviceToHost, stream),
                 "plan readback") &&
-            hip(hipMemcpyAsync(h_status, d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
-                "plan status") &&
             hip(hipStreamSynchronize(stream), "plan synchronize");
   if (!ok) {
     release_all();
@@ -305,7 +305,8 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     return DV_ERR_DEVICE;
   }
   // plan_mode 2 always takes the fallback (tests)
-  const uint32_t st = *h_status | (tuning.plan_mode == 2 ? 1u : 0u);
+  const uint32_t st = *reinterpret_cast<const uint32_t *>(h_plan_ctr + nops + 2) |
+                      (tuning.plan_mode == 2 ? 1u : 0u);
   // side use per row (all entries, and the fullest partition's share), for
   // the layout choice and the next prediction
   double use = 0, part_use = 0;

@@ -2407,13 +2407,15 @@ __global__ __launch_bounds__(BLOCK) void k_slot_groups(
     uint2 *__restrict__ c_side, SideAlloc sa, OpCounters *__restrict__ ctr,
     unsigned int *__restrict__ status, uint32_t plus_i) {
   extern __shared__ uint64_t lds_all[];
-  if (*status) return;
-  if (blockIdx.x < nb1) {
-    if (*n1 == 0) return;
+  // the status and the block's list length in one round trip (the lists
+  // are usually empty, and then that is the launch's whole work)
+  const bool medium = blockIdx.x < nb1;
+  const unsigned int st = *status, nl = *(medium ? n1 : n2);
+  if (st | (nl == 0u)) return;
+  if (medium) {
     slot_group_rows<G1>(A, B, W1, RPG, rows1, n1, c_slot, c_side, sa, ctr, status, plus_i,
                         lds_all, blockIdx.x, nb1);
   } else {
-    if (*n2 == 0) return;
     slot_group_rows<64>(A, B, W2, RPG, rows2, n2, c_slot, c_side, sa, ctr, status, plus_i,
                         lds_all, blockIdx.x - nb1, nb2);
   }
