@@ -153,17 +153,16 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   const bool timing = prof && plan_events_reserve(nops);
   PartBlocks used{};
   auto enqueue_products = [&]() {
-    fill_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
-             (uint32_t)((nops + 3) * sizeof(OpCounters) / 4), 0u);
-    // per-block parts: each launched block writes its own, so only the side
-    // counters are cleared (all parts too when the operations are too many
-    // to pass their block counts)
+    // the counters and the status, and (one launch) the side counters: the
+    // per-block parts need no clearing, each launched block writes its own
+    // (all parts are cleared too when the operations are too many to pass
+    // their block counts)
     used = PartBlocks{};
     used.all = nops > (int)PART_BLOCKS_MAX_OPS ? 1u : 0u;
-    if (used.all)
-      fill_u32(stream, reinterpret_cast<uint32_t *>(slot_parts), 2 * (parts_words + ctr_words), 0u);
-    else
-      fill_u32(stream, reinterpret_cast<uint32_t *>(side_ctr), 2 * ctr_words, 0u);
+    fill2_u32(stream, reinterpret_cast<uint32_t *>(d_plan_ctr),
+              (uint64_t)(nops + 3) * sizeof(OpCounters) / 4,
+              reinterpret_cast<uint32_t *>(used.all ? slot_parts : side_ctr),
+              used.all ? 2 * (parts_words + ctr_words) : 2 * ctr_words, 0u);
     slots_from_rcrs(stream, a.sb, view(a), in.rowmap, in.entries, in.cap,
                     side_ctr + (uint64_t)SLOT_SIDE_PARTS * nops, st_dev);
     for (int i = 0; i < nops; ++i) {

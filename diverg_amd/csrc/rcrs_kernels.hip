@@ -1657,6 +1657,19 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *__restrict__ p, ui
     p[i] = v;
 }
 
+// Two buffers filled by one launch: index i < n is p[i], the rest q[i - n].
+__global__ __launch_bounds__(BLOCK) void k_fill2_u32(uint32_t *__restrict__ p, uint64_t n,
+                                                     uint32_t *__restrict__ q, uint64_t m,
+                                                     uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n + m;
+       i += (uint64_t)gridDim.x * BLOCK) {
+    if (i < n)
+      p[i] = v;
+    else
+      q[i - n] = v;
+  }
+}
+
 inline uint32_t grid_for(uint64_t n, uint64_t per_block = BLOCK, uint32_t cap = 65536) {
   uint64_t g = (n + per_block - 1) / per_block;
   if (g < 1) g = 1;
@@ -2713,6 +2726,11 @@ __global__ __launch_bounds__(BLOCK) void k_slot_emit(Slots A, uint64_t nrows,
 void fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v) {
   if (!n) return;
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(BLOCK), 0, s, p, n, v);
+}
+
+void fill2_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t *q, uint64_t m, uint32_t v) {
+  if (n + m == 0) return;
+  hipLaunchKernelGGL(k_fill2_u32, dim3(grid_for(n + m)), dim3(BLOCK), 0, s, p, n, q, m, v);
 }
 
 void zero_counters(hipStream_t s, OpCounters *c) {

@@ -51,6 +51,8 @@ struct GroupLaunch {
 };
 
 void fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
+// p[0, n) and q[0, m) set to v in one launch
+void fill2_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t *q, uint64_t m, uint32_t v);
 void zero_counters(hipStream_t s, OpCounters *c);
 
 void identity(hipStream_t s, uint32_t sb, uint64_t n, void *rowmap, uint32_t *entries);
