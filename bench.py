@@ -24,6 +24,11 @@ Further legs (fields of the same JSON line; --legs selects them):
                 checked against the CPU lookup, builds in the mix against
                 plain builds; then the same queries all at once (saturation)
                 through the server and through plain dv_query_batch calls
+  wide          an index of about 2.1 ranges per row (the figure reported for
+                real graphs; the plain workload's index has 1.003): 1.6e6 bp
+                backbone, SNP 0.1/bp, indel 0.05/bp, SV 3e-3/bp (half
+                inversions), [300, 600]; build time in the default layout
+                (slots) and in range CRS, both against the oracle
   batched       24 components (log-normal lengths, median 5e6 bp) built in
                 one planned build over their block-diagonal assembly vs 24
                 plain builds; bit-exact per component
@@ -153,7 +158,7 @@ def main():
     ap.add_argument("--dlo", type=int, default=300)
     ap.add_argument("--dup", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="plain,chromosome,queries,serving,batched",
+    ap.add_argument("--legs", default="plain,chromosome,queries,serving,wide,batched",
                     help="comma-separated legs to run (plain is always run)")
     ap.add_argument("--full-check", action="store_true",
                     help="compare the whole chromosome index and every batched component with "
@@ -366,6 +371,8 @@ def main():
         if "serving" in legs:
             out["serving"] = leg_serving(chrom, result_host, args)
         del chrom
+    if "wide" in legs:
+        out["wide"] = leg_wide(ctx, args.no_cpu)
     if "batched" in legs:
         out["batched"] = leg_batched(ctx, args.full_check)
     ctx.close()
@@ -664,6 +671,65 @@ def leg_serving(chrom, idx2, args):
         f"p99 {out['p99_us']} us, exact {ok}, builds {out['build_completion_s']}; "
         f"saturated {sat_qps:.3g} q/s served vs {n / plain_s:.3g} q/s plain calls")
     return out
+
+
+def leg_wide(ctx, no_cpu=False, steps=5):
+    """The index shape of real graphs (about 2.1 ranges per row): build time
+    in the slot layout (the default) and in range CRS, each the median of
+    `steps` timed builds after one warm-up, both outputs against the oracle."""
+    import diverg_amd as dv
+    from oracle import oracle as orc
+    params = dict(backbone_bp=1_600_000, snp_rate=0.1, indel_rate=0.05, sv_rate=3e-3,
+                  sv_inversion_frac=0.5, sv_deletion_frac=0.3)
+    a = dv.generate_graph(2, size_bytes=4, **params)
+    log(f"wide: {a.nrows} loci, {len(a.entries) // 2} ranges")
+    da = ctx.to_device(a)
+    user = ctx.tuning()["layout"]
+    res = {"workload": "1.6e6 bp backbone, SNP 0.1, indel 0.05, SV 3e-3 (50% inversions, "
+                       "30% deletions), uint32 rowmap, [300,600]",
+           "loci": a.nrows, "ranges_A": len(a.entries) // 2}
+    outs = {}
+    for name, layout in (("slots", 0), ("range_crs", 1)):
+        ctx.set_tuning(layout=layout)
+        for _ in range(2):  # warm-up: the first builds learn capacity predictions
+            d = ctx.create_distance_index(da, 300, 600, location="device")
+            d.free()
+        ms, st, retried = [], None, 0
+        for _ in range(steps):
+            d, st = ctx.create_distance_index(da, 300, 600, location="device", stats=True)
+            ms.append(st["build_ms"])
+            retried += st["retried"]
+            if len(ms) < steps:
+                d.free()
+        outs[name] = d.to_host()
+        d.free()
+        build_ms = float(np.median(ms))
+        res[name] = {"build_s": round(build_ms / 1e3, 5),
+                     "loci_per_s": round(a.nrows / (build_ms / 1e3), 1),
+                     "layout_used": st["layout"], "timed_builds_retried": retried,
+                     "spgemms": st["n_spgemm"],
+                     "alg_GBps": round(st["alg_bytes"] / (build_ms * 1e-3) / 1e9, 2)}
+    ctx.set_tuning(layout=user)
+    del da
+    dh = outs["slots"]
+    res["index_ranges"] = len(dh.entries) // 2
+    res["ranges_per_row"] = round(len(dh.entries) / 2 / max(1, dh.nrows), 3)
+    res["index_nnz"] = dh.nnz
+    ok = {n: None for n in outs}
+    if not no_cpu:
+        orc.set_threads(cpu_threads())
+        t = time.perf_counter()
+        e = orc.dindex(a, 300, 600)
+        res["oracle_s"] = round(time.perf_counter() - t, 2)
+        res["oracle_threads"] = cpu_threads()
+        for n, o in outs.items():
+            ok[n] = bool(o.nnz == e.nnz and np.array_equal(o.rowmap, e.rowmap)
+                         and np.array_equal(o.entries, e.entries))
+    res["bit_exact"] = ok
+    log(f"wide: {res['ranges_per_row']} ranges per row; slots {res['slots']['build_s']}s "
+        f"(layout {res['slots']['layout_used']}, retried {res['slots']['timed_builds_retried']}), "
+        f"range CRS {res['range_crs']['build_s']}s; bit-exact {ok}")
+    return res
 
 
 def leg_batched(ctx, full_check=False):

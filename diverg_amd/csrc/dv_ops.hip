@@ -58,10 +58,17 @@ GroupLaunch Ctx::group_launch(uint32_t lanes, uint64_t nstage) const {
   g.lds_words = std::max<uint32_t>(1, tuning.lds_words * lanes / 64);
   g.stage = tuning.stage_entries;
   g.rows_per_group = std::max<uint32_t>(1, tuning.rows_per_group);
-  // one block per CU: the lists are usually short, and a larger grid costs
-  // more to launch and retire than its blocks save (rows are taken in a
-  // grid-stride loop)
-  g.grid = std::max<uint32_t>(8, (num_cu + 7) / 8 * 8);
+  // four blocks per CU (four waves per SIMD; the group kernels fit eight):
+  // a row's slice gathers are latency-bound, and indexes of several ranges
+  // per row hand tens of thousands of rows to the groups.  Rows are taken in
+  // a grid-stride loop, so an empty or short list costs no more on a large
+  // grid (its blocks exit at once, or add nothing), and the grid is capped
+  // by the rows there could be.  Always a multiple of 8: the kernels' XCD-
+  // aware block order is a permutation of the grid only then.
+  const uint64_t per_block = (uint64_t)(256 / lanes) * g.rows_per_group;  // 256-thread blocks
+  const uint64_t want = (std::max<uint64_t>(1, nstage) + per_block - 1) / per_block;
+  const uint64_t most = (uint64_t)std::max<uint32_t>(8, (num_cu + 7) / 8 * 8) * 4;
+  g.grid = (uint32_t)std::min<uint64_t>((want + 7) / 8 * 8, most);
   g.nstage = nstage;
   return g;
 }
@@ -121,8 +128,10 @@ void Ctx::enq_spgemm_count(const DevMat &a, const DevMat &b, void *c_rowmap, uin
     g.ctr = ctr;
     g.status = status;
     spgemm_group(stream, a.sb, view(a), view(b), g);
+    const GroupLaunch g64 = group_launch(64, n);
     g.lanes = 64;
-    g.lds_words = group_launch(64, n).lds_words;
+    g.lds_words = g64.lds_words;
+    g.grid = g64.grid;
     g.rows = s.big;
     g.n_dev = &ctr->n_big;
     spgemm_group(stream, a.sb, view(a), view(b), g);

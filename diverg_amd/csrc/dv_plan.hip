@@ -569,10 +569,6 @@ int Ctx::plan_run(Plan &p, DevMat &out, const std::function<int(DevMat &)> &exac
     release_all();
     return rc;
   }
-  if (st & 2) {
-    release_all();
-    return fail("entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
-  }
   if (st & 4) {  // a non-canonical input: the exact form
     release_all();
     reset_stats();
@@ -590,6 +586,13 @@ int Ctx::plan_run(Plan &p, DevMat &out, const std::function<int(DevMat &)> &exac
     rc = exact(out);
     if (!rc && tuning.plan_mode != 2) plan_factor = std::max(plan_factor, stats_max_ratio * 1.25);
     return rc;
+  }
+  // a scan's 32-bit check (2) holds only when no product failed: after a
+  // failed one (the cases above, which the exact form checks again) the
+  // counts the later scans read were never written
+  if (st & 2) {
+    release_all();
+    return fail("entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
   }
   stats_planned = 1;
   const PlanSlot &o = p.slots[p.out];

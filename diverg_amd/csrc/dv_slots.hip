@@ -315,11 +315,6 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
       use = std::max(use, (double)h_plan_ctr[i].total / n);
       part_use = std::max(part_use, (double)h_plan_ctr[i].pad * SLOT_SIDE_PARTS / n);
     }
-  if (st & 2) {
-    release_all();
-    release(d);
-    return fail("entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
-  }
   const std::array<uint64_t, 6> key{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup};
   if (st & 12) {
     // the input does not suit the slot form: a row that is not canonical (4;
@@ -358,6 +353,14 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
         slot_side_factor = want;
     }
     return rc;
+  }
+  // the output scan's 32-bit check (2) holds only for a build whose every
+  // product completed: after a failed one (the cases above) the counts it
+  // scanned were never written
+  if (st & 2) {
+    release_all();
+    release(d);
+    return fail("entry count exceeds a 32-bit rowmap", DV_ERR_OVERFLOW);
   }
   // ---- the index's entries
   const uint64_t total = h_plan_ctr[nops + 1].total;

@@ -51,6 +51,24 @@ def test_chromosome_rates_u64_200_1000(ctx, layout):
     assert_identical(d, orc.dindex(a, 200, 1000), "D[200,1000]")
 
 
+@pytest.mark.parametrize("layout", [0, 1])
+def test_wide_index_two_ranges_per_row(ctx, layout):
+    """An index of about two ranges per row (the shape reported for real
+    graphs): SNP 0.1, indel 0.05, SVs at 3e-3 (half inversions) on a 2e5-bp
+    backbone, uint32 rowmap, [300, 600].  Two builds in a row: a slot build
+    that runs out of its predicted side entries is repeated in the exact form
+    and the next one runs with the prediction it learned; each build, in
+    either layout, matches the oracle."""
+    ctx.set_tuning(layout=layout)
+    a = dv.generate_graph(2, size_bytes=4, backbone_bp=200_000, snp_rate=0.1, indel_rate=0.05,
+                          sv_rate=3e-3, sv_inversion_frac=0.5, sv_deletion_frac=0.3)
+    e = orc.dindex(a, 300, 600)
+    assert len(e.entries) / 2 / e.nrows > 1.9
+    for k in range(2):
+        print(f"wide index: {a.nrows} loci, [300,600], layout {layout}, build {k}", flush=True)
+        assert_identical(ctx.create_distance_index(a, 300, 600), e, f"D[300,600] build {k}")
+
+
 @pytest.mark.parametrize("graphs", [0, 1])
 def test_batched_preset4_components(ctx, graphs):
     """Components of the batched workload (preset 4: log-normal lengths,
