@@ -28,7 +28,7 @@ Further legs (fields of the same JSON line; --legs selects them):
                 real graphs; the plain workload's index has 1.003): 1.6e6 bp
                 backbone, SNP 0.1/bp, indel 0.05/bp, SV 3e-3/bp (half
                 inversions), [300, 600]; build time in the default layout
-                (slots) and in range CRS, both against the oracle
+                (slots), in range CRS and autotuned, each against the oracle
   batched       24 components (log-normal lengths, median 5e6 bp) built in
                 one planned build over their block-diagonal assembly vs 24
                 plain builds; bit-exact per component
@@ -675,8 +675,9 @@ def leg_serving(chrom, idx2, args):
 
 def leg_wide(ctx, no_cpu=False, steps=5):
     """The index shape of real graphs (about 2.1 ranges per row): build time
-    in the slot layout (the default) and in range CRS, each the median of
-    `steps` timed builds after one warm-up, both outputs against the oracle."""
+    in the slot layout (the default), in range CRS, and autotuned, each the
+    median of `steps` timed builds after two warm-ups (the autotuned one's
+    first warm-up tunes), every output against the oracle."""
     import diverg_amd as dv
     from oracle import oracle as orc
     params = dict(backbone_bp=1_600_000, snp_rate=0.1, indel_rate=0.05, sv_rate=3e-3,
@@ -689,8 +690,11 @@ def leg_wide(ctx, no_cpu=False, steps=5):
                        "30% deletions), uint32 rowmap, [300,600]",
            "loci": a.nrows, "ranges_A": len(a.entries) // 2}
     outs = {}
-    for name, layout in (("slots", 0), ("range_crs", 1)):
-        ctx.set_tuning(layout=layout)
+    user_autotune = ctx.tuning()["autotune"]
+    for name, layout, tune in (("slots", 0, 0), ("range_crs", 1, 0), ("autotuned", 0, 1)):
+        # autotuned: the first build of the shape times candidate launch
+        # shapes (untimed here, in the warm-up), later builds reuse the winner
+        ctx.set_tuning(layout=layout, autotune=tune)
         for _ in range(2):  # warm-up: the first builds learn capacity predictions
             d = ctx.create_distance_index(da, 300, 600, location="device")
             d.free()
@@ -709,7 +713,13 @@ def leg_wide(ctx, no_cpu=False, steps=5):
                      "layout_used": st["layout"], "timed_builds_retried": retried,
                      "spgemms": st["n_spgemm"],
                      "alg_GBps": round(st["alg_bytes"] / (build_ms * 1e-3) / 1e9, 2)}
-    ctx.set_tuning(layout=user)
+        if tune:
+            lt = ctx.last_tuning()
+            res[name]["tuning"] = {k: lt[k] for k in ("lds_words", "stage_entries",
+                                                      "rows_per_group", "lanes_small",
+                                                      "small_slice_max", "lane_slice_max",
+                                                      "summary_span", "chain_span")}
+    ctx.set_tuning(layout=user, autotune=user_autotune)
     del da
     dh = outs["slots"]
     res["index_ranges"] = len(dh.entries) // 2
@@ -728,7 +738,8 @@ def leg_wide(ctx, no_cpu=False, steps=5):
     res["bit_exact"] = ok
     log(f"wide: {res['ranges_per_row']} ranges per row; slots {res['slots']['build_s']}s "
         f"(layout {res['slots']['layout_used']}, retried {res['slots']['timed_builds_retried']}), "
-        f"range CRS {res['range_crs']['build_s']}s; bit-exact {ok}")
+        f"range CRS {res['range_crs']['build_s']}s, autotuned {res['autotuned']['build_s']}s; "
+        f"bit-exact {ok}")
     return res
 
 
