@@ -28,9 +28,9 @@ int Ctx::identity_dev(uint64_t n, uint32_t sb, DevMat &out) {
   return hip(hipGetLastError(), "identity launch") ? DV_OK : DV_ERR_DEVICE;
 }
 
-bool Ctx::op_scratch(uint64_t n, OpScratch &s) {
+bool Ctx::op_scratch(uint64_t n, OpScratch &s, bool staged) {
   const uint64_t n1 = n ? n : 1;
-  const uint32_t KS = tuning.stage_entries;
+  const uint32_t KS = staged ? tuning.stage_entries : 0u;
   // the lane-tier grid is at most the resident blocks of 256 threads, at
   // most 8 per CU; parts for twice that
   const uint32_t parts_cap = 16 * std::max<uint32_t>(1, num_cu);

@@ -137,7 +137,7 @@ struct Ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t num_cu = 256;
   uint32_t lds_per_block = 65536;
-  dv_tuning tuning{512, 8, 4, 16, 64, 4096, 0, 0, 0, 32, 0, 0, 5};
+  dv_tuning tuning{512, 16, 4, 16, 64, 4096, 0, 0, 0, 32, 0, 0, 5};
   std::string last_error;
 
   // pool accounting
@@ -230,7 +230,8 @@ struct Ctx {
   int clone(const DevMat &src, DevMat &dst);
 
   // dv_ops.hip
-  bool op_scratch(uint64_t n, OpScratch &s);
+  // staged = false: no stage (slot-form builds keep no staged rows)
+  bool op_scratch(uint64_t n, OpScratch &s, bool staged = true);
   GroupLaunch group_launch(uint32_t lanes, uint64_t nstage) const;
   void enq_spadd_count(const DevMat &a, const DevMat &b, void *c_rowmap, uint32_t c_sb,
                        const OpScratch &s, OpCounters *ctr, uint64_t cap, unsigned int *status);

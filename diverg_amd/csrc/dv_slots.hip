@@ -60,7 +60,8 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   uint64_t nmax = a.nrows;
   for (const PlanOp &o : p.ops) nmax = std::max(nmax, p.slots[o.c].nrows);
   OpScratch s;
-  if (!op_scratch(nmax, s)) return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
+  if (!op_scratch(nmax, s, false))
+    return fail("device allocation failed (plan scratch)", DV_ERR_NOMEM);
   const uint32_t per_op = s.parts_cap;  // lane-tier blocks per operation at most
   // per-block partial sums of every operation, then the side-partition
   // counters of every operation and of the input conversion
