@@ -695,8 +695,10 @@ def leg_wide(ctx, no_cpu=False, steps=5):
         # autotuned: the first build of the shape times candidate launch
         # shapes (untimed here, in the warm-up), later builds reuse the winner
         ctx.set_tuning(layout=layout, autotune=tune)
+        first_ms = None
         for _ in range(2):  # warm-up: the first builds learn capacity predictions
-            d = ctx.create_distance_index(da, 300, 600, location="device")
+            d, st = ctx.create_distance_index(da, 300, 600, location="device", stats=True)
+            first_ms = st["build_ms"] if first_ms is None else first_ms
             d.free()
         ms, st, retried = [], None, 0
         for _ in range(steps):
@@ -709,6 +711,8 @@ def leg_wide(ctx, no_cpu=False, steps=5):
         d.free()
         build_ms = float(np.median(ms))
         res[name] = {"build_s": round(build_ms / 1e3, 5),
+                     # the shape's first build (with tuning: the tuner's own build)
+                     "first_build_s": round(first_ms / 1e3, 5),
                      "loci_per_s": round(a.nrows / (build_ms / 1e3), 1),
                      "layout_used": st["layout"], "timed_builds_retried": retried,
                      "spgemms": st["n_spgemm"],

@@ -19,6 +19,15 @@ namespace dv {
 
 namespace {
 SlotView view_of(const PlanSlot &s) { return SlotView{s.rowmap, s.entries}; }
+
+// Rows of several ranges cost the slot form a dependent read each (the side
+// array), where range CRS reads them with the rest of a slice: when they
+// dominate (a product with more side entries than SIDE_SHARE of its rows),
+// builds of that shape use range CRS.  Measured: slot form is faster up to
+// ~0.6 (the 1e5-locus graph of [150, 450]: 0.70 vs 0.97 ms; the chromosome
+// graph, up to 0.43: 29 vs 32 ms) and slower at 0.82 (a graph of 8% SNPs and
+// 2% indels: 1.64 vs 1.40 ms).
+constexpr double SIDE_SHARE = 0.75;
 }  // namespace
 
 bool Ctx::slot_parts_reserve(uint64_t words) {
@@ -340,6 +349,15 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
     release_all();
     release(d);
     plan_retries += 1;
+    if (tuning.layout == 0 && tuning.plan_mode == 0 && use > SIDE_SHARE) {
+      // rows of several ranges dominate, as a completed build would have
+      // found (below): this build and later ones of its shape in range CRS,
+      // without growing the side prediction build by build
+      rcrs_shapes[key] = true;
+      const int rc = dindex_planned(a, dlo, dup, d);
+      stats_retried = 1;
+      return rc;
+    }
     const int rc = exact(d);
     stats_retried = 1;
     if (!rc && tuning.plan_mode != 2) {
@@ -427,14 +445,8 @@ int Ctx::dindex_slots(const DevMat &a, uint32_t dlo, uint32_t dup, DevMat &d) {
   stats_planned = 1;
   stats_layout = 0;
   release_all();
-  // Rows of several ranges cost the slot form a dependent read each (the
-  // side array), where range CRS reads them with the rest of a slice: when
-  // they dominate (a product with more side entries than SIDE_SHARE of its
-  // rows), later builds of this shape use range CRS.  Measured: slot form
-  // is faster up to ~0.6 (the 1e5-locus graph of [150, 450]: 0.70 vs
-  // 0.97 ms; the chromosome graph, up to 0.43: 29 vs 32 ms) and slower at
-  // 0.82 (a graph of 8% SNPs and 2% indels: 1.64 vs 1.40 ms).
-  constexpr double SIDE_SHARE = 0.75;
+  // (SIDE_SHARE: later builds of a shape whose rows of several ranges
+  // dominate use range CRS)
   if (tuning.layout == 0 && use > SIDE_SHARE)
     rcrs_shapes[{a.nrows, a.n_entries, a.nnz, a.sb, dlo, dup}] = true;
   return DV_OK;
