@@ -711,7 +711,7 @@ def leg_wide(ctx, no_cpu=False, steps=5):
         d.free()
         build_ms = float(np.median(ms))
         res[name] = {"build_s": round(build_ms / 1e3, 5),
-                     # the shape's first build (with tuning: the tuner's own build)
+                     # the shape's first build (autotuned: the tuning builds included)
                      "first_build_s": round(first_ms / 1e3, 5),
                      "loci_per_s": round(a.nrows / (build_ms / 1e3), 1),
                      "layout_used": st["layout"], "timed_builds_retried": retried,
