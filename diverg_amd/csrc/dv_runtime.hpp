@@ -137,7 +137,7 @@ struct Ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t num_cu = 256;
   uint32_t lds_per_block = 65536;
-  dv_tuning tuning{512, 16, 4, 16, 64, 4096, 0, 0, 0, 32, 0, 0, 5};
+  dv_tuning tuning{512, 16, 1, 16, 64, 4096, 0, 0, 0, 32, 0, 0, 5};
   std::string last_error;
 
   // pool accounting
