@@ -2545,85 +2545,111 @@ __global__ __launch_bounds__(BLOCK) void k_slot_identity(uint64_t n, uint2 *__re
 }
 
 // C = I + A on slots (range_sparse.hpp:112-357 for canonical rows: the
-// union of A's row and {i}).
+// union of A's row and {i}).  A lane takes PI_ROWS rows per visit (rows
+// k * BLOCK apart, so every load is coalesced): their slots are loaded
+// together, and the side entries of all of them come from one reservation
+// per wave, so a visit costs one round trip for the slots, one for the
+// marked rows' headers and one atomic, for four rows.
+constexpr int PI_ROWS = 4;
+
 __global__ __launch_bounds__(BLOCK) void k_slot_plus_identity(
     Slots A, uint64_t nrows, uint2 *__restrict__ c_slot, uint2 *__restrict__ c_side,
     SideAlloc sa, unsigned int *__restrict__ status, unsigned long long *__restrict__ parts) {
   if (*status) return;
   unsigned long long nnz_acc = 0;
   uint32_t ranges_acc = 0;  // per lane (the block sums are 64-bit)
-  for (uint64_t b0 = (uint64_t)blockIdx.x * BLOCK; b0 < nrows; b0 += (uint64_t)gridDim.x * BLOCK) {
-    const uint64_t row = b0 + threadIdx.x;
-    const bool live = row < nrows;
-    const uint32_t i = (uint32_t)row;
-    uint2 a = make_uint2(SLOT_EMPTY, 0u);
-    if (live) a = A.slot[row];
-    uint32_t na = 0;  // A's ranges (a marked row: from the side array)
-    if (a.x <= a.y) na = 1;
-    else if (a.x == SLOT_MARK) na = A.side[a.y].x;
-    auto arange = [&](uint32_t q) { return na == 1 && a.x <= a.y ? a : A.side[a.y + 1 + q]; };
-    // one range that touches i stays one range; otherwise the merge of
-    // A's ranges with [i, i] goes to the side array (at most na + 1 runs)
-    bool one = false;
-    uint2 u = make_uint2(i, i);
-    if (na == 0) {
-      one = true;
-    } else if (na == 1 && iv_touch(a.x, a.y, i, i)) {
-      one = true;
-      u = make_uint2(min(a.x, i), max(a.y, i));
+  const uint32_t part = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) & (SLOT_SIDE_PARTS - 1);
+  for (uint64_t b0 = (uint64_t)blockIdx.x * BLOCK * PI_ROWS; b0 < nrows;
+       b0 += (uint64_t)gridDim.x * BLOCK * PI_ROWS) {
+    uint2 a[PI_ROWS];
+    uint32_t na[PI_ROWS], need[PI_ROWS];
+    // every load is made (rows past the end read the last row and are then
+    // emptied): a load under a branch waits at the branch's end
+#pragma unroll
+    for (int k = 0; k < PI_ROWS; ++k)
+      a[k] = A.slot[min(b0 + (uint64_t)k * BLOCK + threadIdx.x, nrows - 1)];
+#pragma unroll
+    for (int k = 0; k < PI_ROWS; ++k) {
+      if (b0 + (uint64_t)k * BLOCK + threadIdx.x >= nrows) a[k] = make_uint2(SLOT_EMPTY, 0u);
+      na[k] = a[k].x <= a[k].y ? 1u : 0u;
     }
-    const uint32_t need_side = live && !one ? na + 2 : 0u;
-    const uint32_t part = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) & (SLOT_SIDE_PARTS - 1);
-    const unsigned long long loc = wave_alloc(need_side, &sa.ctr[part]);
-    const unsigned long long off = part * sa.pcap + loc;
-    if (!live) continue;
-    if (one) {
-      c_slot[row] = u;
-      nnz_acc += (unsigned long long)(u.y - u.x) + 1;
-      ranges_acc += 1;
-      continue;
+    // A's ranges of a marked row: from the side array
+#pragma unroll
+    for (int k = 0; k < PI_ROWS; ++k)
+      if (a[k].x == SLOT_MARK) na[k] = A.side[a[k].y].x;
+    // one range that touches i stays one range; otherwise the merge of A's
+    // ranges with [i, i] goes to the side array (at most na + 1 runs and a
+    // header)
+    uint32_t total = 0;
+#pragma unroll
+    for (int k = 0; k < PI_ROWS; ++k) {
+      const uint64_t row = b0 + (uint64_t)k * BLOCK + threadIdx.x;
+      const uint32_t i = (uint32_t)row;
+      const bool one = na[k] == 0 || (na[k] == 1 && iv_touch(a[k].x, a[k].y, i, i));
+      need[k] = row < nrows && !one ? na[k] + 2 : 0u;
+      total += need[k];
     }
-    if (loc + need_side > sa.pcap) {
-      atomicOr(status, 1u);
-      continue;
-    }
-    // merge: i goes in order; touching neighbours fold together
-    uint32_t k = 0;
-    bool placed = false;
-    uint32_t plo = 0, phi = 0;
-    bool have = false;
-    auto put = [&](uint32_t lo, uint32_t hi) {
-      if (have && lo <= phi + 1u) {
-        phi = max(phi, hi);
-      } else {
-        if (have) {
-          c_side[off + 1 + k] = make_uint2(plo, phi);
-          nnz_acc += (unsigned long long)(phi - plo) + 1;
-          ++k;
+    const unsigned long long loc = wave_alloc(total, &sa.ctr[part]);
+    const bool fits = loc + total <= sa.pcap;
+    if (!fits) atomicOr(status, 1u);
+    unsigned long long off = part * sa.pcap + loc;
+#pragma unroll
+    for (int k = 0; k < PI_ROWS; ++k) {
+      const uint64_t row = b0 + (uint64_t)k * BLOCK + threadIdx.x;
+      if (row >= nrows) continue;
+      const uint32_t i = (uint32_t)row;
+      const uint2 x0 = a[k];
+      const uint32_t n0 = na[k];
+      if (need[k] == 0) {
+        const uint2 u = n0 == 0 ? make_uint2(i, i) : make_uint2(min(x0.x, i), max(x0.y, i));
+        c_slot[row] = u;
+        nnz_acc += (unsigned long long)(u.y - u.x) + 1;
+        ranges_acc += 1;
+        continue;
+      }
+      if (!fits) continue;
+      auto arange = [&](uint32_t q) {
+        return n0 == 1 && x0.x <= x0.y ? x0 : A.side[x0.y + 1 + q];
+      };
+      // merge: i goes in order; touching neighbours fold together
+      uint32_t m = 0;
+      bool placed = false;
+      uint32_t plo = 0, phi = 0;
+      bool have = false;
+      auto put = [&](uint32_t lo, uint32_t hi) {
+        if (have && lo <= phi + 1u) {
+          phi = max(phi, hi);
+        } else {
+          if (have) {
+            c_side[off + 1 + m] = make_uint2(plo, phi);
+            nnz_acc += (unsigned long long)(phi - plo) + 1;
+            ++m;
+          }
+          plo = lo;
+          phi = hi;
+          have = true;
         }
-        plo = lo;
-        phi = hi;
-        have = true;
+      };
+      for (uint32_t q = 0; q < n0; ++q) {
+        const uint2 x = arange(q);
+        if (!placed && i < x.x) {
+          put(i, i);
+          placed = true;
+        }
+        put(x.x, x.y);
       }
-    };
-    for (uint32_t q = 0; q < na; ++q) {
-      const uint2 x = arange(q);
-      if (!placed && i < x.x) {
-        put(i, i);
-        placed = true;
+      if (!placed) put(i, i);
+      c_side[off + 1 + m] = make_uint2(plo, phi);
+      nnz_acc += (unsigned long long)(phi - plo) + 1;
+      ++m;
+      ranges_acc += m;
+      if (m == 1) {
+        c_slot[row] = make_uint2(plo, phi);
+      } else {
+        c_side[off] = make_uint2(m, 0u);
+        c_slot[row] = make_uint2(SLOT_MARK, (uint32_t)off);
       }
-      put(x.x, x.y);
-    }
-    if (!placed) put(i, i);
-    c_side[off + 1 + k] = make_uint2(plo, phi);
-    nnz_acc += (unsigned long long)(phi - plo) + 1;
-    ++k;
-    ranges_acc += k;
-    if (k == 1) {
-      c_slot[row] = make_uint2(plo, phi);
-    } else {
-      c_side[off] = make_uint2(k, 0u);
-      c_slot[row] = make_uint2(SLOT_MARK, (uint32_t)off);
+      off += need[k];
     }
   }
   // per-block partial sums (as the lane tier)
@@ -3090,7 +3116,7 @@ void slots_identity(hipStream_t s, uint64_t n, void *slot) {
 }
 
 uint32_t slots_plus_identity_blocks(uint64_t n, uint32_t max_blocks) {
-  return n ? grid_for(n, BLOCK, max_blocks) : 0u;
+  return n ? grid_for(n, (uint64_t)BLOCK * PI_ROWS, max_blocks) : 0u;
 }
 
 void slots_plus_identity(hipStream_t s, SlotView a, uint64_t n, void *c_slot, void *c_side,
