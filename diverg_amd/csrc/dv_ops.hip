@@ -59,8 +59,10 @@ GroupLaunch Ctx::group_launch(uint32_t lanes, uint64_t nstage) const {
   g.stage = tuning.stage_entries;
   g.rows_per_group = std::max<uint32_t>(1, tuning.rows_per_group);
   // four blocks per CU (four waves per SIMD; the group kernels fit eight):
-  // a row's slice gathers are latency-bound, and indexes of several ranges
-  // per row hand tens of thousands of rows to the groups.  Rows are taken in
+  // a row's slice gathers are latency-bound, and an index of two ranges per
+  // row hands its widest products' rows to the groups (the last product's
+  // 64-lane count took 11.0 ms at one block per CU, 5.5 at four, the same
+  // at eight).  Rows are taken in
   // a grid-stride loop, so an empty or short list costs no more on a large
   // grid (its blocks exit at once, or add nothing), and the grid is capped
   // by the rows there could be.  Always a multiple of 8: the kernels' XCD-
